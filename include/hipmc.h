@@ -1,0 +1,152 @@
+/* C ABI of the MI355X transport-and-scoring engine (libhipmc.so).
+ *
+ * The boundary the Python HipEngine binds through ctypes. Plain pointers and sizes;
+ * no torch or HIP types cross it. Every entry returns HMC_OK (0) or a negative status;
+ * hmc_last_error() holds the message. Host arrays passed in are copied; the caller
+ * keeps ownership. Device memory is owned by the context and released by
+ * hmc_destroy(). One context per (device, host thread).
+ *
+ * Each entry names the reference interface it replaces (paths in the pyRadMC tree).
+ */
+#ifndef HIPMC_H
+#define HIPMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HMC_OK = 0,
+  HMC_EINVAL = -1,    /* invalid argument: ValueError on the Python side */
+  HMC_EHIP = -2,      /* HIP runtime failure: RuntimeError */
+  HMC_EOVERFLOW = -3, /* secondary queue overflow: RuntimeError */
+  HMC_EMAJORANT = -4, /* Woodcock majorant violated in-kernel: RuntimeError */
+  HMC_ENOMEM = -5,    /* device allocation failed: MemoryError */
+  HMC_ESTATE = -6     /* call order (e.g. transport before tables): RuntimeError */
+};
+
+typedef struct hmc_ctx hmc_ctx;
+
+/* Flattened cross-section tables, float32, as the reference's CrossSectionTables
+ * fields (pyradmc/data/tables.py:139-219). Per-material arrays are [n_mat][n_points]. */
+typedef struct hmc_tables {
+  int n_points, n_mat, n_coherent;
+  const float *majorant;                 /* [n_points] */
+  const float *mu_compton, *mu_photo, *mu_pair, *mu_rayleigh;
+  const float *csda_range, *moller, *restricted_range, *stopping_radiative;
+  const float *scattering_power, *log_eta, *stopping_restricted;
+  const float *energy_of_restricted_range;
+  const float *coherent_x;               /* [n_coherent] */
+  const float *coherent_cumulative;      /* [n_mat][n_coherent] */
+  double p_log_min, p_inv_dlog, e_log_min, e_inv_dlog, r_log_min, r_inv_dlog;
+} hmc_tables;
+
+/* Source kinds generated in-kernel (pyradmc/backends/warp/kernels.py:1888-2401). */
+enum {
+  HMC_SRC_PENCIL = 0,          /* PencilBeamSource */
+  HMC_SRC_PARALLEL = 1,        /* ParallelBeamSource */
+  HMC_SRC_PARALLEL_SPECTRAL = 2, /* parallel field with spectral energies (extension) */
+  HMC_SRC_SPECTRAL_FAN = 3,    /* SpectralBeamSource / SpectralBeamletSource */
+  HMC_SRC_LATTICE = 4,         /* BeamletGridSource */
+  HMC_SRC_UPLOADED = 5         /* host-sampled primaries */
+};
+
+typedef struct hmc_source {
+  int kind;
+  int particle;        /* 0 photon, 1 electron, 2 positron (mono sources) */
+  double energy;       /* mono energy, MeV */
+  double position[3];  /* pencil start */
+  double direction[3]; /* pencil direction (unit) */
+  double z0, x0, x_extent, y0, y_extent; /* parallel field */
+  double focal[3], u_axis[3], v_axis[3], width_u, width_v; /* divergent fan */
+  int n_bins;          /* spectrum bins */
+  const double *edges; /* [n_bins + 1] */
+  const double *cdf;   /* [n_bins] */
+  int n_beamlets;      /* per-column parameters below */
+  const double *beamlet_params; /* lattice: [n][4] x_lo, x_ext, y_lo, y_ext; fan: [n][3] centre */
+} hmc_source;
+
+typedef struct hmc_physics {
+  double pcut, ecut, step_energy_fraction, deposit_resolution_cm;
+  int transport_electrons, msc_gs, dose_to_water, reserved;
+} hmc_physics;
+
+/* One contiguous run of histories feeding one column: history key of the i-th
+ * primary is key_base + i; column is the group-local Dij column (0 for run()). */
+typedef struct hmc_work_item {
+  int64_t key_base;
+  int64_t count;
+  int32_t column;
+  int32_t reserved;
+} hmc_work_item;
+
+/* Integer books of one hmc_transport call (quanta of 1e-9 MeV) and event counts. */
+typedef struct hmc_ledger {
+  int64_t escaped, unscored, deposited, emitted;
+  int64_t voxel_fetches, deposits, pushes, photon_steps, electron_steps;
+  int64_t violations, trapped, overflow, rounds;
+} hmc_ledger;
+
+/* Lifetime ------------------------------------------------------------------- */
+/* Replaces WarpEngine construction (pyradmc/backends/warp/engine.py:369-407). */
+int hmc_create(int device, hmc_ctx **out);
+void hmc_destroy(hmc_ctx *ctx);
+const char *hmc_last_error(const hmc_ctx *ctx);
+/* Device name, CU count and memory, for auto-sizing (engine.py:175-255). */
+int hmc_device_info(int device, char *name, int name_len, int *cu_count, int64_t *total_bytes,
+                    int64_t *free_bytes);
+
+/* Uploads -------------------------------------------------------------------- */
+/* WarpEngine._upload_grid (engine.py:3336-3366). */
+int hmc_set_grid(hmc_ctx *ctx, int nx, int ny, int nz, const double *origin, const double *spacing,
+                 const float *density, const uint8_t *material);
+/* _scoring_info + dense per-column maps (engine.py:287-330, 1413-1426). */
+int hmc_set_scoring(hmc_ctx *ctx, int nx, int ny, int nz, const double *origin,
+                    const double *spacing, const double *voxel_mass, int64_t n_columns);
+/* WarpEngine._upload_tables (engine.py:3368-3429). */
+int hmc_set_tables(hmc_ctx *ctx, const hmc_tables *tables);
+/* GS grid upload (engine.py:3418-3425, goudsmit_saunderson.py:789-823). */
+int hmc_set_gs(hmc_ctx *ctx, const float *values, int n_eta, int n_theta2, int n_u, int ix0,
+               int iy0);
+/* Generator parameters (engine.py:2833-3006). */
+int hmc_set_source(hmc_ctx *ctx, const hmc_source *source);
+/* Host-sampled primaries for HMC_SRC_UPLOADED (kernels.py:1922-1960): columns of n. */
+int hmc_set_uploaded(hmc_ctx *ctx, int64_t n, const int32_t *particle, const float *energy,
+                     const float *x, const float *y, const float *z, const float *ux,
+                     const float *uy, const float *uz, const float *weight);
+int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
+
+/* Work ----------------------------------------------------------------------- */
+/* Generate and transport every primary of the work items into the dense tallies:
+ * replaces _transport_chunk* + _drain_queues (engine.py:2097-2831). chunk bounds
+ * the histories resident at once; queue_factor sizes the secondary queues. */
+int hmc_transport(hmc_ctx *ctx, uint64_t seed, const hmc_work_item *items, int n_items,
+                  int64_t chunk, int queue_factor, hmc_ledger *ledger);
+/* Fold one batch into the running f64 sums, consuming the map: accumulate_run_batch /
+ * accumulate_dij_batch (kernels.py:804-878). nhist[column] = histories per column in
+ * this batch. total_quanta receives the folded quanta. */
+int hmc_fold(hmc_ctx *ctx, const double *nhist, int64_t *total_quanta);
+/* Mean and sigma per (column, voxel): finalize_run (kernels.py:881-909). With
+ * mean/sigma NULL the result stays on the device for hmc_truncate. */
+int hmc_finalize(hmc_ctx *ctx, int n_batches, double *mean, double *sigma);
+/* Per-column truncation and ordered CSC compaction on the device: column_max,
+ * count_kept_per_chunk, compact_column_chunked (kernels.py:1032-1168) and the
+ * host prefix (engine.py:1496-1557). counts[n_columns] out; entries fetched next. */
+int hmc_truncate(hmc_ctx *ctx, double truncation, int64_t *counts, int64_t *total);
+int hmc_fetch_csc(hmc_ctx *ctx, int64_t *indices, double *dose, double *sigma);
+/* Clear the dense maps and sums for a new group. */
+int hmc_reset_maps(hmc_ctx *ctx);
+
+/* Host-side Goudsmit-Saunderson node grid (build_gs_grid, goudsmit_saunderson.py:
+ * 1108-1219): fills out[(ix1-ix0+1)][(iy1-iy0+1)][n_u] with scaled-deflection tables
+ * at eta = exp(i/bins), <theta^2> = exp(j/bins). n_threads <= 0 uses all cores. */
+int hmc_build_gs_grid(int ix0, int ix1, int iy0, int iy1, int n_u, double bins_per_log,
+                      int n_threads, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPMC_H */

@@ -1,0 +1,68 @@
+"""pyradmc_amd: MI355X-native transport-and-scoring engine for pyRadMC workloads.
+
+``HipEngine`` drops in for pyRadMC's ``WarpEngine``: the same constructor, ``run`` and
+``run_dij`` signatures and result types, backed by hand-written HIP kernels for gfx950
+through the C ABI in ``include/hipmc.h``.
+"""
+
+from __future__ import annotations
+
+from pyradmc_amd.constants import (
+    DIJ_TRUNCATION_RELATIVE,
+    ECUT_MEV,
+    ELECTRON_MASS_MEV,
+    GY_PER_MEV_PER_G,
+    PCUT_MEV,
+    VERSION as __version__,
+)
+from pyradmc_amd.data.analytic import AnalyticCrossSections
+from pyradmc_amd.data.materials import WATER
+from pyradmc_amd.geometry import (
+    BeamletGridSource,
+    ParallelBeamSource,
+    PencilBeamSource,
+    SpectralBeamletSource,
+    SpectralBeamSource,
+    SpectralParallelBeamSource,
+    Spectrum,
+    VoxelGrid,
+    bremsstrahlung_6mv_like,
+)
+from pyradmc_amd.progress import ProgressEvent
+from pyradmc_amd.scoring import DijResult, RunProvenance, ScoringGrid, TransportResult
+
+
+def __getattr__(name: str):
+    # The engine loads the native library; import it lazily so the host-side types
+    # stay importable on a machine without the built library or a GPU.
+    if name == "HipEngine":
+        from pyradmc_amd.engine import HipEngine
+
+        return HipEngine
+    raise AttributeError(name)
+
+
+__all__ = [
+    "AnalyticCrossSections",
+    "BeamletGridSource",
+    "DIJ_TRUNCATION_RELATIVE",
+    "DijResult",
+    "ECUT_MEV",
+    "ELECTRON_MASS_MEV",
+    "GY_PER_MEV_PER_G",
+    "HipEngine",
+    "PCUT_MEV",
+    "ParallelBeamSource",
+    "PencilBeamSource",
+    "ProgressEvent",
+    "RunProvenance",
+    "ScoringGrid",
+    "SpectralBeamSource",
+    "SpectralBeamletSource",
+    "SpectralParallelBeamSource",
+    "Spectrum",
+    "TransportResult",
+    "VoxelGrid",
+    "WATER",
+    "bremsstrahlung_6mv_like",
+]

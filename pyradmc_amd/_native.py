@@ -1,0 +1,257 @@
+"""ctypes binding of ``libhipmc.so`` (declarations in ``include/hipmc.h``).
+
+Loads the in-tree library and fails loudly when it is missing: the engine has no
+CPU fallback. Status codes map onto the exception types the reference engine raises
+(ValueError for bad arguments, RuntimeError for in-kernel failures and call order,
+MemoryError for device allocation).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_uint8, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libhipmc.so"
+
+HMC_OK = 0
+HMC_EINVAL = -1
+HMC_EHIP = -2
+HMC_EOVERFLOW = -3
+HMC_EMAJORANT = -4
+HMC_ENOMEM = -5
+HMC_ESTATE = -6
+
+SRC_PENCIL = 0
+SRC_PARALLEL = 1
+SRC_PARALLEL_SPECTRAL = 2
+SRC_SPECTRAL_FAN = 3
+SRC_LATTICE = 4
+SRC_UPLOADED = 5
+
+_f32p = POINTER(ctypes.c_float)
+_f64p = POINTER(c_double)
+
+
+class Tables(Structure):
+    _fields_ = [
+        ("n_points", c_int),
+        ("n_mat", c_int),
+        ("n_coherent", c_int),
+        ("majorant", _f32p),
+        ("mu_compton", _f32p),
+        ("mu_photo", _f32p),
+        ("mu_pair", _f32p),
+        ("mu_rayleigh", _f32p),
+        ("csda_range", _f32p),
+        ("moller", _f32p),
+        ("restricted_range", _f32p),
+        ("stopping_radiative", _f32p),
+        ("scattering_power", _f32p),
+        ("log_eta", _f32p),
+        ("stopping_restricted", _f32p),
+        ("energy_of_restricted_range", _f32p),
+        ("coherent_x", _f32p),
+        ("coherent_cumulative", _f32p),
+        ("p_log_min", c_double),
+        ("p_inv_dlog", c_double),
+        ("e_log_min", c_double),
+        ("e_inv_dlog", c_double),
+        ("r_log_min", c_double),
+        ("r_inv_dlog", c_double),
+    ]
+
+
+class SourceDesc(Structure):
+    _fields_ = [
+        ("kind", c_int),
+        ("particle", c_int),
+        ("energy", c_double),
+        ("position", c_double * 3),
+        ("direction", c_double * 3),
+        ("z0", c_double),
+        ("x0", c_double),
+        ("x_extent", c_double),
+        ("y0", c_double),
+        ("y_extent", c_double),
+        ("focal", c_double * 3),
+        ("u_axis", c_double * 3),
+        ("v_axis", c_double * 3),
+        ("width_u", c_double),
+        ("width_v", c_double),
+        ("n_bins", c_int),
+        ("edges", _f64p),
+        ("cdf", _f64p),
+        ("n_beamlets", c_int),
+        ("beamlet_params", _f64p),
+    ]
+
+
+class Physics(Structure):
+    _fields_ = [
+        ("pcut", c_double),
+        ("ecut", c_double),
+        ("step_energy_fraction", c_double),
+        ("deposit_resolution_cm", c_double),
+        ("transport_electrons", c_int),
+        ("msc_gs", c_int),
+        ("dose_to_water", c_int),
+        ("reserved", c_int),
+    ]
+
+
+class WorkItem(Structure):
+    _fields_ = [("key_base", c_int64), ("count", c_int64), ("column", c_int32), ("reserved", c_int32)]
+
+
+class Ledger(Structure):
+    _fields_ = [
+        (name, c_int64)
+        for name in (
+            "escaped",
+            "unscored",
+            "deposited",
+            "emitted",
+            "voxel_fetches",
+            "deposits",
+            "pushes",
+            "photon_steps",
+            "electron_steps",
+            "violations",
+            "trapped",
+            "overflow",
+            "rounds",
+        )
+    ]
+
+    def as_dict(self) -> dict[str, int]:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load ``libhipmc.so`` once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.is_file():
+        raise RuntimeError(
+            f"native library {LIB_PATH} is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (or pyradmc_amd.build.build())"
+        )
+    h = ctypes.CDLL(str(LIB_PATH))
+    ctx_p = c_void_p
+    sig = {
+        "hmc_create": (c_int, [c_int, POINTER(ctx_p)]),
+        "hmc_destroy": (None, [ctx_p]),
+        "hmc_last_error": (c_char_p, [ctx_p]),
+        "hmc_device_info": (c_int, [c_int, c_char_p, c_int, POINTER(c_int), POINTER(c_int64), POINTER(c_int64)]),
+        "hmc_set_grid": (c_int, [ctx_p, c_int, c_int, c_int, _f64p, _f64p, _f32p, POINTER(c_uint8)]),
+        "hmc_set_scoring": (c_int, [ctx_p, c_int, c_int, c_int, _f64p, _f64p, _f64p, c_int64]),
+        "hmc_set_tables": (c_int, [ctx_p, POINTER(Tables)]),
+        "hmc_set_gs": (c_int, [ctx_p, _f32p, c_int, c_int, c_int, c_int, c_int]),
+        "hmc_set_source": (c_int, [ctx_p, POINTER(SourceDesc)]),
+        "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
+        "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
+        "hmc_transport": (c_int, [ctx_p, c_uint64, POINTER(WorkItem), c_int, c_int64, c_int, POINTER(Ledger)]),
+        "hmc_fold": (c_int, [ctx_p, _f64p, POINTER(c_int64)]),
+        "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),
+        "hmc_truncate": (c_int, [ctx_p, c_double, POINTER(c_int64), POINTER(c_int64)]),
+        "hmc_fetch_csc": (c_int, [ctx_p, POINTER(c_int64), _f64p, _f64p]),
+        "hmc_reset_maps": (c_int, [ctx_p]),
+        "hmc_build_gs_grid": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, _f64p]),
+    }
+    for name, (restype, argtypes) in sig.items():
+        fn = getattr(h, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = h
+    return h
+
+
+def exported_symbols() -> list[str]:
+    """Every ``hmc_*`` entry point the header declares."""
+    header = Path(__file__).resolve().parent.parent / "include" / "hipmc.h"
+    import re
+
+    return sorted(set(re.findall(r"\b(hmc_[a-z_]+)\s*\(", header.read_text())))
+
+
+def ptr(array: np.ndarray, ctype):
+    return array.ctypes.data_as(POINTER(ctype))
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    if rc == HMC_OK:
+        return
+    message = ""
+    if ctx is not None:
+        raw = lib().hmc_last_error(ctx)
+        message = raw.decode() if raw else ""
+    text = f"{what}: {message}" if what else message
+    if rc == HMC_EINVAL:
+        raise ValueError(text)
+    if rc == HMC_ENOMEM:
+        raise MemoryError(text)
+    raise RuntimeError(text or f"native status {rc}")
+
+
+class Context:
+    """Owns one ``hmc_ctx`` on one device."""
+
+    def __init__(self, device: int = 0) -> None:
+        self._lib = lib()
+        handle = c_void_p()
+        rc = self._lib.hmc_create(int(device), ctypes.byref(handle))
+        if rc != HMC_OK:
+            raise RuntimeError(
+                f"hmc_create(device={device}) failed with status {rc}: no usable HIP device"
+            )
+        self.handle = handle
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.hmc_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name: str, *args) -> None:
+        rc = getattr(self._lib, name)(self.handle, *args)
+        check(rc, self.handle, name)
+
+
+def device_info(device: int = 0) -> dict:
+    h = lib()
+    name = ctypes.create_string_buffer(256)
+    cu = c_int()
+    total = c_int64()
+    free = c_int64()
+    rc = h.hmc_device_info(int(device), name, 256, ctypes.byref(cu), ctypes.byref(total), ctypes.byref(free))
+    check(rc, None, "hmc_device_info")
+    return {
+        "name": name.value.decode(),
+        "cu_count": int(cu.value),
+        "total_bytes": int(total.value),
+        "free_bytes": int(free.value),
+    }
+
+
+def build_gs_grid_native(ix0: int, ix1: int, iy0: int, iy1: int, n_u: int, bins: float, threads: int = 0) -> np.ndarray:
+    out = np.empty((ix1 - ix0 + 1, iy1 - iy0 + 1, n_u), dtype=np.float64)
+    rc = lib().hmc_build_gs_grid(int(ix0), int(ix1), int(iy0), int(iy1), int(n_u), float(bins), int(threads), ptr(out, c_double))
+    check(rc, None, "hmc_build_gs_grid")
+    return out

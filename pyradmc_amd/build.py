@@ -1,0 +1,82 @@
+"""Compile the native library ``libhipmc.so`` in-tree for gfx950.
+
+``hipmc_kernels.hip`` (device kernels and the C ABI) is compiled by hipcc for
+``--offload-arch=gfx950``; ``gs_build.cpp`` (host-only GS grid builder) by g++; both
+link into one shared library beside this file, which travels with the source tree.
+Packed-FP32 instruction selection is disabled for the device code.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+ROOT = HERE.parent
+LIB_NAME = "libhipmc.so"
+LIB_PATH = HERE / LIB_NAME
+ARCH = os.environ.get("PYRADMC_AMD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    f"--offload-arch={ARCH}",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode != 0:
+        sys.stderr.write(result.stdout + result.stderr)
+        raise RuntimeError(f"build step failed ({result.returncode}): {' '.join(cmd[:3])} ...")
+    if verbose and (result.stdout or result.stderr):
+        # hipcc's host pass warns about the device-only packed-fp32 feature; harmless.
+        text = "\n".join(
+            line for line in (result.stdout + result.stderr).splitlines()
+            if "packed-fp32-ops" not in line
+        )
+        if text.strip():
+            print(text)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every native source and link ``libhipmc.so``; returns its path."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    build_dir = HERE / "_build"
+    build_dir.mkdir(exist_ok=True)
+    kernels_o = build_dir / "hipmc_kernels.o"
+    gs_o = build_dir / "gs_build.o"
+    _run(
+        [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
+         str(CSRC / "hipmc_kernels.hip"), "-o", str(kernels_o)],
+        verbose,
+    )
+    _run(
+        ["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-I", str(ROOT / "include"), "-c",
+         str(CSRC / "gs_build.cpp"), "-o", str(gs_o)],
+        verbose,
+    )
+    tmp = HERE / (LIB_NAME + ".tmp")
+    _run(
+        [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", str(kernels_o),
+         str(gs_o), "-o", str(tmp)],
+        verbose,
+    )
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

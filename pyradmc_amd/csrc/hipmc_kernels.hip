@@ -1,0 +1,1588 @@
+// Transport, tally, fold and truncation kernels of the MI355X engine, and the
+// C ABI that drives them (include/hipmc.h).
+//
+// Layout in HBM:
+//   voxel maps      density f32 [nx][ny][nz], material u8 [nx][ny][nz]
+//   tables          float4 rows per material: photon (compton, photo, pair, rayleigh),
+//                   electron A (csda, moller, restricted range, radiative),
+//                   electron B (scattering power, log eta, restricted stopping, 0)
+//   GS grid         f32 [n_eta][n_theta2][512]
+//   particles       48-byte records (3 x 16 B) in four ping-pong queues
+//   tallies         int64 quanta [column][scoring voxel]; f64 sums s1, s2 alike
+//
+// Transport runs in rounds like the reference's queue ping-pong
+// (pyradmc/backends/warp/engine.py:2732-2831): a photon round consumes the photon
+// queue, an electron round the electron queue, each pushing secondaries into the
+// other or the next round's queue. Within a round every lane is persistent and
+// refills itself: it transports its particle one Woodcock flight or one condensed
+// substep per loop iteration and pops the next particle the moment its own ends,
+// so a wave does not idle behind its longest history. Ledger sums are kept per
+// lane in registers and flushed once per lane.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hipmc.h"
+#include "hipmc_device.h"
+
+using namespace hmc;
+
+namespace {
+
+// ------------------------------------------------------------------- particles
+
+struct __align__(16) Particle {
+  float x, y, z, e;
+  float ux, uy, uz, w;
+  uint64_t rng;
+  uint32_t meta;  // kind (2 bits) | primary (1 bit) << 2 | column << 3
+  uint32_t pad;
+};
+static_assert(sizeof(Particle) == 48, "particle record must stay 48 bytes");
+
+__host__ __device__ inline uint32_t pack_meta(uint32_t kind, uint32_t primary, uint32_t column) {
+  return kind | (primary << 2) | (column << 3);
+}
+
+struct Queue {
+  Particle* data;
+  unsigned int* count;
+  unsigned int capacity;
+};
+
+enum LedgerSlot {
+  L_ESCAPED = 0,
+  L_UNSCORED,
+  L_DEPOSITED,
+  L_EMITTED,
+  L_VOX,
+  L_DEP,
+  L_PUSH,
+  L_PSTEPS,
+  L_ESTEPS,
+  L_VIOLATIONS,
+  L_TRAPPED,
+  L_COUNT
+};
+
+struct TransportParams {
+  DevGrid g;
+  DevScoring s;
+  DevTables t;
+  long long* edep;
+  unsigned long long* ledger;
+  float pcut, ecut, step_fraction, resolution;
+  int transport_electrons, msc_gs, dose_to_water, max_steps;
+};
+
+struct LaneTally {
+  long long escaped = 0, unscored = 0, deposited = 0;
+  unsigned long long vox = 0, dep = 0, push = 0, steps = 0;
+  unsigned long long violations = 0, trapped = 0;
+
+  __device__ void flush(unsigned long long* ledger, int step_slot) const {
+    if (escaped) atomicAdd(ledger + L_ESCAPED, static_cast<unsigned long long>(escaped));
+    if (unscored) atomicAdd(ledger + L_UNSCORED, static_cast<unsigned long long>(unscored));
+    if (deposited) atomicAdd(ledger + L_DEPOSITED, static_cast<unsigned long long>(deposited));
+    if (vox) atomicAdd(ledger + L_VOX, vox);
+    if (dep) atomicAdd(ledger + L_DEP, dep);
+    if (push) atomicAdd(ledger + L_PUSH, push);
+    if (steps) atomicAdd(ledger + step_slot, steps);
+    if (violations) atomicAdd(ledger + L_VIOLATIONS, violations);
+    if (trapped) atomicAdd(ledger + L_TRAPPED, trapped);
+  }
+};
+
+__device__ inline void push(const Queue& q, LaneTally& tal, uint32_t kind, uint32_t column,
+                            float e, float w, float x, float y, float z, float ux, float uy,
+                            float uz, uint64_t rng) {
+  const unsigned int idx = atomicAdd(q.count, 1u);
+  tal.push++;
+  if (idx < q.capacity) {
+    Particle p;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.e = e;
+    p.ux = ux;
+    p.uy = uy;
+    p.uz = uz;
+    p.w = w;
+    p.rng = rng;
+    p.meta = pack_meta(kind, 0u, column);
+    p.pad = 0u;
+    q.data[idx] = p;
+  }
+}
+
+// Fixed-point deposit at a position inside the transport grid, routed into the
+// scoring grid or the unscored book (pyradmc/backends/warp/kernels.py:535-584).
+__device__ inline void deposit(const TransportParams& P, LaneTally& tal, long long base, float x,
+                               float y, float z, float amount, float factor) {
+  const long long idx = scoring_index(P.s, x, y, z);
+  if (idx >= 0) {
+    const long long q = static_cast<long long>(static_cast<double>(amount) *
+                                                   static_cast<double>(factor) * kInvQuantum +
+                                               0.5);
+    if (q != 0) atomicAdd(reinterpret_cast<unsigned long long*>(P.edep + base + idx),
+                          static_cast<unsigned long long>(q));
+    tal.dep++;
+    if (P.dose_to_water)
+      tal.deposited += static_cast<long long>(static_cast<double>(amount) * kInvQuantum + 0.5);
+  } else {
+    tal.unscored += static_cast<long long>(static_cast<double>(amount) * kInvQuantum + 0.5);
+  }
+}
+
+__device__ inline void escape(LaneTally& tal, float amount) {
+  tal.escaped += quanta_round(static_cast<double>(amount));
+}
+
+// Deposit if inside the transport grid, else escaped (kernels.py:600-633).
+__device__ inline void deposit_or_escape(const TransportParams& P, LaneTally& tal, long long base,
+                                         float x, float y, float z, float amount, float factor) {
+  if (amount <= 0.0f) return;
+  if (inside_grid(P.g, x, y, z)) deposit(P, tal, base, x, y, z, amount, factor);
+  else escape(tal, amount);
+}
+
+// One half-substep's continuous loss, in equal pieces no longer than the
+// resolution (pyradmc/transport/electron.py:169-191, 361-380).
+__device__ inline void deposit_spread(const TransportParams& P, LaneTally& tal, long long base,
+                                      float x, float y, float z, float ux, float uy, float uz,
+                                      float length, float amount, float factor) {
+  int n = 1;
+  if (P.resolution > 0.0f) {
+    n = static_cast<int>(ceilf(length / P.resolution));
+    n = min(max(n, 1), kSubstepDepositCap);
+  }
+  const float share = amount / static_cast<float>(n);
+  const float piece = length / static_cast<float>(n);
+  for (int k = 0; k < n; ++k) {
+    const float t = (static_cast<float>(k) + 0.5f) * piece;
+    deposit_or_escape(P, tal, base, x + ux * t, y + uy * t, z + uz * t, share, factor);
+  }
+}
+
+// Positron annihilation at rest: two back-to-back 511 keV photons, isotropic
+// (pyradmc/transport/particles.py:75-102).
+__device__ inline void annihilate(const TransportParams& P, LaneTally& tal, const Queue& qp,
+                                  long long base, uint32_t column, float x, float y, float z,
+                                  float w, float factor, uint64_t& rng) {
+  if (P.pcut >= kElectronMass) {
+    deposit(P, tal, base, x, y, z, w * 2.0f * kElectronMass, factor);
+    return;
+  }
+  const float c = 1.0f - 2.0f * uniform(rng);
+  const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+  const float phi = kTwoPi * uniform(rng);
+  float sp, cp;
+  __sincosf(phi, &sp, &cp);
+  const float ax = s * cp, ay = s * sp, az = c;
+  const uint64_t r1 = spawn_stream(rng);
+  push(qp, tal, PHOTON, column, kElectronMass, w, x, y, z, ax, ay, az, r1);
+  const uint64_t r2 = spawn_stream(rng);
+  push(qp, tal, PHOTON, column, kElectronMass, w, x, y, z, -ax, -ay, -az, r2);
+}
+
+__device__ inline void load_particle(const Particle* data, unsigned int i, float& x, float& y,
+                                     float& z, float& e, float& ux, float& uy, float& uz,
+                                     float& w, uint64_t& rng, uint32_t& meta) {
+  const float4* p = reinterpret_cast<const float4*>(data + i);
+  const float4 a = p[0];
+  const float4 b = p[1];
+  const uint4 c = reinterpret_cast<const uint4*>(p)[2];
+  x = a.x;
+  y = a.y;
+  z = a.z;
+  e = a.w;
+  ux = b.x;
+  uy = b.y;
+  uz = b.z;
+  w = b.w;
+  rng = static_cast<uint64_t>(c.x) | (static_cast<uint64_t>(c.y) << 32);
+  meta = c.z;
+}
+
+// Vacuum flight to the grid for a particle born outside; false when it misses
+// (pyradmc/backends/warp/kernels.py:1214-1229).
+__device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z, float ux,
+                                  float uy, float uz) {
+  if (inside_grid(g, x, y, z)) return true;
+  float t = slab_entry(g, x, y, z, ux, uy, uz);
+  if (t >= kInfinityCm) return false;
+  t += kEntryNudge;
+  x += t * ux;
+  y += t * uy;
+  z += t * uz;
+  return inside_grid(g, x, y, z);
+}
+
+// ----------------------------------------------------------------- photon round
+//
+// Woodcock flight on the majorant, real/virtual test, channel selection; Compton
+// continues in-lane, photo and pair hand electrons to the electron queue
+// (pyradmc/transport/photon.py:161-282, kernels.py:1171-1529).
+__global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue qin,
+                                                      unsigned int* head, Queue qe, Queue qp) {
+  const unsigned int n_in = min(*qin.count, qin.capacity);
+  LaneTally tal;
+  bool active = false;
+  float x = 0, y = 0, z = 0, e = 0, ux = 0, uy = 0, uz = 1, w = 0;
+  uint64_t rng = 0;
+  uint32_t column = 0;
+  long long base = 0;
+  float mu_maj = 0.0f;
+  int pi = 0;
+  float pw = 0.0f;
+  int steps = 0;
+  const int n = P.t.n_points;
+
+  while (true) {
+    if (!active) {
+      const unsigned int i = atomicAdd(head, 1u);
+      if (i >= n_in) break;
+      uint32_t meta;
+      load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      column = meta >> 3;
+      base = static_cast<long long>(column) * P.s.n_voxels;
+      if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
+        escape(tal, w * e);
+        continue;
+      }
+      active = true;
+      steps = 0;
+      grid_locate(__logf(e), P.t.p_log_min, P.t.p_inv_dlog, n, pi, pw);
+      mu_maj = lerp1(P.t.majorant, pi, pw);
+    }
+    tal.steps++;
+    if (++steps > P.max_steps) {
+      tal.trapped++;
+      escape(tal, w * e);
+      active = false;
+      continue;
+    }
+
+    const float step = sample_path(mu_maj, rng);
+    x += step * ux;
+    y += step * uy;
+    z += step * uz;
+    if (!inside_grid(P.g, x, y, z)) {
+      escape(tal, w * e);
+      active = false;
+      continue;
+    }
+    const int ix = axis_index(x, P.g.x_lo, P.g.inv_sx, P.g.nx);
+    const int iy = axis_index(y, P.g.y_lo, P.g.inv_sy, P.g.ny);
+    const int iz = axis_index(z, P.g.z_lo, P.g.inv_sz, P.g.nz);
+    const long long vox = (static_cast<long long>(ix) * P.g.ny + iy) * P.g.nz + iz;
+    const float rho = P.g.density[vox];
+    const int mat = P.g.material[vox];
+    tal.vox++;
+    const float4 c = lerp4(P.t.photon + static_cast<long long>(mat) * n, pi, pw);
+    const float mu_c = rho * c.x;
+    const float mu_p = rho * c.y;
+    const float mu_pp = rho * c.z;
+    const float mu_r = rho * c.w;
+    const float mu_real = mu_c + mu_p + mu_pp + mu_r;
+    if (mu_real > mu_maj) {
+      tal.violations++;
+      escape(tal, w * e);
+      active = false;
+      continue;
+    }
+    if (uniform(rng) * mu_maj >= mu_real) continue;  // delta scattering
+
+    const float threshold = uniform(rng) * (mu_c + mu_p + mu_pp + mu_r);
+    int process;
+    if (threshold < mu_c) process = COMPTON;
+    else if (threshold < mu_c + mu_p) process = PHOTOELECTRIC;
+    else if (threshold < mu_c + mu_p + mu_pp) process = PAIR;
+    else process = RAYLEIGH;
+
+    if (process == RAYLEIGH) {
+      const float cos_r = sample_coherent(P.t, mat, e, rng);
+      const float phi = kTwoPi * uniform(rng);
+      rotate(ux, uy, uz, cos_r, phi);
+    } else if (process == COMPTON) {
+      const float ratio = sample_compton_ratio(e, rng);
+      const float recoil = e * (1.0f - ratio);
+      const float phi = kTwoPi * uniform(rng);
+      if (P.transport_electrons && recoil > P.ecut) {
+        const float ce = compton_electron_cos(e, ratio);
+        float ex = ux, ey = uy, ez = uz;
+        rotate(ex, ey, ez, ce, phi + kPi);
+        const uint64_t child = spawn_stream(rng);
+        push(qe, tal, ELECTRON, column, recoil, w, x, y, z, ex, ey, ez, child);
+      } else {
+        deposit(P, tal, base, x, y, z, w * recoil,
+                spr_factor(P.t, P.dose_to_water, recoil, mat, P.ecut));
+      }
+      const float cg = compton_cos(e, ratio);
+      rotate(ux, uy, uz, cg, phi);
+      e *= ratio;
+      if (e <= P.pcut) {
+        deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+        active = false;
+        continue;
+      }
+      if (e < kRouletteMeV && w < kRouletteWeightCap) {
+        const float new_w = uniform(rng) < kRouletteSurvival ? w / kRouletteSurvival : 0.0f;
+        escape(tal, (w - new_w) * e);
+        if (new_w == 0.0f) {
+          active = false;
+          continue;
+        }
+        w = new_w;
+      }
+      grid_locate(__logf(e), P.t.p_log_min, P.t.p_inv_dlog, n, pi, pw);
+      mu_maj = lerp1(P.t.majorant, pi, pw);
+    } else if (process == PHOTOELECTRIC) {
+      if (P.transport_electrons && e > P.ecut) {
+        const uint64_t child = spawn_stream(rng);
+        push(qe, tal, ELECTRON, column, e, w, x, y, z, ux, uy, uz, child);
+      } else {
+        deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+      }
+      active = false;
+    } else {  // pair production
+      const float kinetic = e - 2.0f * kElectronMass;
+      const float f_ann = spr_factor(P.t, P.dose_to_water, kElectronMass, mat, P.ecut);
+      if (P.transport_electrons) {
+        const float fraction = uniform(rng);
+        const float share_e = fraction * kinetic;
+        const float share_p = (1.0f - fraction) * kinetic;
+        if (share_e > P.ecut) {
+          const uint64_t child = spawn_stream(rng);
+          push(qe, tal, ELECTRON, column, share_e, w, x, y, z, ux, uy, uz, child);
+        } else {
+          deposit(P, tal, base, x, y, z, w * share_e,
+                  spr_factor(P.t, P.dose_to_water, share_e, mat, P.ecut));
+        }
+        if (share_p > P.ecut) {
+          const uint64_t child = spawn_stream(rng);
+          push(qe, tal, POSITRON, column, share_p, w, x, y, z, ux, uy, uz, child);
+        } else {
+          deposit(P, tal, base, x, y, z, w * share_p,
+                  spr_factor(P.t, P.dose_to_water, share_p, mat, P.ecut));
+          annihilate(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
+        }
+      } else {
+        deposit(P, tal, base, x, y, z, w * kinetic,
+                spr_factor(P.t, P.dose_to_water, kinetic, mat, P.ecut));
+        annihilate(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
+      }
+      active = false;
+    }
+  }
+  tal.flush(P.ledger, L_PSTEPS);
+}
+
+// --------------------------------------------------------------- electron round
+//
+// Class-II condensed-history substep: range/boundary/Moller-limited step, exact
+// CSDA loss through the restricted range and its inverse, Bernoulli brems, random
+// hinge with GS (or Gaussian) deflection, two half-step deposits, Moller delta
+// (pyradmc/transport/electron.py:292-481, kernels.py:1532-1885).
+__global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue qin,
+                                                        unsigned int* head, Queue qp, Queue qe) {
+  const unsigned int n_in = min(*qin.count, qin.capacity);
+  LaneTally tal;
+  bool active = false;
+  float x = 0, y = 0, z = 0, e = 0, ux = 0, uy = 0, uz = 1, w = 0;
+  uint64_t rng = 0;
+  uint32_t column = 0;
+  long long base = 0;
+  bool positron = false;
+  int steps = 0;
+  const int n = P.t.n_points;
+
+  while (true) {
+    if (!active) {
+      const unsigned int i = atomicAdd(head, 1u);
+      if (i >= n_in) break;
+      uint32_t meta;
+      load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      column = meta >> 3;
+      positron = (meta & 3u) == POSITRON;
+      base = static_cast<long long>(column) * P.s.n_voxels;
+      const float latent = positron ? 2.0f * kElectronMass : 0.0f;
+      if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
+        escape(tal, w * (e + latent));
+        continue;
+      }
+      active = true;
+      steps = 0;
+    }
+    tal.steps++;
+    const float latent = positron ? 2.0f * kElectronMass : 0.0f;
+    if (++steps > P.max_steps) {
+      tal.trapped++;
+      escape(tal, w * (e + latent));
+      active = false;
+      continue;
+    }
+
+    const int ix = axis_index(x, P.g.x_lo, P.g.inv_sx, P.g.nx);
+    const int iy = axis_index(y, P.g.y_lo, P.g.inv_sy, P.g.ny);
+    const int iz = axis_index(z, P.g.z_lo, P.g.inv_sz, P.g.nz);
+    const long long vox = (static_cast<long long>(ix) * P.g.ny + iy) * P.g.nz + iz;
+    const int mat = P.g.material[vox];
+    tal.vox++;
+
+    if (e <= P.ecut) {
+      // Terminal: local deposit, annihilation for a positron.
+      deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+      if (positron)
+        annihilate(P, tal, qp, base, column, x, y, z, w,
+                   spr_factor(P.t, P.dose_to_water, kElectronMass, mat, P.ecut), rng);
+      active = false;
+      continue;
+    }
+    const float rho = P.g.density[vox];
+    const long long row = static_cast<long long>(mat) * n;
+
+    int ei;
+    float ew;
+    grid_locate(__logf(e), P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
+    const float4 ea = lerp4(P.t.elec_a + row, ei, ew);  // csda, moller, r_range, radiative
+
+    const float range_cm = ea.x / rho;
+    float s_max = P.step_fraction * range_cm;
+    if (!P.msc_gs) {
+      const float tp = lerp4(P.t.elec_b + row, ei, ew).x;
+      s_max = fminf(s_max, kHingeTheta2Max / (tp * rho));
+    }
+    const float s_boundary = boundary_distance(P.g, x, y, z, ux, uy, uz) + kBoundaryNudge;
+    const float s_geometry = fminf(s_max, s_boundary);
+    const float sigma_moller = rho * ea.y;
+    float s = s_geometry;
+    bool moller_pending = false;
+    if (sigma_moller > 0.0f) {
+      const float s_int = sample_path(sigma_moller, rng);
+      if (s_int <= s_geometry) {
+        s = s_int;
+        moller_pending = true;
+      }
+    }
+
+    const float available = ea.z;
+    float de;
+    if (rho * s >= available) {
+      s = available / rho;
+      de = e - P.ecut;
+      moller_pending = false;
+    } else {
+      int ri;
+      float rw;
+      grid_locate(__logf(available - rho * s), P.t.r_log_min, P.t.r_inv_dlog, n, ri, rw);
+      float e_end = lerp1(P.t.inv_range + row, ri, rw);
+      e_end = fminf(fmaxf(e_end, P.ecut), e);
+      de = e - e_end;
+    }
+    // Bernoulli bremsstrahlung matched to the radiative stopping power
+    // (pyradmc/physics/brems.py:30-66).
+    const float radiative_loss = ea.w * rho * s;
+    float emit_probability = 0.0f;
+    float local_brems = radiative_loss;
+    if (e > P.pcut) {
+      const float emitted_fraction = 1.0f - P.pcut / e;
+      const float mean_k = (e - P.pcut) / __logf(e / P.pcut);
+      emit_probability = radiative_loss * emitted_fraction / mean_k;
+      local_brems = radiative_loss * (1.0f - emitted_fraction);
+    }
+    const float continuous = de + local_brems;
+    const bool emit_brems = uniform(rng) < emit_probability;
+    const float factor = spr_factor(P.t, P.dose_to_water, e, mat, P.ecut);
+
+    const float s1 = uniform(rng) * s;
+    const float s2 = s - s1;
+    const float d1 = s > 0.0f ? continuous * (s1 / s) : 0.0f;
+    const float d2 = continuous - d1;
+
+    deposit_spread(P, tal, base, x, y, z, ux, uy, uz, s1, w * d1, factor);
+    e -= d1;
+    x += ux * s1;
+    y += uy * s1;
+    z += uz * s1;
+    if (!inside_grid(P.g, x, y, z)) {
+      escape(tal, w * (e + latent));
+      active = false;
+      continue;
+    }
+
+    grid_locate(__logf(e), P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
+    const float4 eb = lerp4(P.t.elec_b + row, ei, ew);  // scattering power, log eta
+    const float mean_square = eb.x * rho * s;
+    float cos_hinge;
+    if (P.msc_gs) {
+      const float fx = eb.y * kGsBinsPerLog;
+      const float fy = __logf(fmaxf(mean_square, kGsTheta2Min)) * kGsBinsPerLog;
+      cos_hinge = sample_gs(P.t, fx, fy, mean_square, rng);
+    } else {
+      cos_hinge = sample_gaussian_hinge(mean_square, rng);
+    }
+    const float phi = kTwoPi * uniform(rng);
+    rotate(ux, uy, uz, cos_hinge, phi);
+
+    if (emit_brems && e > P.pcut) {
+      float k = P.pcut * __expf(uniform(rng) * __logf(e / P.pcut));
+      k = fminf(k, e - d2);
+      if (k > P.pcut) {
+        float photon_w = w;
+        if (k < kRouletteMeV && w < kRouletteWeightCap) {
+          photon_w = uniform(rng) < kRouletteSurvival ? w / kRouletteSurvival : 0.0f;
+          escape(tal, (w - photon_w) * k);
+        }
+        if (photon_w > 0.0f) {
+          const uint64_t child = spawn_stream(rng);
+          push(qp, tal, PHOTON, column, k, photon_w, x, y, z, ux, uy, uz, child);
+        }
+      } else {
+        deposit_or_escape(P, tal, base, x, y, z, w * k,
+                          spr_factor(P.t, P.dose_to_water, k, mat, P.ecut));
+      }
+      e -= k;
+    }
+
+    deposit_spread(P, tal, base, x, y, z, ux, uy, uz, s2, w * d2, factor);
+    e -= d2;
+    x += ux * s2;
+    y += uy * s2;
+    z += uz * s2;
+    if (!inside_grid(P.g, x, y, z)) {
+      escape(tal, w * (e + latent));
+      active = false;
+      continue;
+    }
+
+    if (moller_pending && e > 2.0f * P.ecut) {
+      const float delta = sample_moller(e, P.ecut, rng);
+      float cd, cp;
+      moller_cosines(e, delta, cd, cp);
+      const float mphi = kTwoPi * uniform(rng);
+      float dx = ux, dy = uy, dz = uz;
+      rotate(dx, dy, dz, cd, mphi);
+      const uint64_t child = spawn_stream(rng);
+      push(qe, tal, ELECTRON, column, delta, w, x, y, z, dx, dy, dz, child);
+      rotate(ux, uy, uz, cp, mphi + kPi);
+      e -= delta;
+    }
+  }
+  tal.flush(P.ledger, L_ESTEPS);
+}
+
+// -------------------------------------------------------------------- generate
+
+struct DevSource {
+  int kind, particle;
+  float energy;
+  float px, py, pz, dx, dy, dz;
+  float z0, x0, xe, y0, ye;
+  float fx, fy, fz, uax, uay, uaz, vax, vay, vaz, width_u, width_v;
+  int n_bins;
+  const float* edges;
+  const float* cdf;
+  const float* beamlet;  // lattice [n][4], fan [n][3]
+  // uploaded primaries
+  const int32_t* up_particle;
+  const float* up;  // [9][n]: energy, x, y, z, ux, uy, uz, weight
+  long long up_n;
+};
+
+struct DevItem {
+  long long key_base;
+  long long start;  // exclusive prefix of counts
+  int column;
+  int pad;
+};
+
+// Spectrum inversion: first cdf entry >= u, clamped to the top bin
+// (pyradmc/geometry/spectrum.py:120-143, kernels.py:2110-2117).
+__device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
+  const float u_bin = uniform(rng);
+  const float u_within = uniform(rng);
+  int lo = 0, hi = S.n_bins;  // lower_bound over [0, n_bins)
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (S.cdf[mid] < u_bin) lo = mid + 1;
+    else hi = mid;
+  }
+  const int k = min(lo, S.n_bins - 1);
+  const float a = S.edges[k];
+  return a + (S.edges[k + 1] - a) * u_within;
+}
+
+__global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* items, int n_items,
+                                                  long long p0, long long n_gen, uint64_t seed,
+                                                  Queue qp, Queue qe,
+                                                  unsigned long long* ledger) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  long long emitted = 0;
+  if (t < n_gen) {
+    const long long g = p0 + t;
+    int lo = 0, hi = n_items;  // last item with start <= g
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (items[mid].start <= g) lo = mid;
+      else hi = mid;
+    }
+    const DevItem item = items[lo];
+    const long long i = g - item.start;
+    const uint32_t column = static_cast<uint32_t>(item.column);
+    uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
+    uint32_t kind = static_cast<uint32_t>(S.particle);
+    float e = S.energy, w = 1.0f;
+    float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
+    bool book = false;
+    switch (S.kind) {
+      case HMC_SRC_PENCIL:
+        x = S.px; y = S.py; z = S.pz;
+        ux = S.dx; uy = S.dy; uz = S.dz;
+        break;
+      case HMC_SRC_PARALLEL:
+        x = S.x0 + S.xe * uniform(rng);
+        y = S.y0 + S.ye * uniform(rng);
+        z = S.z0;
+        break;
+      case HMC_SRC_PARALLEL_SPECTRAL:
+        e = sample_spectrum(S, rng);
+        x = S.x0 + S.xe * uniform(rng);
+        y = S.y0 + S.ye * uniform(rng);
+        z = S.z0;
+        kind = PHOTON;
+        book = true;
+        break;
+      case HMC_SRC_LATTICE: {
+        const float* b = S.beamlet + 4 * static_cast<long long>(column);
+        x = b[0] + b[1] * uniform(rng);
+        y = b[2] + b[3] * uniform(rng);
+        z = S.z0;
+        kind = PHOTON;
+        break;
+      }
+      case HMC_SRC_SPECTRAL_FAN: {
+        e = sample_spectrum(S, rng);
+        const float du = (uniform(rng) - 0.5f) * S.width_u;
+        const float dv = (uniform(rng) - 0.5f) * S.width_v;
+        const float* c = S.beamlet + 3 * static_cast<long long>(column);
+        const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
+        const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
+        const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
+        const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        x = S.fx; y = S.fy; z = S.fz;
+        ux = dx * inv; uy = dy * inv; uz = dz * inv;
+        kind = PHOTON;
+        book = true;
+        break;
+      }
+      default: {  // uploaded
+        const long long j = g;
+        kind = static_cast<uint32_t>(S.up_particle[j]);
+        e = S.up[j];
+        x = S.up[S.up_n + j];
+        y = S.up[2 * S.up_n + j];
+        z = S.up[3 * S.up_n + j];
+        ux = S.up[4 * S.up_n + j];
+        uy = S.up[5 * S.up_n + j];
+        uz = S.up[6 * S.up_n + j];
+        w = S.up[7 * S.up_n + j];
+        book = true;
+        break;
+      }
+    }
+    if (book) {
+      const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
+      emitted = quanta_round(static_cast<double>(w * (e + latent)));
+    }
+    const Queue& q = kind == PHOTON ? qp : qe;
+    const unsigned int idx = atomicAdd(q.count, 1u);
+    if (idx < q.capacity) {
+      Particle p;
+      p.x = x; p.y = y; p.z = z; p.e = e;
+      p.ux = ux; p.uy = uy; p.uz = uz; p.w = w;
+      p.rng = rng;
+      p.meta = pack_meta(kind, 1u, column);
+      p.pad = 0u;
+      q.data[idx] = p;
+    }
+  }
+  if (emitted) atomicAdd(ledger + L_EMITTED, static_cast<unsigned long long>(emitted));
+}
+
+__global__ void k_fill_u32(unsigned int* a, int n, unsigned int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    a[i] = v;
+}
+
+// ---------------------------------------------------------------- fold/finalize
+//
+// Exact IEEE float64, contraction off, so the device fold reproduces the host
+// scorer arithmetic bit for bit (pyradmc/backends/warp/kernels.py:771-909,
+// pyradmc/scoring/dose.py:169-207).
+#pragma clang fp contract(off)
+
+__global__ void __launch_bounds__(256) k_fold(long long* edep, const double* mass,
+                                              const double* nhist, long long n_vox,
+                                              long long total, double* s1, double* s2,
+                                              unsigned long long* folded) {
+  long long local = 0;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long q = edep[i];
+    if (q == 0) continue;
+    edep[i] = 0;
+    const long long col = i / n_vox;
+    const long long v = i - col * n_vox;
+    const double m = mass[v];
+    double d = 0.0;
+    if (m > 0.0) {
+      const double energy = static_cast<double>(q) * 1.0e-9;
+      d = energy / (m * nhist[col]);
+    }
+    s1[i] = s1[i] + d;
+    s2[i] = s2[i] + d * d;
+    local += q;
+  }
+  if (local) atomicAdd(folded, static_cast<unsigned long long>(local));
+}
+
+__global__ void __launch_bounds__(256) k_finalize(double* s1, double* s2, long long total,
+                                                  int n_batches) {
+  const double n = static_cast<double>(n_batches);
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double a = s1[i];
+    const double b = s2[i];
+    if (a == 0.0 && b == 0.0) continue;  // exact zeros stay
+    const double mean = a / n;
+    double sigma = 0.0;
+    if (n_batches > 1) {
+      const double var = fmax(0.0, b / n - mean * mean) * n / (n - 1.0);
+      sigma = sqrt(var / n);
+    }
+    s1[i] = mean;
+    s2[i] = sigma;
+  }
+}
+
+// ------------------------------------------------------- truncation/compaction
+//
+// Keep an entry when m > 0 and m >= truncation * column max (one f64 product, as
+// the host assembler, pyradmc/scoring/dij.py:351-360). Each block owns one chunk of
+// one column; ascending scans within chunks and ascending chunk offsets reproduce
+// np.flatnonzero order, so the CSC row indices come out sorted.
+constexpr int kTruncBlock = 256;
+constexpr int kTruncPerThread = 16;
+constexpr int kTruncChunk = kTruncBlock * kTruncPerThread;
+
+__global__ void __launch_bounds__(kTruncBlock) k_column_max(const double* mean, long long n_vox,
+                                                            int chunks_per_col,
+                                                            unsigned long long* col_max) {
+  const long long col = blockIdx.x / chunks_per_col;
+  const long long chunk = blockIdx.x - col * chunks_per_col;
+  const long long v0 = chunk * kTruncChunk;
+  const long long v1 = min(v0 + kTruncChunk, n_vox);
+  double m = 0.0;
+  for (long long v = v0 + threadIdx.x; v < v1; v += kTruncBlock)
+    m = fmax(m, mean[col * n_vox + v]);
+  __shared__ double red[kTruncBlock];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kTruncBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // Non-negative doubles order like their bit patterns.
+    unsigned long long bits;
+    const double v = red[0];
+    memcpy(&bits, &v, sizeof(bits));
+    atomicMax(col_max + col, bits);
+  }
+}
+
+__device__ inline bool keep_entry(double m, double thr) { return m > 0.0 && m >= thr; }
+
+__global__ void __launch_bounds__(kTruncBlock) k_count_kept(const double* mean, long long n_vox,
+                                                            int chunks_per_col,
+                                                            const unsigned long long* col_max,
+                                                            double truncation, long long* counts) {
+  const long long col = blockIdx.x / chunks_per_col;
+  const long long chunk = blockIdx.x - col * chunks_per_col;
+  const long long v0 = chunk * kTruncChunk;
+  const long long v1 = min(v0 + kTruncChunk, n_vox);
+  double cm;
+  const unsigned long long bits = col_max[col];
+  memcpy(&cm, &bits, sizeof(cm));
+  const double thr = truncation * cm;
+  int c = 0;
+  for (long long v = v0 + threadIdx.x; v < v1; v += kTruncBlock)
+    c += keep_entry(mean[col * n_vox + v], thr) ? 1 : 0;
+  __shared__ int red[kTruncBlock];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = kTruncBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kTruncBlock) k_compact(const double* mean, const double* sigma,
+                                                         long long n_vox, int chunks_per_col,
+                                                         const unsigned long long* col_max,
+                                                         double truncation,
+                                                         const long long* offsets,
+                                                         long long* out_idx, double* out_dose,
+                                                         double* out_sigma) {
+  const long long col = blockIdx.x / chunks_per_col;
+  const long long chunk = blockIdx.x - col * chunks_per_col;
+  const long long v0 = chunk * kTruncChunk + static_cast<long long>(threadIdx.x) * kTruncPerThread;
+  double cm;
+  const unsigned long long bits = col_max[col];
+  memcpy(&cm, &bits, sizeof(cm));
+  const double thr = truncation * cm;
+  // Each thread owns kTruncPerThread consecutive voxels; an exclusive block scan of
+  // the per-thread counts gives ordered write cursors.
+  int c = 0;
+  for (int k = 0; k < kTruncPerThread; ++k) {
+    const long long v = v0 + k;
+    if (v < n_vox && keep_entry(mean[col * n_vox + v], thr)) ++c;
+  }
+  __shared__ int scan[kTruncBlock];
+  scan[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 1; s < kTruncBlock; s <<= 1) {
+    const int add = threadIdx.x >= s ? scan[threadIdx.x - s] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += add;
+    __syncthreads();
+  }
+  long long cursor = offsets[blockIdx.x] + scan[threadIdx.x] - c;
+  for (int k = 0; k < kTruncPerThread; ++k) {
+    const long long v = v0 + k;
+    if (v < n_vox) {
+      const double m = mean[col * n_vox + v];
+      if (keep_entry(m, thr)) {
+        out_idx[cursor] = v;
+        out_dose[cursor] = m;
+        out_sigma[cursor] = sigma[col * n_vox + v];
+        ++cursor;
+      }
+    }
+  }
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace
+
+// =============================================================== host context
+
+struct hmc_ctx {
+  int device = 0;
+  std::string error;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+
+  // transport grid
+  DevGrid g{};
+  float* d_density = nullptr;
+  uint8_t* d_material = nullptr;
+  int max_material = 0;
+  bool has_grid = false;
+
+  // scoring
+  DevScoring s{};
+  double* d_mass = nullptr;
+  long long n_columns = 0;
+  long long* d_edep = nullptr;
+  double* d_s1 = nullptr;
+  double* d_s2 = nullptr;
+  double* d_nhist = nullptr;
+  bool has_scoring = false;
+  bool finalized = false;
+
+  // tables
+  DevTables t{};
+  std::vector<void*> table_buffers;
+  bool has_tables = false;
+  float* d_gs = nullptr;
+  bool has_gs = false;
+
+  // source
+  DevSource src{};
+  float* d_edges = nullptr;
+  float* d_cdf = nullptr;
+  float* d_beamlet = nullptr;
+  int32_t* d_up_particle = nullptr;
+  float* d_up = nullptr;
+  bool has_source = false;
+
+  hmc_physics phys{};
+  bool has_physics = false;
+
+  // queues: 4 queues + counts (4) + heads (4)
+  Particle* d_q[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t q_capacity = 0;
+  unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads
+  unsigned long long* d_ledger = nullptr;
+  DevItem* d_items = nullptr;
+  size_t items_cap = 0;
+
+  // truncation
+  unsigned long long* d_colmax = nullptr;
+  long long* d_chunk_counts = nullptr;
+  long long* d_chunk_offsets = nullptr;
+  long long* d_out_idx = nullptr;
+  double* d_out_dose = nullptr;
+  double* d_out_sigma = nullptr;
+  long long trunc_total = 0;
+  size_t out_cap = 0;
+};
+
+namespace {
+
+int fail(hmc_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+#define HMC_HIP(ctx, call)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(ctx, _e == hipErrorOutOfMemory ? HMC_ENOMEM : HMC_EHIP,               \
+                  std::string(#call) + ": " + hipGetErrorString(_e));                   \
+  } while (0)
+
+template <typename T>
+int dev_alloc(hmc_ctx* ctx, T** ptr, size_t count) {
+  if (*ptr) {
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+  }
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(ctx, e == hipErrorOutOfMemory ? HMC_ENOMEM : HMC_EHIP,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return HMC_OK;
+}
+
+template <typename T>
+int upload(hmc_ctx* ctx, T** ptr, const T* host, size_t count) {
+  int rc = dev_alloc(ctx, ptr, count);
+  if (rc) return rc;
+  if (host && count) HMC_HIP(ctx, hipMemcpy(*ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return HMC_OK;
+}
+
+void free_ptr(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int blocks_for(long long n, int block = 256, long long cap = 1 << 20) {
+  long long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return static_cast<int>(b);
+}
+
+}  // namespace
+
+// ======================================================================== ABI
+
+extern "C" {
+
+int hmc_create(int device, hmc_ctx** out) {
+  if (!out) return HMC_EINVAL;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HMC_EINVAL;
+  auto* ctx = new hmc_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return HMC_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return HMC_EHIP;
+  }
+  if (dev_alloc(ctx, &ctx->d_counts, 8) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1)) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HMC_ENOMEM;
+  }
+  *out = ctx;
+  return HMC_OK;
+}
+
+void hmc_destroy(hmc_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  void* ptrs[] = {ctx->d_density,      ctx->d_material,    ctx->d_mass,       ctx->d_edep,
+                  ctx->d_s1,           ctx->d_s2,          ctx->d_nhist,      ctx->d_gs,
+                  ctx->d_edges,        ctx->d_cdf,         ctx->d_beamlet,    ctx->d_up_particle,
+                  ctx->d_up,           ctx->d_q[0],        ctx->d_q[1],       ctx->d_q[2],
+                  ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
+                  ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
+                  ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma};
+  for (void* p : ptrs) free_ptr(p);
+  for (void* p : ctx->table_buffers) free_ptr(p);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* hmc_last_error(const hmc_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int hmc_device_info(int device, char* name, int name_len, int* cu_count, int64_t* total_bytes,
+                    int64_t* free_bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HMC_EINVAL;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HMC_EHIP;
+  if (name && name_len > 0) {
+    std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (total_bytes) *total_bytes = static_cast<int64_t>(prop.totalGlobalMem);
+  if (free_bytes) {
+    size_t f = 0, tot = 0;
+    (void)hipSetDevice(device);
+    if (hipMemGetInfo(&f, &tot) != hipSuccess) return HMC_EHIP;
+    *free_bytes = static_cast<int64_t>(f);
+  }
+  return HMC_OK;
+}
+
+int hmc_set_grid(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
+                 const double* spacing, const float* density, const uint8_t* material) {
+  if (!ctx || !origin || !spacing || !density || !material) return fail(ctx, HMC_EINVAL, "null");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(ctx, HMC_EINVAL, "empty grid shape");
+  if (spacing[0] <= 0 || spacing[1] <= 0 || spacing[2] <= 0)
+    return fail(ctx, HMC_EINVAL, "non-positive spacing");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t n = static_cast<size_t>(nx) * ny * nz;
+  int rc = upload(ctx, &ctx->d_density, density, n);
+  if (rc) return rc;
+  rc = upload(ctx, &ctx->d_material, material, n);
+  if (rc) return rc;
+  DevGrid& g = ctx->g;
+  g.x_lo = static_cast<float>(origin[0]);
+  g.y_lo = static_cast<float>(origin[1]);
+  g.z_lo = static_cast<float>(origin[2]);
+  g.x_hi = static_cast<float>(origin[0] + nx * spacing[0]);
+  g.y_hi = static_cast<float>(origin[1] + ny * spacing[1]);
+  g.z_hi = static_cast<float>(origin[2] + nz * spacing[2]);
+  g.sx = static_cast<float>(spacing[0]);
+  g.sy = static_cast<float>(spacing[1]);
+  g.sz = static_cast<float>(spacing[2]);
+  g.inv_sx = static_cast<float>(1.0 / spacing[0]);
+  g.inv_sy = static_cast<float>(1.0 / spacing[1]);
+  g.inv_sz = static_cast<float>(1.0 / spacing[2]);
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.density = ctx->d_density;
+  g.material = ctx->d_material;
+  ctx->max_material = static_cast<int>(*std::max_element(material, material + n));
+  ctx->has_grid = true;
+  return HMC_OK;
+}
+
+int hmc_set_scoring(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
+                    const double* spacing, const double* voxel_mass, int64_t n_columns) {
+  if (!ctx || !origin || !spacing || !voxel_mass) return fail(ctx, HMC_EINVAL, "null");
+  if (nx < 1 || ny < 1 || nz < 1 || n_columns < 1 || n_columns >= (1ll << 28))
+    return fail(ctx, HMC_EINVAL, "bad scoring shape or column count");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const long long nv = static_cast<long long>(nx) * ny * nz;
+  int rc = upload(ctx, &ctx->d_mass, voxel_mass, static_cast<size_t>(nv));
+  if (rc) return rc;
+  const size_t total = static_cast<size_t>(nv) * static_cast<size_t>(n_columns);
+  if ((rc = dev_alloc(ctx, &ctx->d_edep, total))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_s1, total))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_s2, total))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_nhist, static_cast<size_t>(n_columns)))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_colmax, static_cast<size_t>(n_columns)))) return rc;
+  DevScoring& s = ctx->s;
+  s.x_lo = static_cast<float>(origin[0]);
+  s.y_lo = static_cast<float>(origin[1]);
+  s.z_lo = static_cast<float>(origin[2]);
+  s.x_hi = static_cast<float>(origin[0] + nx * spacing[0]);
+  s.y_hi = static_cast<float>(origin[1] + ny * spacing[1]);
+  s.z_hi = static_cast<float>(origin[2] + nz * spacing[2]);
+  s.inv_sx = static_cast<float>(1.0 / spacing[0]);
+  s.inv_sy = static_cast<float>(1.0 / spacing[1]);
+  s.inv_sz = static_cast<float>(1.0 / spacing[2]);
+  s.nx = nx;
+  s.ny = ny;
+  s.nz = nz;
+  s.n_voxels = nv;
+  ctx->n_columns = n_columns;
+  ctx->has_scoring = true;
+  return hmc_reset_maps(ctx);
+}
+
+int hmc_reset_maps(hmc_ctx* ctx) {
+  if (!ctx || !ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const long long total = ctx->s.n_voxels * ctx->n_columns;
+  const int blocks = blocks_for(total, 256, 65536);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_edep), total, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_s1), total, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_s2), total, 0ull);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->finalized = false;
+  return HMC_OK;
+}
+
+int hmc_set_tables(hmc_ctx* ctx, const hmc_tables* tb) {
+  if (!ctx || !tb) return fail(ctx, HMC_EINVAL, "null");
+  const int n = tb->n_points, nm = tb->n_mat, nc = tb->n_coherent;
+  if (n < 2 || nm < 1 || nm > 256 || nc < 2)
+    return fail(ctx, HMC_EINVAL, "table sizes out of range (n_points>=2, 1<=n_mat<=256)");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  for (void* p : ctx->table_buffers) free_ptr(p);
+  ctx->table_buffers.clear();
+  const size_t rows = static_cast<size_t>(nm) * n;
+  std::vector<float4> photon(rows), ea(rows), eb(rows);
+  for (size_t i = 0; i < rows; ++i) {
+    photon[i] = make_float4(tb->mu_compton[i], tb->mu_photo[i], tb->mu_pair[i], tb->mu_rayleigh[i]);
+    ea[i] = make_float4(tb->csda_range[i], tb->moller[i], tb->restricted_range[i],
+                        tb->stopping_radiative[i]);
+    eb[i] = make_float4(tb->scattering_power[i], tb->log_eta[i], tb->stopping_restricted[i], 0.0f);
+  }
+  float* d_maj = nullptr;
+  float4 *d_ph = nullptr, *d_ea = nullptr, *d_eb = nullptr;
+  float *d_inv = nullptr, *d_cx = nullptr, *d_cc = nullptr;
+  int rc;
+  if ((rc = upload(ctx, &d_maj, tb->majorant, n))) return rc;
+  ctx->table_buffers.push_back(d_maj);
+  if ((rc = upload(ctx, &d_ph, photon.data(), rows))) return rc;
+  ctx->table_buffers.push_back(d_ph);
+  if ((rc = upload(ctx, &d_ea, ea.data(), rows))) return rc;
+  ctx->table_buffers.push_back(d_ea);
+  if ((rc = upload(ctx, &d_eb, eb.data(), rows))) return rc;
+  ctx->table_buffers.push_back(d_eb);
+  if ((rc = upload(ctx, &d_inv, tb->energy_of_restricted_range, rows))) return rc;
+  ctx->table_buffers.push_back(d_inv);
+  if ((rc = upload(ctx, &d_cx, tb->coherent_x, static_cast<size_t>(nc)))) return rc;
+  ctx->table_buffers.push_back(d_cx);
+  if ((rc = upload(ctx, &d_cc, tb->coherent_cumulative, static_cast<size_t>(nm) * nc))) return rc;
+  ctx->table_buffers.push_back(d_cc);
+  DevTables& t = ctx->t;
+  t.majorant = d_maj;
+  t.photon = d_ph;
+  t.elec_a = d_ea;
+  t.elec_b = d_eb;
+  t.inv_range = d_inv;
+  t.coherent_x = d_cx;
+  t.coherent_cumulative = d_cc;
+  t.p_log_min = static_cast<float>(tb->p_log_min);
+  t.p_inv_dlog = static_cast<float>(tb->p_inv_dlog);
+  t.e_log_min = static_cast<float>(tb->e_log_min);
+  t.e_inv_dlog = static_cast<float>(tb->e_inv_dlog);
+  t.r_log_min = static_cast<float>(tb->r_log_min);
+  t.r_inv_dlog = static_cast<float>(tb->r_inv_dlog);
+  t.n_points = n;
+  t.n_mat = nm;
+  t.n_coherent = nc;
+  ctx->has_tables = true;
+  return HMC_OK;
+}
+
+int hmc_set_gs(hmc_ctx* ctx, const float* values, int n_eta, int n_theta2, int n_u, int ix0,
+               int iy0) {
+  if (!ctx || !values) return fail(ctx, HMC_EINVAL, "null");
+  if (n_eta < 2 || n_theta2 < 2 || n_u < 1)
+    return fail(ctx, HMC_EINVAL, "GS grid needs at least 2x2 nodes");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t count = static_cast<size_t>(n_eta) * n_theta2 * n_u;
+  int rc = upload(ctx, &ctx->d_gs, values, count);
+  if (rc) return rc;
+  ctx->t.gs = ctx->d_gs;
+  ctx->t.gs_ix0 = ix0;
+  ctx->t.gs_iy0 = iy0;
+  ctx->t.gs_n_eta = n_eta;
+  ctx->t.gs_n_theta2 = n_theta2;
+  ctx->t.gs_n_u = n_u;
+  ctx->has_gs = true;
+  return HMC_OK;
+}
+
+int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
+  if (!ctx || !s) return fail(ctx, HMC_EINVAL, "null");
+  if (s->kind < HMC_SRC_PENCIL || s->kind > HMC_SRC_UPLOADED)
+    return fail(ctx, HMC_EINVAL, "unknown source kind");
+  if (s->particle < 0 || s->particle > 2) return fail(ctx, HMC_EINVAL, "unknown particle kind");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  DevSource& d = ctx->src;
+  d.kind = s->kind;
+  d.particle = s->particle;
+  d.energy = static_cast<float>(s->energy);
+  d.px = static_cast<float>(s->position[0]);
+  d.py = static_cast<float>(s->position[1]);
+  d.pz = static_cast<float>(s->position[2]);
+  d.dx = static_cast<float>(s->direction[0]);
+  d.dy = static_cast<float>(s->direction[1]);
+  d.dz = static_cast<float>(s->direction[2]);
+  d.z0 = static_cast<float>(s->z0);
+  d.x0 = static_cast<float>(s->x0);
+  d.xe = static_cast<float>(s->x_extent);
+  d.y0 = static_cast<float>(s->y0);
+  d.ye = static_cast<float>(s->y_extent);
+  d.fx = static_cast<float>(s->focal[0]);
+  d.fy = static_cast<float>(s->focal[1]);
+  d.fz = static_cast<float>(s->focal[2]);
+  d.uax = static_cast<float>(s->u_axis[0]);
+  d.uay = static_cast<float>(s->u_axis[1]);
+  d.uaz = static_cast<float>(s->u_axis[2]);
+  d.vax = static_cast<float>(s->v_axis[0]);
+  d.vay = static_cast<float>(s->v_axis[1]);
+  d.vaz = static_cast<float>(s->v_axis[2]);
+  d.width_u = static_cast<float>(s->width_u);
+  d.width_v = static_cast<float>(s->width_v);
+  d.n_bins = s->n_bins;
+  int rc;
+  const bool spectral = s->kind == HMC_SRC_PARALLEL_SPECTRAL || s->kind == HMC_SRC_SPECTRAL_FAN;
+  if (spectral) {
+    if (s->n_bins < 1 || !s->edges || !s->cdf) return fail(ctx, HMC_EINVAL, "spectrum missing");
+    std::vector<float> e(s->edges, s->edges + s->n_bins + 1), c(s->cdf, s->cdf + s->n_bins);
+    if ((rc = upload(ctx, &ctx->d_edges, e.data(), e.size()))) return rc;
+    if ((rc = upload(ctx, &ctx->d_cdf, c.data(), c.size()))) return rc;
+    d.edges = ctx->d_edges;
+    d.cdf = ctx->d_cdf;
+  }
+  if (s->kind == HMC_SRC_LATTICE || s->kind == HMC_SRC_SPECTRAL_FAN) {
+    const int stride = s->kind == HMC_SRC_LATTICE ? 4 : 3;
+    if (s->n_beamlets < 1 || !s->beamlet_params)
+      return fail(ctx, HMC_EINVAL, "beamlet parameters missing");
+    std::vector<float> b(s->beamlet_params, s->beamlet_params + s->n_beamlets * stride);
+    if ((rc = upload(ctx, &ctx->d_beamlet, b.data(), b.size()))) return rc;
+    d.beamlet = ctx->d_beamlet;
+  }
+  ctx->has_source = true;
+  return HMC_OK;
+}
+
+int hmc_set_uploaded(hmc_ctx* ctx, int64_t n, const int32_t* particle, const float* energy,
+                     const float* x, const float* y, const float* z, const float* ux,
+                     const float* uy, const float* uz, const float* weight) {
+  if (!ctx || n < 1 || !particle || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
+    return fail(ctx, HMC_EINVAL, "uploaded primaries incomplete");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  for (int64_t i = 0; i < n; ++i)
+    if (particle[i] < 0 || particle[i] > 2) return fail(ctx, HMC_EINVAL, "bad particle kind");
+  int rc = upload(ctx, &ctx->d_up_particle, particle, static_cast<size_t>(n));
+  if (rc) return rc;
+  std::vector<float> cols(static_cast<size_t>(n) * 8);
+  const float* src[8] = {energy, x, y, z, ux, uy, uz, weight};
+  for (int c = 0; c < 8; ++c) std::memcpy(cols.data() + c * n, src[c], sizeof(float) * n);
+  if ((rc = upload(ctx, &ctx->d_up, cols.data(), cols.size()))) return rc;
+  ctx->src.up_particle = ctx->d_up_particle;
+  ctx->src.up = ctx->d_up;
+  ctx->src.up_n = n;
+  return HMC_OK;
+}
+
+int hmc_set_physics(hmc_ctx* ctx, const hmc_physics* p) {
+  if (!ctx || !p) return fail(ctx, HMC_EINVAL, "null");
+  if (p->pcut <= 0 || p->ecut <= 0) return fail(ctx, HMC_EINVAL, "non-positive cutoff");
+  if (p->step_energy_fraction <= 0) return fail(ctx, HMC_EINVAL, "non-positive step fraction");
+  ctx->phys = *p;
+  ctx->has_physics = true;
+  return HMC_OK;
+}
+
+int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n_items,
+                  int64_t chunk, int queue_factor, hmc_ledger* ledger) {
+  if (!ctx || !items || n_items < 1 || chunk < 1 || queue_factor < 1 || !ledger)
+    return fail(ctx, HMC_EINVAL, "bad transport arguments");
+  if (!ctx->has_grid || !ctx->has_scoring || !ctx->has_tables || !ctx->has_source ||
+      !ctx->has_physics)
+    return fail(ctx, HMC_ESTATE, "grid, scoring, tables, source and physics must be set");
+  if (ctx->phys.msc_gs && !ctx->has_gs) return fail(ctx, HMC_ESTATE, "GS grid not set");
+  if (ctx->max_material >= ctx->t.n_mat)
+    return fail(ctx, HMC_EINVAL, "voxel material index " + std::to_string(ctx->max_material) +
+                                     " beyond the " + std::to_string(ctx->t.n_mat) +
+                                     " material rows of the tables");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+
+  std::vector<DevItem> dev_items(static_cast<size_t>(n_items));
+  long long total = 0;
+  for (int i = 0; i < n_items; ++i) {
+    if (items[i].count < 0 || items[i].column < 0 || items[i].column >= ctx->n_columns)
+      return fail(ctx, HMC_EINVAL, "work item count/column out of range");
+    dev_items[i].key_base = items[i].key_base;
+    dev_items[i].start = total;
+    dev_items[i].column = items[i].column;
+    dev_items[i].pad = 0;
+    total += items[i].count;
+  }
+  if (ctx->src.kind == HMC_SRC_UPLOADED && total > ctx->src.up_n)
+    return fail(ctx, HMC_EINVAL, "more histories than uploaded primaries");
+  if (ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN) {
+    for (int i = 0; i < n_items; ++i)
+      if (items[i].column >= static_cast<int>(0x7fffffff)) return fail(ctx, HMC_EINVAL, "column");
+  }
+  int rc;
+  if (static_cast<size_t>(n_items) > ctx->items_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->d_items, static_cast<size_t>(n_items)))) return rc;
+    ctx->items_cap = static_cast<size_t>(n_items);
+  }
+  HMC_HIP(ctx, hipMemcpyAsync(ctx->d_items, dev_items.data(), sizeof(DevItem) * n_items,
+                              hipMemcpyHostToDevice, ctx->stream));
+
+  const long long chunk_n = std::min<long long>(chunk, std::max<long long>(total, 1));
+  const size_t capacity = static_cast<size_t>(chunk_n) * static_cast<size_t>(queue_factor);
+  if (capacity > 0xFFFFFFF0ull) return fail(ctx, HMC_EINVAL, "queue capacity exceeds 2^32");
+  if (capacity > ctx->q_capacity) {
+    for (int q = 0; q < 4; ++q)
+      if ((rc = dev_alloc(ctx, &ctx->d_q[q], capacity))) return rc;
+    ctx->q_capacity = capacity;
+  }
+
+  TransportParams P;
+  P.g = ctx->g;
+  P.s = ctx->s;
+  P.t = ctx->t;
+  P.edep = ctx->d_edep;
+  P.ledger = ctx->d_ledger;
+  P.pcut = static_cast<float>(ctx->phys.pcut);
+  P.ecut = static_cast<float>(ctx->phys.ecut);
+  P.step_fraction = static_cast<float>(ctx->phys.step_energy_fraction);
+  P.resolution = static_cast<float>(ctx->phys.deposit_resolution_cm);
+  P.transport_electrons = ctx->phys.transport_electrons;
+  P.msc_gs = ctx->phys.msc_gs;
+  P.dose_to_water = ctx->phys.dose_to_water;
+  P.max_steps = 1 << 20;
+
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ledger,
+                     static_cast<long long>(L_COUNT), 0ull);
+
+  const unsigned int cap = static_cast<unsigned int>(ctx->q_capacity);
+  const long long max_lanes = static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048;
+  long long overflow = 0, rounds = 0;
+  unsigned int* counts = ctx->d_counts;
+  unsigned int* heads = ctx->d_counts + 4;
+
+  for (long long p0 = 0; p0 < total; p0 += chunk_n) {
+    const long long n_gen = std::min(chunk_n, total - p0);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, 8, 0u);
+    int pa = 0, ea = 1, pb = 2, eb = 3;  // queue roles
+    Queue qpa{ctx->d_q[pa], counts + pa, cap}, qea{ctx->d_q[ea], counts + ea, cap};
+    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 1ll << 31)), dim3(256), 0,
+                       ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
+                       static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
+    HMC_HIP(ctx, hipGetLastError());
+    for (int round = 0; round < 100000; ++round) {
+      unsigned int host_counts[4];
+      HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+      HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      for (int q = 0; q < 4; ++q)
+        if (host_counts[q] > cap) overflow = std::max<long long>(overflow, host_counts[q]);
+      if (overflow) break;
+      const unsigned int n_p = host_counts[pa], n_e = host_counts[ea];
+      if (n_p == 0 && n_e == 0) break;
+      ++rounds;
+      Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
+      Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
+      if (n_p) {
+        // Photon round: consume pa; electrons append to ea, photons to pb.
+        const long long lanes = std::min<long long>(n_p, max_lanes);
+        hipLaunchKernelGGL(k_photon_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream, P,
+                           qa_p, heads + pa, qa_e, qb_p);
+        HMC_HIP(ctx, hipGetLastError());
+      }
+      // Electron round: consume ea (including the photon round's pushes).
+      unsigned int n_e_now = 0;
+      HMC_HIP(ctx, hipMemcpyAsync(&n_e_now, counts + ea, sizeof(unsigned int),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+      HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (n_e_now > cap) {
+        overflow = n_e_now;
+        break;
+      }
+      if (n_e_now) {
+        const long long lanes = std::min<long long>(n_e_now, max_lanes);
+        hipLaunchKernelGGL(k_electron_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream,
+                           P, qa_e, heads + ea, qb_p, qb_e);
+        HMC_HIP(ctx, hipGetLastError());
+      }
+      // Reset the consumed queues (counts and heads), then swap roles.
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + pa, 1, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + ea, 1, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, heads + pa, 1, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, heads + ea, 1, 0u);
+      std::swap(pa, pb);
+      std::swap(ea, eb);
+    }
+    if (overflow) break;
+  }
+  unsigned long long host_ledger[L_COUNT];
+  HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ledger->escaped = static_cast<int64_t>(host_ledger[L_ESCAPED]);
+  ledger->unscored = static_cast<int64_t>(host_ledger[L_UNSCORED]);
+  ledger->deposited = static_cast<int64_t>(host_ledger[L_DEPOSITED]);
+  ledger->emitted = static_cast<int64_t>(host_ledger[L_EMITTED]);
+  ledger->voxel_fetches = static_cast<int64_t>(host_ledger[L_VOX]);
+  ledger->deposits = static_cast<int64_t>(host_ledger[L_DEP]);
+  ledger->pushes = static_cast<int64_t>(host_ledger[L_PUSH]);
+  ledger->photon_steps = static_cast<int64_t>(host_ledger[L_PSTEPS]);
+  ledger->electron_steps = static_cast<int64_t>(host_ledger[L_ESTEPS]);
+  ledger->violations = static_cast<int64_t>(host_ledger[L_VIOLATIONS]);
+  ledger->trapped = static_cast<int64_t>(host_ledger[L_TRAPPED]);
+  ledger->overflow = overflow;
+  ledger->rounds = rounds;
+  if (overflow)
+    return fail(ctx, HMC_EOVERFLOW,
+                "secondary queue overflow (" + std::to_string(overflow) + " > capacity " +
+                    std::to_string(cap) + "); raise queue_factor or lower chunk_size");
+  if (ledger->violations)
+    return fail(ctx, HMC_EMAJORANT,
+                "Woodcock majorant violated in the kernel despite table headroom. The geometry "
+                "contains material or density the majorant declaration did not cover.");
+  return HMC_OK;
+}
+
+int hmc_fold(hmc_ctx* ctx, const double* nhist, int64_t* total_quanta) {
+  if (!ctx || !nhist) return fail(ctx, HMC_EINVAL, "null");
+  if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  for (long long c = 0; c < ctx->n_columns; ++c)
+    if (!(nhist[c] > 0.0)) return fail(ctx, HMC_EINVAL, "empty batch for a column");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipMemcpyAsync(ctx->d_nhist, nhist, sizeof(double) * ctx->n_columns,
+                              hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ledger + L_COUNT,
+                     1ll, 0ull);
+  const long long total = ctx->s.n_voxels * ctx->n_columns;
+  unsigned long long* folded = ctx->d_ledger + L_COUNT;  // scratch slot past the ledger
+  hipLaunchKernelGGL(k_fold, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream,
+                     ctx->d_edep, ctx->d_mass, ctx->d_nhist, ctx->s.n_voxels, total, ctx->d_s1,
+                     ctx->d_s2, folded);
+  HMC_HIP(ctx, hipGetLastError());
+  unsigned long long q = 0;
+  HMC_HIP(ctx, hipMemcpyAsync(&q, folded, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (total_quanta) *total_quanta = static_cast<int64_t>(q);
+  return HMC_OK;
+}
+
+int hmc_finalize(hmc_ctx* ctx, int n_batches, double* mean, double* sigma) {
+  if (!ctx || n_batches < 1) return fail(ctx, HMC_EINVAL, "bad finalize arguments");
+  if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const long long total = ctx->s.n_voxels * ctx->n_columns;
+  if (!ctx->finalized) {
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(total, 256, 65536)), dim3(256), 0,
+                       ctx->stream, ctx->d_s1, ctx->d_s2, total, n_batches);
+    HMC_HIP(ctx, hipGetLastError());
+    ctx->finalized = true;
+  }
+  if (mean)
+    HMC_HIP(ctx, hipMemcpyAsync(mean, ctx->d_s1, sizeof(double) * total, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  if (sigma)
+    HMC_HIP(ctx, hipMemcpyAsync(sigma, ctx->d_s2, sizeof(double) * total, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+int hmc_truncate(hmc_ctx* ctx, double truncation, int64_t* counts, int64_t* total_out) {
+  if (!ctx || !counts || truncation < 0.0) return fail(ctx, HMC_EINVAL, "bad truncation arguments");
+  if (!ctx->finalized) return fail(ctx, HMC_ESTATE, "finalize before truncation");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const long long nv = ctx->s.n_voxels;
+  const int chunks = static_cast<int>((nv + kTruncChunk - 1) / kTruncChunk);
+  const long long n_blocks = static_cast<long long>(chunks) * ctx->n_columns;
+  if (n_blocks > 0x7fffffffll) return fail(ctx, HMC_EINVAL, "too many truncation chunks");
+  int rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_chunk_counts, static_cast<size_t>(n_blocks)))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_chunk_offsets, static_cast<size_t>(n_blocks)))) return rc;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, ctx->stream, ctx->d_colmax,
+                     ctx->n_columns, 0ull);
+  hipLaunchKernelGGL(k_column_max, dim3(static_cast<unsigned>(n_blocks)), dim3(kTruncBlock), 0,
+                     ctx->stream, ctx->d_s1, nv, chunks, ctx->d_colmax);
+  hipLaunchKernelGGL(k_count_kept, dim3(static_cast<unsigned>(n_blocks)), dim3(kTruncBlock), 0,
+                     ctx->stream, ctx->d_s1, nv, chunks, ctx->d_colmax, truncation,
+                     ctx->d_chunk_counts);
+  HMC_HIP(ctx, hipGetLastError());
+  std::vector<long long> chunk_counts(static_cast<size_t>(n_blocks));
+  HMC_HIP(ctx, hipMemcpyAsync(chunk_counts.data(), ctx->d_chunk_counts,
+                              sizeof(long long) * n_blocks, hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<long long> offsets(static_cast<size_t>(n_blocks));
+  long long running = 0;
+  for (long long c = 0; c < ctx->n_columns; ++c) {
+    long long col_count = 0;
+    for (int k = 0; k < chunks; ++k) {
+      const long long b = c * chunks + k;
+      offsets[b] = running;
+      running += chunk_counts[b];
+      col_count += chunk_counts[b];
+    }
+    counts[c] = col_count;
+  }
+  ctx->trunc_total = running;
+  if (total_out) *total_out = running;
+  if (static_cast<size_t>(running) > ctx->out_cap || !ctx->d_out_idx) {
+    const size_t cap = static_cast<size_t>(std::max<long long>(running, 1));
+    if ((rc = dev_alloc(ctx, &ctx->d_out_idx, cap))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_out_dose, cap))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_out_sigma, cap))) return rc;
+    ctx->out_cap = cap;
+  }
+  HMC_HIP(ctx, hipMemcpyAsync(ctx->d_chunk_offsets, offsets.data(), sizeof(long long) * n_blocks,
+                              hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(n_blocks)), dim3(kTruncBlock), 0,
+                     ctx->stream, ctx->d_s1, ctx->d_s2, nv, chunks, ctx->d_colmax, truncation,
+                     ctx->d_chunk_offsets, ctx->d_out_idx, ctx->d_out_dose, ctx->d_out_sigma);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+int hmc_fetch_csc(hmc_ctx* ctx, int64_t* indices, double* dose, double* sigma) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  const long long n = ctx->trunc_total;
+  if (n == 0) return HMC_OK;
+  if (!indices || !dose || !sigma) return fail(ctx, HMC_EINVAL, "null output");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipMemcpyAsync(indices, ctx->d_out_idx, sizeof(long long) * n,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipMemcpyAsync(dose, ctx->d_out_dose, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HMC_HIP(ctx, hipMemcpyAsync(sigma, ctx->d_out_sigma, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,557 @@
+"""HipEngine: drop-in for the reference's ``WarpEngine`` on one MI355X.
+
+Same constructor (``grid, cross_sections, device, chunk_size, queue_factor``) and the
+same ``run`` / ``run_dij`` signatures and semantics as
+``pyradmc/backends/warp/engine.py:369-1230`` (and ``ReferenceEngine``,
+``pyradmc/backends/ref/engine.py:110-463``). The host side validates arguments,
+flattens the cross-sections (``build_tables``), derives the Goudsmit-Saunderson
+window, and drives the native library through its C ABI (``include/hipmc.h``):
+generation, transport, per-batch float64 fold, finalize, and the device truncation and
+CSC compaction all run on the GPU. There is no CPU fallback: the engine refuses to
+construct without the native library and a HIP device.
+
+History-to-stream mapping (reproducible on one device under any chunking, grouping or
+batching): ``run`` keys history ``h`` (batch-major order) by ``h``; ``run_dij`` keys
+the ``r``-th history of beamlet ``j`` by ``r`` (correlated, the default) or by its
+global index ``h = offset_j + r`` (independent), where ``offset_j = j * n_per`` for
+equal counts. ``run_dij`` additionally accepts a per-beamlet count array (uneven
+beamlets); each count must divide by ``n_batches``.
+"""
+
+from __future__ import annotations
+
+import math
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pyradmc_amd import _native as nat
+from pyradmc_amd.constants import (
+    DIJ_TRUNCATION_RELATIVE,
+    E_MAX_MARGIN,
+    ECUT_MEV,
+    ENERGY_QUANTUM_MEV,
+    PCUT_MEV,
+    VERSION,
+    default_step_energy_fraction,
+)
+from pyradmc_amd.data.gs import build_gs_grid, gs_window_from_tables
+from pyradmc_amd.progress import ProgressEmitter
+from pyradmc_amd.scoring import DijResult, RunProvenance, ScoringGrid, TransportResult
+
+_CHUNK_HISTORIES_PER_CU = 8192
+_CHUNK_SIZE_CAP = 4 * 1024 * 1024
+_CHUNK_MEMORY_FRACTION = 0.25
+_QUEUE_BYTES_PER_SLOT = 4 * 48
+_GROUP_MEMORY_FRACTION = 0.5
+
+
+def _device_index(device) -> int:
+    if isinstance(device, int):
+        return device
+    text = str(device)
+    if text in ("hip", "cuda", "gpu"):
+        return 0
+    for prefix in ("hip:", "cuda:", "gpu:"):
+        if text.startswith(prefix):
+            return int(text[len(prefix):])
+    raise ValueError(f"unsupported device {device!r}; expected 'hip:N'")
+
+
+def _validate_msc(msc_model: str) -> int:
+    if msc_model not in ("gaussian", "gs"):
+        raise ValueError(f"unknown msc_model {msc_model!r}; expected 'gaussian' or 'gs'")
+    return 1 if msc_model == "gs" else 0
+
+
+def _validate_scoring_mode(scoring_mode: str, transport_electrons: bool) -> bool:
+    if scoring_mode not in ("dose_to_medium", "dose_to_water"):
+        raise ValueError(
+            f"unknown scoring_mode {scoring_mode!r}; expected 'dose_to_medium' or 'dose_to_water'"
+        )
+    if scoring_mode == "dose_to_water" and not transport_electrons:
+        raise ValueError("dose_to_water needs transport_electrons=True (no electron, no SPR)")
+    return scoring_mode == "dose_to_water"
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _type_name(obj) -> str:
+    return type(obj).__name__
+
+
+@dataclass
+class HipEngine:
+    """One-GPU transport-and-scoring engine over a voxel grid."""
+
+    grid: object
+    cross_sections: object
+    device: object = "hip:0"
+    chunk_size: int | None = None
+    queue_factor: int = 16
+    _ctx: object = field(default=None, init=False, repr=False)
+    _table_cache: dict = field(default_factory=dict, init=False, repr=False)
+    _lock: threading.Lock = field(default_factory=threading.Lock, init=False, repr=False)
+    _grid_uploaded: bool = field(default=False, init=False, repr=False)
+    _uploaded_tables: object = field(default=None, init=False, repr=False)
+    _info: dict | None = field(default=None, init=False, repr=False)
+
+    def __post_init__(self) -> None:
+        if self.queue_factor < 1:
+            raise ValueError(f"queue_factor must be positive, got {self.queue_factor}")
+        if self.chunk_size is not None and self.chunk_size < 1:
+            raise ValueError(f"chunk_size must be positive, got {self.chunk_size}")
+        if int(self.cross_sections.n_materials) > 256:
+            raise ValueError(
+                "the HIP engine addresses at most 256 materials (uint8 voxel map); this "
+                f"source declares {self.cross_sections.n_materials}"
+            )
+        self._index = _device_index(self.device)
+        self._ctx = nat.Context(self._index)
+        self._info = nat.device_info(self._index)
+
+    def close(self) -> None:
+        if self._ctx is not None:
+            self._ctx.close()
+
+    # ------------------------------------------------------------------ helpers
+
+    @property
+    def device_name(self) -> str:
+        return self._info["name"] if self._info else "?"
+
+    def _provenance(self, seed, pcut, ecut, msc_model, step, deposit_resolution_cm) -> RunProvenance:
+        return RunProvenance(
+            version=VERSION,
+            backend="hip",
+            device=f"hip:{self._index}",
+            seed=int(seed),
+            pcut_mev=pcut,
+            ecut_mev=ecut,
+            msc_model=msc_model,
+            step_energy_fraction=step,
+            cross_sections=self.cross_sections.provenance,
+            deposit_resolution_cm=deposit_resolution_cm,
+        )
+
+    def _chunk(self, total: int) -> int:
+        if self.chunk_size is not None:
+            return int(min(self.chunk_size, max(total, 1)))
+        target = min(_CHUNK_HISTORIES_PER_CU * max(self._info["cu_count"], 1), _CHUNK_SIZE_CAP)
+        free = int(self._info.get("free_bytes", 0))
+        if free > 0:
+            target = min(target, int(free * _CHUNK_MEMORY_FRACTION) // (self.queue_factor * _QUEUE_BYTES_PER_SLOT))
+        return int(max(1, min(target, max(total, 1))))
+
+    def _upload_grid(self) -> None:
+        if self._grid_uploaded:
+            return
+        g = self.grid
+        density = _f32(g.density)
+        material = np.ascontiguousarray(np.asarray(g.material).astype(np.uint8))
+        origin = np.asarray(g.origin, dtype=np.float64)
+        spacing = np.asarray(g.spacing, dtype=np.float64)
+        self._ctx.call(
+            "hmc_set_grid", int(g.shape[0]), int(g.shape[1]), int(g.shape[2]),
+            nat.ptr(origin, nat.c_double), nat.ptr(spacing, nat.c_double),
+            nat.ptr(density, nat.ctypes.c_float), nat.ptr(material, nat.c_uint8),
+        )
+        self._grid_uploaded = True
+
+    def _host_tables(self, e_max: float, pcut: float, ecut: float):
+        key = (ecut, pcut, e_max)
+        with self._lock:
+            host = self._table_cache.get(key)
+            if host is None:
+                host = self.cross_sections.build_tables(ecut=ecut, pcut=pcut, e_max=e_max * E_MAX_MARGIN)
+                self._table_cache[key] = host
+        return host
+
+    def _upload_tables(self, e_max: float, pcut: float, ecut: float, with_gs: bool) -> None:
+        host = self._host_tables(e_max, pcut, ecut)
+        key = (ecut, pcut, e_max, with_gs)
+        if self._uploaded_tables == key:
+            return
+        keep = {}
+
+        def f(name, arr=None):
+            a = _f32(getattr(host, name) if arr is None else arr)
+            keep[name] = a
+            return nat.ptr(a, nat.ctypes.c_float)
+
+        n_mat = int(np.asarray(host.mu_compton).reshape(-1, host.n_points).shape[0])
+        t = nat.Tables()
+        t.n_points = int(host.n_points)
+        t.n_mat = n_mat
+        t.n_coherent = int(host.n_coherent)
+        for name in (
+            "majorant", "mu_compton", "mu_photo", "mu_pair", "mu_rayleigh", "csda_range", "moller",
+            "restricted_range", "stopping_radiative", "scattering_power", "log_eta",
+            "stopping_restricted", "energy_of_restricted_range", "coherent_x", "coherent_cumulative",
+        ):
+            setattr(t, name, f(name))
+        t.p_log_min = float(host.photon_log_e_min)
+        t.p_inv_dlog = float(host.photon_inv_dlog)
+        t.e_log_min = float(host.electron_log_e_min)
+        t.e_inv_dlog = float(host.electron_inv_dlog)
+        t.r_log_min = float(host.range_log_r_min)
+        t.r_inv_dlog = float(host.range_inv_dlog)
+        self._ctx.call("hmc_set_tables", nat.ctypes.byref(t))
+        if with_gs:
+            present = np.unique(np.asarray(self.grid.material)).astype(int).tolist()
+            gs = build_gs_grid(*gs_window_from_tables(host, present))
+            values = np.ascontiguousarray(gs.values, dtype=np.float32)
+            self._ctx.call(
+                "hmc_set_gs", nat.ptr(values, nat.ctypes.c_float), int(values.shape[0]),
+                int(values.shape[1]), int(values.shape[2]), int(gs.ix0), int(gs.iy0),
+            )
+        self._uploaded_tables = key
+
+    def _set_scoring(self, scoring, n_columns: int) -> None:
+        mass = np.ascontiguousarray(np.asarray(scoring.voxel_mass, dtype=np.float64))
+        origin = np.asarray(scoring.origin, dtype=np.float64)
+        spacing = np.asarray(scoring.spacing, dtype=np.float64)
+        self._ctx.call(
+            "hmc_set_scoring", int(scoring.shape[0]), int(scoring.shape[1]), int(scoring.shape[2]),
+            nat.ptr(origin, nat.c_double), nat.ptr(spacing, nat.c_double),
+            nat.ptr(mass, nat.c_double), int(n_columns),
+        )
+
+    def _set_physics(self, pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w) -> None:
+        p = nat.Physics()
+        p.pcut = float(pcut)
+        p.ecut = float(ecut)
+        p.step_energy_fraction = float(step)
+        p.deposit_resolution_cm = float(resolution)
+        p.transport_electrons = 1 if transport_electrons else 0
+        p.msc_gs = int(msc_gs)
+        p.dose_to_water = 1 if d2w else 0
+        self._ctx.call("hmc_set_physics", nat.ctypes.byref(p))
+
+    def _set_source(self, source, primary_kind: str, beamlets: range | None = None) -> tuple[bool, float]:
+        """Describe the source to the generator; returns (analytic_emitted, energy)."""
+        d = nat.SourceDesc()
+        keep = []
+        name = _type_name(source)
+        particle = {"photon": 0, "electron": 1, "positron": 2}[primary_kind]
+        d.particle = particle
+        analytic = False
+        energy = 0.0
+
+        def spectrum_arrays(spectrum):
+            edges = np.ascontiguousarray(np.asarray(spectrum.edges, dtype=np.float64))
+            cdf = np.ascontiguousarray(np.asarray(spectrum.cdf, dtype=np.float64))
+            keep.extend([edges, cdf])
+            d.n_bins = int(cdf.size)
+            d.edges = nat.ptr(edges, nat.c_double)
+            d.cdf = nat.ptr(cdf, nat.c_double)
+
+        if name == "PencilBeamSource":
+            d.kind = nat.SRC_PENCIL
+            d.energy = float(source.energy)
+            d.position[:] = [float(c) for c in source.position]
+            d.direction[:] = [float(c) for c in source.direction]
+            analytic, energy = True, float(source.energy)
+        elif name == "ParallelBeamSource":
+            d.kind = nat.SRC_PARALLEL
+            d.energy = float(source.energy)
+            d.z0 = float(source.z)
+            d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
+            d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
+            analytic, energy = True, float(source.energy)
+        elif name == "SpectralParallelBeamSource":
+            d.kind = nat.SRC_PARALLEL_SPECTRAL
+            d.particle = 0
+            d.z0 = float(source.z)
+            d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
+            d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
+            spectrum_arrays(source.spectrum)
+        elif name in ("SpectralBeamSource", "SpectralBeamletSource"):
+            d.kind = nat.SRC_SPECTRAL_FAN
+            d.particle = 0
+            spectrum_arrays(source.spectrum)
+            d.focal[:] = [float(c) for c in source.focal_point]
+            d.u_axis[:] = [float(c) for c in source.u_axis]
+            d.v_axis[:] = [float(c) for c in source.v_axis]
+            d.width_u, d.width_v = float(source.width_u), float(source.width_v)
+            centers = source.centers if name == "SpectralBeamletSource" else (source.center,)
+            sel = list(beamlets) if beamlets is not None else list(range(len(centers)))
+            params = np.ascontiguousarray(np.array([centers[j] for j in sel], dtype=np.float64).reshape(-1, 3))
+            keep.append(params)
+            d.n_beamlets = int(params.shape[0])
+            d.beamlet_params = nat.ptr(params, nat.c_double)
+        elif name == "BeamletGridSource":
+            d.kind = nat.SRC_LATTICE
+            d.particle = 0
+            d.energy = float(source.energy)
+            d.z0 = float(source.z)
+            sel = list(beamlets) if beamlets is not None else list(range(source.n_beamlets))
+            rows = []
+            for j in sel:
+                x_lo, x_hi, y_lo, y_hi = source.beamlet_bounds(j)
+                rows.append((x_lo, x_hi - x_lo, y_lo, y_hi - y_lo))
+            params = np.ascontiguousarray(np.array(rows, dtype=np.float64))
+            keep.append(params)
+            d.n_beamlets = int(params.shape[0])
+            d.beamlet_params = nat.ptr(params, nat.c_double)
+            analytic, energy = True, float(source.energy)
+        else:
+            raise ValueError(
+                f"source type {name} has no in-kernel generator in this engine; supported: "
+                "PencilBeamSource, ParallelBeamSource, SpectralParallelBeamSource, "
+                "SpectralBeamSource, BeamletGridSource, SpectralBeamletSource"
+            )
+        self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
+        return analytic, energy
+
+    def _transport(self, seed: int, items: list[tuple[int, int, int]], chunk: int) -> dict:
+        arr = (nat.WorkItem * len(items))()
+        total = 0
+        for k, (key_base, count, column) in enumerate(items):
+            arr[k].key_base = int(key_base)
+            arr[k].count = int(count)
+            arr[k].column = int(column)
+            total += int(count)
+        ledger = nat.Ledger()
+        self._ctx.call(
+            "hmc_transport", nat.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), arr, len(items),
+            int(chunk), int(self.queue_factor), nat.ctypes.byref(ledger),
+        )
+        return ledger.as_dict()
+
+    def _fold(self, nhist: np.ndarray) -> int:
+        total = nat.c_int64()
+        nh = np.ascontiguousarray(np.asarray(nhist, dtype=np.float64))
+        self._ctx.call("hmc_fold", nat.ptr(nh, nat.c_double), nat.ctypes.byref(total))
+        return int(total.value)
+
+    @staticmethod
+    def _merge(books: dict, ledger: dict) -> None:
+        for k, v in ledger.items():
+            books[k] = books.get(k, 0) + int(v)
+
+    # --------------------------------------------------------------------- run
+
+    def run(
+        self,
+        source,
+        n_histories: int,
+        n_batches: int,
+        seed: int,
+        pcut: float = PCUT_MEV,
+        ecut: float = ECUT_MEV,
+        transport_electrons: bool = True,
+        primary_kind: str = "photon",
+        scoring_grid: ScoringGrid | None = None,
+        scoring_mode: str = "dose_to_medium",
+        step_energy_fraction: float | None = None,
+        deposit_resolution_cm: float | None = None,
+        msc_model: str = "gs",
+        progress=None,
+        concurrent_batches: int = 1,
+    ) -> TransportResult:
+        """Transport ``n_histories`` primaries in ``n_batches`` equal batches."""
+        if n_histories < 1:
+            raise ValueError(f"need at least one history, got {n_histories}")
+        if concurrent_batches < 1:
+            raise ValueError(f"need at least one lane, got concurrent_batches={concurrent_batches}")
+        if n_batches < 1:
+            raise ValueError(f"need at least one batch, got {n_batches}")
+        if n_histories % n_batches != 0:
+            raise ValueError(
+                f"n_histories={n_histories} not divisible by n_batches={n_batches}; "
+                "unequal batches would weight batch means inconsistently"
+            )
+        if primary_kind not in ("photon", "electron"):
+            raise ValueError(f"unknown primary_kind {primary_kind!r}")
+        d2w = _validate_scoring_mode(scoring_mode, transport_electrons)
+        msc_gs = _validate_msc(msc_model)
+        step = default_step_energy_fraction(msc_model) if step_energy_fraction is None else float(step_energy_fraction)
+        if deposit_resolution_cm is not None and deposit_resolution_cm <= 0.0:
+            raise ValueError(f"deposit resolution must be positive, got {deposit_resolution_cm}")
+        resolution = 0.0 if deposit_resolution_cm is None else float(deposit_resolution_cm)
+
+        scoring = scoring_grid if scoring_grid is not None else ScoringGrid.for_grid(self.grid)
+        with self._lock:
+            self._upload_grid()
+        self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
+        self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
+        analytic, mono_energy = self._set_source(source, primary_kind)
+        self._set_scoring(scoring, 1)
+
+        per_batch = n_histories // n_batches
+        chunk = self._chunk(per_batch)
+        emitter = ProgressEmitter(progress, n_histories)
+        books: dict = {}
+        deposited_quanta = 0
+        for b in range(n_batches):
+            ledger = self._transport(seed, [(b * per_batch, per_batch, 0)], chunk)
+            self._merge(books, ledger)
+            deposited_quanta += self._fold(np.array([float(per_batch)]))
+            emitter.tick(per_batch)
+        n_vox = scoring.n_voxels
+        mean = np.empty(n_vox, dtype=np.float64)
+        sigma = np.empty(n_vox, dtype=np.float64)
+        self._ctx.call("hmc_finalize", int(n_batches), nat.ptr(mean, nat.c_double), nat.ptr(sigma, nat.c_double))
+
+        if analytic:
+            particle_rest = 0.0
+            energy_emitted = n_histories * mono_energy + particle_rest
+        else:
+            energy_emitted = float(books.get("emitted", 0)) * ENERGY_QUANTUM_MEV
+        energy_deposited = (
+            float(books.get("deposited", 0)) * ENERGY_QUANTUM_MEV if d2w else float(deposited_quanta) * ENERGY_QUANTUM_MEV
+        )
+        return TransportResult(
+            dose=mean.reshape(scoring.shape),
+            dose_sigma=sigma.reshape(scoring.shape),
+            energy_emitted=energy_emitted,
+            energy_deposited=energy_deposited,
+            energy_escaped=float(books.get("escaped", 0)) * ENERGY_QUANTUM_MEV,
+            energy_unscored=float(books.get("unscored", 0)) * ENERGY_QUANTUM_MEV,
+            n_histories=n_histories,
+            n_batches=n_batches,
+            scoring_mode=scoring_mode,
+            provenance=self._provenance(seed, pcut, ecut, msc_model, step, deposit_resolution_cm),
+            counters=books,
+        )
+
+    # ----------------------------------------------------------------- run_dij
+
+    def run_dij(
+        self,
+        source,
+        n_histories_per_beamlet,
+        n_batches: int,
+        seed: int,
+        pcut: float = PCUT_MEV,
+        ecut: float = ECUT_MEV,
+        transport_electrons: bool = True,
+        truncation: float = DIJ_TRUNCATION_RELATIVE,
+        correlated: bool = True,
+        beamlet_group_size: int | None = None,
+        scoring_grid: ScoringGrid | None = None,
+        scoring_mode: str = "dose_to_medium",
+        step_energy_fraction: float | None = None,
+        deposit_resolution_cm: float | None = None,
+        msc_model: str = "gs",
+        devices=None,
+        concurrent_batches: int = 1,
+        progress=None,
+    ) -> DijResult:
+        """Beamlet-resolved dose matrix over the source's beamlets, CSC by column."""
+        n_beamlets = int(source.n_beamlets)
+        counts = np.asarray(n_histories_per_beamlet, dtype=np.int64)
+        uneven = counts.ndim == 1
+        if uneven:
+            if counts.shape != (n_beamlets,):
+                raise ValueError(f"per-beamlet counts shape {counts.shape} != ({n_beamlets},)")
+        else:
+            counts = np.full(n_beamlets, int(counts), dtype=np.int64)
+        if n_batches < 1:
+            raise ValueError(f"need at least one batch, got {n_batches}")
+        if np.any(counts < 1):
+            raise ValueError(f"need at least one history per beamlet, got {int(counts.min())}")
+        if np.any(counts % n_batches != 0):
+            raise ValueError(
+                f"n_histories_per_beamlet not divisible by n_batches={n_batches}; "
+                "unequal batches would weight batch means inconsistently"
+            )
+        if beamlet_group_size is not None and beamlet_group_size < 1:
+            raise ValueError(f"need a positive beamlet group size, got {beamlet_group_size}")
+        if concurrent_batches < 1:
+            raise ValueError(f"need at least one lane, got concurrent_batches={concurrent_batches}")
+        if truncation < 0.0:
+            raise ValueError(f"negative truncation {truncation}")
+        if devices is not None:
+            wanted = [_device_index(d) for d in devices]
+            if len(set(wanted)) != 1 or wanted[0] != self._index:
+                raise ValueError("this engine runs on one GPU; devices must name only its own device")
+        d2w = _validate_scoring_mode(scoring_mode, transport_electrons)
+        msc_gs = _validate_msc(msc_model)
+        step = default_step_energy_fraction(msc_model) if step_energy_fraction is None else float(step_energy_fraction)
+        if deposit_resolution_cm is not None and deposit_resolution_cm <= 0.0:
+            raise ValueError(f"deposit resolution must be positive, got {deposit_resolution_cm}")
+        resolution = 0.0 if deposit_resolution_cm is None else float(deposit_resolution_cm)
+
+        scoring = scoring_grid if scoring_grid is not None else ScoringGrid.for_grid(self.grid)
+        n_vox = scoring.n_voxels
+        per_batch = counts // n_batches
+        offsets = np.concatenate(([0], np.cumsum(counts)[:-1]))
+        if beamlet_group_size is None:
+            free = int(self._info.get("free_bytes", 0))
+            fit = int(free * _GROUP_MEMORY_FRACTION) // (24 * n_vox) if free > 0 else 128
+            beamlet_group_size = max(1, min(fit, n_beamlets, 4096))
+
+        with self._lock:
+            self._upload_grid()
+        self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
+        self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
+
+        emitter = ProgressEmitter(progress, int(counts.sum()))
+        books: dict = {}
+        deposited_quanta = 0
+        col_counts, col_idx, col_dose, col_sigma = [], [], [], []
+        analytic, mono_energy = False, 0.0
+        for g0 in range(0, n_beamlets, beamlet_group_size):
+            group = range(g0, min(g0 + beamlet_group_size, n_beamlets))
+            analytic, mono_energy = self._set_source(source, "photon", beamlets=group)
+            self._set_scoring(scoring, len(group))
+            group_hist = int(counts[g0:group.stop].sum())
+            chunk = self._chunk(int(per_batch[g0:group.stop].sum()))
+            for b in range(n_batches):
+                items = []
+                for j in group:
+                    r0 = b * int(per_batch[j])
+                    key = r0 if correlated else int(offsets[j]) + r0
+                    items.append((key, int(per_batch[j]), j - g0))
+                ledger = self._transport(seed, items, chunk)
+                self._merge(books, ledger)
+                deposited_quanta += self._fold(per_batch[g0:group.stop].astype(np.float64))
+            self._ctx.call("hmc_finalize", int(n_batches), None, None)
+            kept = np.zeros(len(group), dtype=np.int64)
+            total = nat.c_int64()
+            self._ctx.call("hmc_truncate", float(truncation), nat.ptr(kept, nat.c_int64), nat.ctypes.byref(total))
+            m = int(total.value)
+            idx = np.empty(m, dtype=np.int64)
+            dose = np.empty(m, dtype=np.float64)
+            sig = np.empty(m, dtype=np.float64)
+            if m:
+                self._ctx.call("hmc_fetch_csc", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.c_double), nat.ptr(sig, nat.c_double))
+            col_counts.append(kept)
+            col_idx.append(idx)
+            col_dose.append(dose)
+            col_sigma.append(sig)
+            emitter.tick(group_hist)
+
+        indptr = np.zeros(n_beamlets + 1, dtype=np.int64)
+        np.cumsum(np.concatenate(col_counts), out=indptr[1:])
+        if analytic:
+            energy_emitted = float(counts.sum()) * mono_energy
+        else:
+            energy_emitted = float(books.get("emitted", 0)) * ENERGY_QUANTUM_MEV
+        energy_deposited = (
+            float(books.get("deposited", 0)) * ENERGY_QUANTUM_MEV if d2w else float(deposited_quanta) * ENERGY_QUANTUM_MEV
+        )
+        return DijResult(
+            grid_shape=tuple(scoring.shape),
+            n_beamlets=n_beamlets,
+            n_histories_per_beamlet=counts.copy() if uneven else int(counts[0]),
+            n_batches=n_batches,
+            truncation=float(truncation),
+            indptr=indptr,
+            indices=np.concatenate(col_idx) if col_idx else np.zeros(0, np.int64),
+            dose=np.concatenate(col_dose) if col_dose else np.zeros(0),
+            sigma=np.concatenate(col_sigma) if col_sigma else np.zeros(0),
+            energy_emitted=energy_emitted,
+            energy_deposited=energy_deposited,
+            energy_escaped=float(books.get("escaped", 0)) * ENERGY_QUANTUM_MEV,
+            energy_unscored=float(books.get("unscored", 0)) * ENERGY_QUANTUM_MEV,
+            correlated=bool(correlated),
+            scoring_mode=scoring_mode,
+            provenance=self._provenance(seed, pcut, ecut, msc_model, step, deposit_resolution_cm),
+            counters=books,
+        )
