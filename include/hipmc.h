@@ -73,20 +73,27 @@ typedef struct hmc_physics {
   int transport_electrons, msc_gs, dose_to_water, reserved;
 } hmc_physics;
 
-/* One contiguous run of histories feeding one column: history key of the i-th
- * primary is key_base + i; column is the group-local Dij column (0 for run()). */
+/* One contiguous run of histories feeding one tally column: the history key of the
+ * i-th primary is key_base + i. column = batch_slot * n_columns + group-local column;
+ * beamlet indexes the source's per-beamlet parameters (0 for open-field sources). */
 typedef struct hmc_work_item {
   int64_t key_base;
   int64_t count;
   int32_t column;
-  int32_t reserved;
+  int32_t beamlet;
 } hmc_work_item;
 
-/* Integer books of one hmc_transport call (quanta of 1e-9 MeV) and event counts. */
+/* Integer books of one hmc_transport call (quanta of 1e-9 MeV), per-kernel event
+ * counts (the algorithmic-bytes denominators), and per-kernel HIP-event times. */
 typedef struct hmc_ledger {
   int64_t escaped, unscored, deposited, emitted;
-  int64_t voxel_fetches, deposits, pushes, photon_steps, electron_steps;
   int64_t violations, trapped, overflow, rounds;
+  /* photon round, then electron round: voxel fetches, tally atomics, queue pushes,
+   * particles popped, steps (Woodcock flights / condensed substeps) */
+  int64_t photon_voxels, photon_deposits, photon_pushes, photon_pops, photon_steps;
+  int64_t electron_voxels, electron_deposits, electron_pushes, electron_pops, electron_steps;
+  int64_t photon_launches, electron_launches;
+  double photon_kernel_ms, electron_kernel_ms; /* only with hmc_set_profiling(ctx, 1) */
 } hmc_ledger;
 
 /* Lifetime ------------------------------------------------------------------- */
@@ -102,9 +109,12 @@ int hmc_device_info(int device, char *name, int name_len, int *cu_count, int64_t
 /* WarpEngine._upload_grid (engine.py:3336-3366). */
 int hmc_set_grid(hmc_ctx *ctx, int nx, int ny, int nz, const double *origin, const double *spacing,
                  const float *density, const uint8_t *material);
-/* _scoring_info + dense per-column maps (engine.py:287-330, 1413-1426). */
+/* _scoring_info + dense per-column maps (engine.py:287-330, 1413-1426). The int64
+ * quanta maps hold n_slots batches side by side ([n_slots][n_columns][voxels]) so a
+ * block of batches transports in one call; the float64 sums are [n_columns][voxels]. */
 int hmc_set_scoring(hmc_ctx *ctx, int nx, int ny, int nz, const double *origin,
-                    const double *spacing, const double *voxel_mass, int64_t n_columns);
+                    const double *spacing, const double *voxel_mass, int64_t n_columns,
+                    int n_slots);
 /* WarpEngine._upload_tables (engine.py:3368-3429). */
 int hmc_set_tables(hmc_ctx *ctx, const hmc_tables *tables);
 /* GS grid upload (engine.py:3418-3425, goudsmit_saunderson.py:789-823). */
@@ -117,6 +127,9 @@ int hmc_set_uploaded(hmc_ctx *ctx, int64_t n, const int32_t *particle, const flo
                      const float *x, const float *y, const float *z, const float *ux,
                      const float *uy, const float *uz, const float *weight);
 int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
+/* Time every transport launch with HIP events on the context's stream (no
+ * reference counterpart: the measurement hook for the per-kernel roofline). */
+int hmc_set_profiling(hmc_ctx *ctx, int enabled);
 
 /* Work ----------------------------------------------------------------------- */
 /* Generate and transport every primary of the work items into the dense tallies:
@@ -124,10 +137,11 @@ int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
  * the histories resident at once; queue_factor sizes the secondary queues. */
 int hmc_transport(hmc_ctx *ctx, uint64_t seed, const hmc_work_item *items, int n_items,
                   int64_t chunk, int queue_factor, hmc_ledger *ledger);
-/* Fold one batch into the running f64 sums, consuming the map: accumulate_run_batch /
- * accumulate_dij_batch (kernels.py:804-878). nhist[column] = histories per column in
- * this batch. total_quanta receives the folded quanta. */
-int hmc_fold(hmc_ctx *ctx, const double *nhist, int64_t *total_quanta);
+/* Fold batch slots 0..n_slots_used-1, in slot order, into the running f64 sums and
+ * consume the maps: accumulate_run_batch / accumulate_dij_batch (kernels.py:804-878).
+ * nhist[slot * n_columns + column] = histories of that column in that batch.
+ * total_quanta receives the folded quanta. */
+int hmc_fold(hmc_ctx *ctx, int n_slots_used, const double *nhist, int64_t *total_quanta);
 /* Mean and sigma per (column, voxel): finalize_run (kernels.py:881-909). With
  * mean/sigma NULL the result stays on the device for hmc_truncate. */
 int hmc_finalize(hmc_ctx *ctx, int n_batches, double *mean, double *sigma);
@@ -138,6 +152,11 @@ int hmc_truncate(hmc_ctx *ctx, double truncation, int64_t *counts, int64_t *tota
 int hmc_fetch_csc(hmc_ctx *ctx, int64_t *indices, double *dose, double *sigma);
 /* Clear the dense maps and sums for a new group. */
 int hmc_reset_maps(hmc_ctx *ctx);
+/* Replace / read the dense int64 quanta maps [n_slots][n_columns][n_voxels]: the synthetic
+ * fixed-point maps the reference drives its device reduction with
+ * (tests/integration/test_warp_device_reduction.py:38-62). */
+int hmc_set_quanta(hmc_ctx *ctx, const int64_t *quanta);
+int hmc_get_quanta(hmc_ctx *ctx, int64_t *quanta);
 
 /* Host-side Goudsmit-Saunderson node grid (build_gs_grid, goudsmit_saunderson.py:
  * 1108-1219): fills out[(ix1-ix0+1)][(iy1-iy0+1)][n_u] with scaled-deflection tables

@@ -103,31 +103,22 @@ class Physics(Structure):
 
 
 class WorkItem(Structure):
-    _fields_ = [("key_base", c_int64), ("count", c_int64), ("column", c_int32), ("reserved", c_int32)]
+    _fields_ = [("key_base", c_int64), ("count", c_int64), ("column", c_int32), ("beamlet", c_int32)]
 
 
 class Ledger(Structure):
     _fields_ = [
         (name, c_int64)
         for name in (
-            "escaped",
-            "unscored",
-            "deposited",
-            "emitted",
-            "voxel_fetches",
-            "deposits",
-            "pushes",
-            "photon_steps",
-            "electron_steps",
-            "violations",
-            "trapped",
-            "overflow",
-            "rounds",
+            "escaped", "unscored", "deposited", "emitted", "violations", "trapped", "overflow",
+            "rounds", "photon_voxels", "photon_deposits", "photon_pushes", "photon_pops",
+            "photon_steps", "electron_voxels", "electron_deposits", "electron_pushes",
+            "electron_pops", "electron_steps", "photon_launches", "electron_launches",
         )
-    ]
+    ] + [("photon_kernel_ms", c_double), ("electron_kernel_ms", c_double)]
 
-    def as_dict(self) -> dict[str, int]:
-        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -151,18 +142,21 @@ def lib():
         "hmc_last_error": (c_char_p, [ctx_p]),
         "hmc_device_info": (c_int, [c_int, c_char_p, c_int, POINTER(c_int), POINTER(c_int64), POINTER(c_int64)]),
         "hmc_set_grid": (c_int, [ctx_p, c_int, c_int, c_int, _f64p, _f64p, _f32p, POINTER(c_uint8)]),
-        "hmc_set_scoring": (c_int, [ctx_p, c_int, c_int, c_int, _f64p, _f64p, _f64p, c_int64]),
+        "hmc_set_scoring": (c_int, [ctx_p, c_int, c_int, c_int, _f64p, _f64p, _f64p, c_int64, c_int]),
         "hmc_set_tables": (c_int, [ctx_p, POINTER(Tables)]),
         "hmc_set_gs": (c_int, [ctx_p, _f32p, c_int, c_int, c_int, c_int, c_int]),
         "hmc_set_source": (c_int, [ctx_p, POINTER(SourceDesc)]),
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
+        "hmc_set_profiling": (c_int, [ctx_p, c_int]),
         "hmc_transport": (c_int, [ctx_p, c_uint64, POINTER(WorkItem), c_int, c_int64, c_int, POINTER(Ledger)]),
-        "hmc_fold": (c_int, [ctx_p, _f64p, POINTER(c_int64)]),
+        "hmc_fold": (c_int, [ctx_p, c_int, _f64p, POINTER(c_int64)]),
         "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),
         "hmc_truncate": (c_int, [ctx_p, c_double, POINTER(c_int64), POINTER(c_int64)]),
         "hmc_fetch_csc": (c_int, [ctx_p, POINTER(c_int64), _f64p, _f64p]),
         "hmc_reset_maps": (c_int, [ctx_p]),
+        "hmc_set_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
+        "hmc_get_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_build_gs_grid": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, _f64p]),
     }
     for name, (restype, argtypes) in sig.items():

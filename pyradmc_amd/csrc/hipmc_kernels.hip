@@ -60,13 +60,20 @@ enum LedgerSlot {
   L_UNSCORED,
   L_DEPOSITED,
   L_EMITTED,
-  L_VOX,
-  L_DEP,
-  L_PUSH,
-  L_PSTEPS,
-  L_ESTEPS,
   L_VIOLATIONS,
   L_TRAPPED,
+  // per-kernel event counts, photon round then electron round: voxel fetches,
+  // tally atomics, queue pushes, particles popped, steps
+  L_P_VOX,
+  L_P_DEP,
+  L_P_PUSH,
+  L_P_POP,
+  L_P_STEPS,
+  L_E_VOX,
+  L_E_DEP,
+  L_E_PUSH,
+  L_E_POP,
+  L_E_STEPS,
   L_COUNT
 };
 
@@ -82,19 +89,36 @@ struct TransportParams {
 
 struct LaneTally {
   long long escaped = 0, unscored = 0, deposited = 0;
-  unsigned long long vox = 0, dep = 0, push = 0, steps = 0;
+  unsigned long long vox = 0, dep = 0, push = 0, pop = 0, steps = 0;
   unsigned long long violations = 0, trapped = 0;
+  // One pending tally cell per lane: consecutive deposits into the same cell are
+  // summed here and written with one atomic when the cell changes (integer
+  // addition is associative, so the maps are unchanged bit for bit).
+  long long pend_cell = -1;
+  long long pend_q = 0;
 
-  __device__ void flush(unsigned long long* ledger, int step_slot) const {
+  __device__ void flush_pending(long long* edep) {
+    if (pend_q != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(edep + pend_cell),
+                static_cast<unsigned long long>(pend_q));
+      ++dep;
+    }
+    pend_q = 0;
+    pend_cell = -1;
+  }
+
+  // base = L_P_VOX for the photon round, L_E_VOX for the electron round.
+  __device__ void flush(unsigned long long* ledger, int base) const {
     if (escaped) atomicAdd(ledger + L_ESCAPED, static_cast<unsigned long long>(escaped));
     if (unscored) atomicAdd(ledger + L_UNSCORED, static_cast<unsigned long long>(unscored));
     if (deposited) atomicAdd(ledger + L_DEPOSITED, static_cast<unsigned long long>(deposited));
-    if (vox) atomicAdd(ledger + L_VOX, vox);
-    if (dep) atomicAdd(ledger + L_DEP, dep);
-    if (push) atomicAdd(ledger + L_PUSH, push);
-    if (steps) atomicAdd(ledger + step_slot, steps);
     if (violations) atomicAdd(ledger + L_VIOLATIONS, violations);
     if (trapped) atomicAdd(ledger + L_TRAPPED, trapped);
+    if (vox) atomicAdd(ledger + base + 0, vox);
+    if (dep) atomicAdd(ledger + base + 1, dep);
+    if (push) atomicAdd(ledger + base + 2, push);
+    if (pop) atomicAdd(ledger + base + 3, pop);
+    if (steps) atomicAdd(ledger + base + 4, steps);
   }
 };
 
@@ -129,9 +153,12 @@ __device__ inline void deposit(const TransportParams& P, LaneTally& tal, long lo
     const long long q = static_cast<long long>(static_cast<double>(amount) *
                                                    static_cast<double>(factor) * kInvQuantum +
                                                0.5);
-    if (q != 0) atomicAdd(reinterpret_cast<unsigned long long*>(P.edep + base + idx),
-                          static_cast<unsigned long long>(q));
-    tal.dep++;
+    const long long cell = base + idx;
+    if (cell != tal.pend_cell) {
+      tal.flush_pending(P.edep);
+      tal.pend_cell = cell;
+    }
+    tal.pend_q += q;
     if (P.dose_to_water)
       tal.deposited += static_cast<long long>(static_cast<double>(amount) * kInvQuantum + 0.5);
   } else {
@@ -249,6 +276,7 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      tal.pop++;
       column = meta >> 3;
       base = static_cast<long long>(column) * P.s.n_voxels;
       if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
@@ -380,7 +408,8 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
       active = false;
     }
   }
-  tal.flush(P.ledger, L_PSTEPS);
+  tal.flush_pending(P.edep);
+  tal.flush(P.ledger, L_P_VOX);
 }
 
 // --------------------------------------------------------------- electron round
@@ -408,6 +437,7 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      tal.pop++;
       column = meta >> 3;
       positron = (meta & 3u) == POSITRON;
       base = static_cast<long long>(column) * P.s.n_voxels;
@@ -574,7 +604,8 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
       e -= delta;
     }
   }
-  tal.flush(P.ledger, L_ESTEPS);
+  tal.flush_pending(P.edep);
+  tal.flush(P.ledger, L_E_VOX);
 }
 
 // -------------------------------------------------------------------- generate
@@ -598,8 +629,8 @@ struct DevSource {
 struct DevItem {
   long long key_base;
   long long start;  // exclusive prefix of counts
-  int column;
-  int pad;
+  int column;       // flat tally column (batch slot * columns + column)
+  int beamlet;      // index into the source's per-beamlet parameters
 };
 
 // Spectrum inversion: first cdf entry >= u, clamped to the top bin
@@ -618,13 +649,38 @@ __device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
   return a + (S.edges[k + 1] - a) * u_within;
 }
 
+// Slot reservation for the lanes of one wave that want one: a single atomic per
+// wave, ranks by lane order. Must be reached by every active lane of the wave.
+__device__ inline unsigned int wave_reserve(unsigned int* counter, bool want) {
+  const unsigned long long mask = __ballot(want ? 1 : 0);
+  if (mask == 0ull) return 0u;
+  const int lane = static_cast<int>(__lane_id());
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned int base = 0u;
+  if (lane == leader) base = atomicAdd(counter, static_cast<unsigned int>(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ inline long long wave_sum(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Primaries [p0, p0 + n_gen) of the flattened work items. Single-kind sources write
+// fixed slots of the target queue (thread t -> slot t; the count is set once);
+// host-sampled mixed-kind primaries reserve slots per wave in the queue of their
+// kind (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
 __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* items, int n_items,
                                                   long long p0, long long n_gen, uint64_t seed,
                                                   Queue qp, Queue qe,
                                                   unsigned long long* ledger) {
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = t < n_gen;
   long long emitted = 0;
-  if (t < n_gen) {
+  Particle p;
+  uint32_t kind = static_cast<uint32_t>(S.particle);
+  if (live) {
     const long long g = p0 + t;
     int lo = 0, hi = n_items;  // last item with start <= g
     while (hi - lo > 1) {
@@ -636,7 +692,6 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
     const long long i = g - item.start;
     const uint32_t column = static_cast<uint32_t>(item.column);
     uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
-    uint32_t kind = static_cast<uint32_t>(S.particle);
     float e = S.energy, w = 1.0f;
     float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
     bool book = false;
@@ -659,7 +714,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
         book = true;
         break;
       case HMC_SRC_LATTICE: {
-        const float* b = S.beamlet + 4 * static_cast<long long>(column);
+        const float* b = S.beamlet + 4 * static_cast<long long>(item.beamlet);
         x = b[0] + b[1] * uniform(rng);
         y = b[2] + b[3] * uniform(rng);
         z = S.z0;
@@ -670,7 +725,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
         e = sample_spectrum(S, rng);
         const float du = (uniform(rng) - 0.5f) * S.width_u;
         const float dv = (uniform(rng) - 0.5f) * S.width_v;
-        const float* c = S.beamlet + 3 * static_cast<long long>(column);
+        const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
         const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
         const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
         const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
@@ -681,7 +736,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
         book = true;
         break;
       }
-      default: {  // uploaded
+      default: {  // host-sampled primaries, indexed by the flat primary index
         const long long j = g;
         kind = static_cast<uint32_t>(S.up_particle[j]);
         e = S.up[j];
@@ -700,19 +755,28 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
       const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
       emitted = quanta_round(static_cast<double>(w * (e + latent)));
     }
-    const Queue& q = kind == PHOTON ? qp : qe;
-    const unsigned int idx = atomicAdd(q.count, 1u);
-    if (idx < q.capacity) {
-      Particle p;
-      p.x = x; p.y = y; p.z = z; p.e = e;
-      p.ux = ux; p.uy = uy; p.uz = uz; p.w = w;
-      p.rng = rng;
-      p.meta = pack_meta(kind, 1u, column);
-      p.pad = 0u;
-      q.data[idx] = p;
-    }
+    p.x = x; p.y = y; p.z = z; p.e = e;
+    p.ux = ux; p.uy = uy; p.uz = uz; p.w = w;
+    p.rng = rng;
+    p.meta = pack_meta(kind, 1u, column);
+    p.pad = 0u;
   }
-  if (emitted) atomicAdd(ledger + L_EMITTED, static_cast<unsigned long long>(emitted));
+  if (S.kind != HMC_SRC_UPLOADED) {
+    // One kind per launch: fixed slots, no atomics.
+    const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
+    if (live && t < q.capacity) q.data[t] = p;
+    if (t == 0) *q.count = static_cast<unsigned int>(n_gen);
+  } else {
+    const bool to_p = live && kind == PHOTON;
+    const bool to_e = live && kind != PHOTON;
+    const unsigned int sp = wave_reserve(qp.count, to_p);
+    const unsigned int se = wave_reserve(qe.count, to_e);
+    if (to_p && sp < qp.capacity) qp.data[sp] = p;
+    if (to_e && se < qe.capacity) qe.data[se] = p;
+  }
+  const long long total = wave_sum(emitted);
+  if (__lane_id() == 0 && total != 0)
+    atomicAdd(ledger + L_EMITTED, static_cast<unsigned long long>(total));
 }
 
 __global__ void k_fill_u32(unsigned int* a, int n, unsigned int v) {
@@ -735,27 +799,41 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 
 __global__ void __launch_bounds__(256) k_fold(long long* edep, const double* mass,
                                               const double* nhist, long long n_vox,
-                                              long long total, double* s1, double* s2,
-                                              unsigned long long* folded) {
+                                              long long n_cols, int n_slots, double* s1,
+                                              double* s2, unsigned long long* folded) {
+  const long long total = n_vox * n_cols;
+  const long long slot_stride = total;
   long long local = 0;
   for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const long long q = edep[i];
-    if (q == 0) continue;
-    edep[i] = 0;
     const long long col = i / n_vox;
     const long long v = i - col * n_vox;
     const double m = mass[v];
-    double d = 0.0;
-    if (m > 0.0) {
-      const double energy = static_cast<double>(q) * 1.0e-9;
-      d = energy / (m * nhist[col]);
+    double a = s1[i];
+    double b = s2[i];
+    bool touched = false;
+    for (int slot = 0; slot < n_slots; ++slot) {  // batch order
+      const long long j = slot * slot_stride + i;
+      const long long q = edep[j];
+      if (q == 0) continue;
+      edep[j] = 0;
+      double d = 0.0;
+      if (m > 0.0) {
+        const double energy = static_cast<double>(q) * 1.0e-9;
+        d = energy / (m * nhist[slot * n_cols + col]);
+      }
+      a = a + d;
+      b = b + d * d;
+      touched = true;
+      local += q;
     }
-    s1[i] = s1[i] + d;
-    s2[i] = s2[i] + d * d;
-    local += q;
+    if (touched) {
+      s1[i] = a;
+      s2[i] = b;
+    }
   }
-  if (local) atomicAdd(folded, static_cast<unsigned long long>(local));
+  const long long sum = wave_sum(local);
+  if (__lane_id() == 0 && sum != 0) atomicAdd(folded, static_cast<unsigned long long>(sum));
 }
 
 __global__ void __launch_bounds__(256) k_finalize(double* s1, double* s2, long long total,
@@ -908,6 +986,7 @@ struct hmc_ctx {
   DevScoring s{};
   double* d_mass = nullptr;
   long long n_columns = 0;
+  int n_slots = 1;
   long long* d_edep = nullptr;
   double* d_s1 = nullptr;
   double* d_s2 = nullptr;
@@ -929,6 +1008,7 @@ struct hmc_ctx {
   float* d_beamlet = nullptr;
   int32_t* d_up_particle = nullptr;
   float* d_up = nullptr;
+  int src_beamlets = 0;
   bool has_source = false;
 
   hmc_physics phys{};
@@ -941,6 +1021,8 @@ struct hmc_ctx {
   unsigned long long* d_ledger = nullptr;
   DevItem* d_items = nullptr;
   size_t items_cap = 0;
+  bool profile = false;
+  hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 
   // truncation
   unsigned long long* d_colmax = nullptr;
@@ -1047,6 +1129,9 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
+  for (auto& pair : ctx->ev)
+    for (auto& e : pair)
+      if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1109,19 +1194,21 @@ int hmc_set_grid(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
 }
 
 int hmc_set_scoring(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
-                    const double* spacing, const double* voxel_mass, int64_t n_columns) {
+                    const double* spacing, const double* voxel_mass, int64_t n_columns,
+                    int n_slots) {
   if (!ctx || !origin || !spacing || !voxel_mass) return fail(ctx, HMC_EINVAL, "null");
-  if (nx < 1 || ny < 1 || nz < 1 || n_columns < 1 || n_columns >= (1ll << 28))
-    return fail(ctx, HMC_EINVAL, "bad scoring shape or column count");
+  if (nx < 1 || ny < 1 || nz < 1 || n_columns < 1 || n_slots < 1 ||
+      n_columns * n_slots >= (1ll << 28))
+    return fail(ctx, HMC_EINVAL, "bad scoring shape, column or slot count");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   const long long nv = static_cast<long long>(nx) * ny * nz;
   int rc = upload(ctx, &ctx->d_mass, voxel_mass, static_cast<size_t>(nv));
   if (rc) return rc;
   const size_t total = static_cast<size_t>(nv) * static_cast<size_t>(n_columns);
-  if ((rc = dev_alloc(ctx, &ctx->d_edep, total))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_edep, total * static_cast<size_t>(n_slots)))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_s1, total))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_s2, total))) return rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_nhist, static_cast<size_t>(n_columns)))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_nhist, static_cast<size_t>(n_columns) * n_slots))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_colmax, static_cast<size_t>(n_columns)))) return rc;
   DevScoring& s = ctx->s;
   s.x_lo = static_cast<float>(origin[0]);
@@ -1138,6 +1225,7 @@ int hmc_set_scoring(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
   s.nz = nz;
   s.n_voxels = nv;
   ctx->n_columns = n_columns;
+  ctx->n_slots = n_slots;
   ctx->has_scoring = true;
   return hmc_reset_maps(ctx);
 }
@@ -1147,8 +1235,9 @@ int hmc_reset_maps(hmc_ctx* ctx) {
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   const long long total = ctx->s.n_voxels * ctx->n_columns;
   const int blocks = blocks_for(total, 256, 65536);
-  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
-                     reinterpret_cast<unsigned long long*>(ctx->d_edep), total, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(total * ctx->n_slots, 256, 65536)), dim3(256), 0,
+                     ctx->stream, reinterpret_cast<unsigned long long*>(ctx->d_edep),
+                     total * ctx->n_slots, 0ull);
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(ctx->d_s1), total, 0ull);
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
@@ -1156,6 +1245,25 @@ int hmc_reset_maps(hmc_ctx* ctx) {
   HMC_HIP(ctx, hipGetLastError());
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->finalized = false;
+  return HMC_OK;
+}
+
+int hmc_set_quanta(hmc_ctx* ctx, const int64_t* quanta) {
+  if (!ctx || !quanta) return fail(ctx, HMC_EINVAL, "null");
+  if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t total = static_cast<size_t>(ctx->s.n_voxels * ctx->n_columns * ctx->n_slots);
+  HMC_HIP(ctx, hipMemcpy(ctx->d_edep, quanta, sizeof(long long) * total, hipMemcpyHostToDevice));
+  return HMC_OK;
+}
+
+int hmc_get_quanta(hmc_ctx* ctx, int64_t* quanta) {
+  if (!ctx || !quanta) return fail(ctx, HMC_EINVAL, "null");
+  if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t total = static_cast<size_t>(ctx->s.n_voxels * ctx->n_columns * ctx->n_slots);
+  HMC_HIP(ctx, hipMemcpy(quanta, ctx->d_edep, sizeof(long long) * total, hipMemcpyDeviceToHost));
   return HMC_OK;
 }
 
@@ -1283,6 +1391,7 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
     std::vector<float> b(s->beamlet_params, s->beamlet_params + s->n_beamlets * stride);
     if ((rc = upload(ctx, &ctx->d_beamlet, b.data(), b.size()))) return rc;
     d.beamlet = ctx->d_beamlet;
+    ctx->src_beamlets = s->n_beamlets;
   }
   ctx->has_source = true;
   return HMC_OK;
@@ -1317,6 +1426,17 @@ int hmc_set_physics(hmc_ctx* ctx, const hmc_physics* p) {
   return HMC_OK;
 }
 
+int hmc_set_profiling(hmc_ctx* ctx, int enabled) {
+  if (!ctx) return HMC_EINVAL;
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  if (enabled && !ctx->ev[0][0]) {
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j) HMC_HIP(ctx, hipEventCreate(&ctx->ev[k][j]));
+  }
+  ctx->profile = enabled != 0;
+  return HMC_OK;
+}
+
 int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n_items,
                   int64_t chunk, int queue_factor, hmc_ledger* ledger) {
   if (!ctx || !items || n_items < 1 || chunk < 1 || queue_factor < 1 || !ledger)
@@ -1334,20 +1454,20 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   std::vector<DevItem> dev_items(static_cast<size_t>(n_items));
   long long total = 0;
   for (int i = 0; i < n_items; ++i) {
-    if (items[i].count < 0 || items[i].column < 0 || items[i].column >= ctx->n_columns)
+    if (items[i].count < 0 || items[i].column < 0 ||
+        items[i].column >= ctx->n_columns * ctx->n_slots)
       return fail(ctx, HMC_EINVAL, "work item count/column out of range");
+    const bool per_beamlet = ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN;
+    if (items[i].beamlet < 0 || (per_beamlet && items[i].beamlet >= ctx->src_beamlets))
+      return fail(ctx, HMC_EINVAL, "work item beamlet out of range");
     dev_items[i].key_base = items[i].key_base;
     dev_items[i].start = total;
     dev_items[i].column = items[i].column;
-    dev_items[i].pad = 0;
+    dev_items[i].beamlet = items[i].beamlet;
     total += items[i].count;
   }
   if (ctx->src.kind == HMC_SRC_UPLOADED && total > ctx->src.up_n)
     return fail(ctx, HMC_EINVAL, "more histories than uploaded primaries");
-  if (ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN) {
-    for (int i = 0; i < n_items; ++i)
-      if (items[i].column >= static_cast<int>(0x7fffffff)) return fail(ctx, HMC_EINVAL, "column");
-  }
   int rc;
   if (static_cast<size_t>(n_items) > ctx->items_cap) {
     if ((rc = dev_alloc(ctx, &ctx->d_items, static_cast<size_t>(n_items)))) return rc;
@@ -1385,7 +1505,17 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
 
   const unsigned int cap = static_cast<unsigned int>(ctx->q_capacity);
   const long long max_lanes = static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048;
-  long long overflow = 0, rounds = 0;
+  long long overflow = 0, rounds = 0, launches[2] = {0, 0};
+  double kernel_ms[2] = {0.0, 0.0};
+  bool pending[2] = {false, false};
+  auto harvest = [&](int k) -> int {  // after a stream sync
+    if (!pending[k]) return HMC_OK;
+    float ms = 0.0f;
+    HMC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[k][0], ctx->ev[k][1]));
+    kernel_ms[k] += ms;
+    pending[k] = false;
+    return HMC_OK;
+  };
   unsigned int* counts = ctx->d_counts;
   unsigned int* heads = ctx->d_counts + 4;
 
@@ -1403,6 +1533,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts),
                                   hipMemcpyDeviceToHost, ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if ((rc = harvest(1))) return rc;
       for (int q = 0; q < 4; ++q)
         if (host_counts[q] > cap) overflow = std::max<long long>(overflow, host_counts[q]);
       if (overflow) break;
@@ -1414,24 +1545,37 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       if (n_p) {
         // Photon round: consume pa; electrons append to ea, photons to pb.
         const long long lanes = std::min<long long>(n_p, max_lanes);
+        if (ctx->profile) HMC_HIP(ctx, hipEventRecord(ctx->ev[0][0], ctx->stream));
         hipLaunchKernelGGL(k_photon_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream, P,
                            qa_p, heads + pa, qa_e, qb_p);
         HMC_HIP(ctx, hipGetLastError());
+        if (ctx->profile) {
+          HMC_HIP(ctx, hipEventRecord(ctx->ev[0][1], ctx->stream));
+          pending[0] = true;
+        }
+        ++launches[0];
       }
       // Electron round: consume ea (including the photon round's pushes).
       unsigned int n_e_now = 0;
       HMC_HIP(ctx, hipMemcpyAsync(&n_e_now, counts + ea, sizeof(unsigned int),
                                   hipMemcpyDeviceToHost, ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if ((rc = harvest(0))) return rc;
       if (n_e_now > cap) {
         overflow = n_e_now;
         break;
       }
       if (n_e_now) {
         const long long lanes = std::min<long long>(n_e_now, max_lanes);
+        if (ctx->profile) HMC_HIP(ctx, hipEventRecord(ctx->ev[1][0], ctx->stream));
         hipLaunchKernelGGL(k_electron_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream,
                            P, qa_e, heads + ea, qb_p, qb_e);
         HMC_HIP(ctx, hipGetLastError());
+        if (ctx->profile) {
+          HMC_HIP(ctx, hipEventRecord(ctx->ev[1][1], ctx->stream));
+          pending[1] = true;
+        }
+        ++launches[1];
       }
       // Reset the consumed queues (counts and heads), then swap roles.
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + pa, 1, 0u);
@@ -1447,19 +1591,30 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger),
                               hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  ledger->escaped = static_cast<int64_t>(host_ledger[L_ESCAPED]);
-  ledger->unscored = static_cast<int64_t>(host_ledger[L_UNSCORED]);
-  ledger->deposited = static_cast<int64_t>(host_ledger[L_DEPOSITED]);
-  ledger->emitted = static_cast<int64_t>(host_ledger[L_EMITTED]);
-  ledger->voxel_fetches = static_cast<int64_t>(host_ledger[L_VOX]);
-  ledger->deposits = static_cast<int64_t>(host_ledger[L_DEP]);
-  ledger->pushes = static_cast<int64_t>(host_ledger[L_PUSH]);
-  ledger->photon_steps = static_cast<int64_t>(host_ledger[L_PSTEPS]);
-  ledger->electron_steps = static_cast<int64_t>(host_ledger[L_ESTEPS]);
-  ledger->violations = static_cast<int64_t>(host_ledger[L_VIOLATIONS]);
-  ledger->trapped = static_cast<int64_t>(host_ledger[L_TRAPPED]);
+  if ((rc = harvest(0)) || (rc = harvest(1))) return rc;
+  auto L = [&](int slot) { return static_cast<int64_t>(host_ledger[slot]); };
+  ledger->escaped = L(L_ESCAPED);
+  ledger->unscored = L(L_UNSCORED);
+  ledger->deposited = L(L_DEPOSITED);
+  ledger->emitted = L(L_EMITTED);
+  ledger->violations = L(L_VIOLATIONS);
+  ledger->trapped = L(L_TRAPPED);
   ledger->overflow = overflow;
   ledger->rounds = rounds;
+  ledger->photon_voxels = L(L_P_VOX);
+  ledger->photon_deposits = L(L_P_DEP);
+  ledger->photon_pushes = L(L_P_PUSH);
+  ledger->photon_pops = L(L_P_POP);
+  ledger->photon_steps = L(L_P_STEPS);
+  ledger->electron_voxels = L(L_E_VOX);
+  ledger->electron_deposits = L(L_E_DEP);
+  ledger->electron_pushes = L(L_E_PUSH);
+  ledger->electron_pops = L(L_E_POP);
+  ledger->electron_steps = L(L_E_STEPS);
+  ledger->photon_launches = launches[0];
+  ledger->electron_launches = launches[1];
+  ledger->photon_kernel_ms = kernel_ms[0];
+  ledger->electron_kernel_ms = kernel_ms[1];
   if (overflow)
     return fail(ctx, HMC_EOVERFLOW,
                 "secondary queue overflow (" + std::to_string(overflow) + " > capacity " +
@@ -1471,21 +1626,23 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   return HMC_OK;
 }
 
-int hmc_fold(hmc_ctx* ctx, const double* nhist, int64_t* total_quanta) {
+int hmc_fold(hmc_ctx* ctx, int n_slots_used, const double* nhist, int64_t* total_quanta) {
   if (!ctx || !nhist) return fail(ctx, HMC_EINVAL, "null");
   if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
-  for (long long c = 0; c < ctx->n_columns; ++c)
+  if (n_slots_used < 1 || n_slots_used > ctx->n_slots)
+    return fail(ctx, HMC_EINVAL, "slot count out of range");
+  const long long n_nh = ctx->n_columns * n_slots_used;
+  for (long long c = 0; c < n_nh; ++c)
     if (!(nhist[c] > 0.0)) return fail(ctx, HMC_EINVAL, "empty batch for a column");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
-  HMC_HIP(ctx, hipMemcpyAsync(ctx->d_nhist, nhist, sizeof(double) * ctx->n_columns,
-                              hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ledger + L_COUNT,
-                     1ll, 0ull);
-  const long long total = ctx->s.n_voxels * ctx->n_columns;
+  HMC_HIP(ctx, hipMemcpyAsync(ctx->d_nhist, nhist, sizeof(double) * n_nh, hipMemcpyHostToDevice,
+                              ctx->stream));
   unsigned long long* folded = ctx->d_ledger + L_COUNT;  // scratch slot past the ledger
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, folded, 1ll, 0ull);
+  const long long total = ctx->s.n_voxels * ctx->n_columns;
   hipLaunchKernelGGL(k_fold, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream,
-                     ctx->d_edep, ctx->d_mass, ctx->d_nhist, ctx->s.n_voxels, total, ctx->d_s1,
-                     ctx->d_s2, folded);
+                     ctx->d_edep, ctx->d_mass, ctx->d_nhist, ctx->s.n_voxels, ctx->n_columns,
+                     n_slots_used, ctx->d_s1, ctx->d_s2, folded);
   HMC_HIP(ctx, hipGetLastError());
   unsigned long long q = 0;
   HMC_HIP(ctx, hipMemcpyAsync(&q, folded, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));

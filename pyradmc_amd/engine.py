@@ -40,11 +40,12 @@ from pyradmc_amd.data.gs import build_gs_grid, gs_window_from_tables
 from pyradmc_amd.progress import ProgressEmitter
 from pyradmc_amd.scoring import DijResult, RunProvenance, ScoringGrid, TransportResult
 
-_CHUNK_HISTORIES_PER_CU = 8192
-_CHUNK_SIZE_CAP = 4 * 1024 * 1024
+_CHUNK_HISTORIES_PER_CU = 16384
+_CHUNK_SIZE_CAP = 8 * 1024 * 1024
 _CHUNK_MEMORY_FRACTION = 0.25
 _QUEUE_BYTES_PER_SLOT = 4 * 48
-_GROUP_MEMORY_FRACTION = 0.5
+_GROUP_MEMORY_FRACTION = 0.35
+_SLOT_MEMORY_FRACTION = 0.25
 
 
 def _device_index(device) -> int:
@@ -210,14 +211,23 @@ class HipEngine:
             )
         self._uploaded_tables = key
 
-    def _set_scoring(self, scoring, n_columns: int) -> None:
+    def _batch_slots(self, n_batches: int, n_columns: int, n_vox: int) -> int:
+        """Batches transported together: each holds its own int64 map of the group's
+        columns, folded in batch order afterwards (so any slot count gives the same
+        bits). As many as a quarter of free HBM allows."""
+        free = int(self._info.get("free_bytes", 0))
+        per_slot = 8 * n_columns * n_vox
+        fit = int(free * _SLOT_MEMORY_FRACTION) // per_slot if free > 0 else 1
+        return max(1, min(n_batches, fit))
+
+    def _set_scoring(self, scoring, n_columns: int, n_slots: int = 1) -> None:
         mass = np.ascontiguousarray(np.asarray(scoring.voxel_mass, dtype=np.float64))
         origin = np.asarray(scoring.origin, dtype=np.float64)
         spacing = np.asarray(scoring.spacing, dtype=np.float64)
         self._ctx.call(
             "hmc_set_scoring", int(scoring.shape[0]), int(scoring.shape[1]), int(scoring.shape[2]),
             nat.ptr(origin, nat.c_double), nat.ptr(spacing, nat.c_double),
-            nat.ptr(mass, nat.c_double), int(n_columns),
+            nat.ptr(mass, nat.c_double), int(n_columns), int(n_slots),
         )
 
     def _set_physics(self, pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w) -> None:
@@ -307,13 +317,15 @@ class HipEngine:
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
         return analytic, energy
 
-    def _transport(self, seed: int, items: list[tuple[int, int, int]], chunk: int) -> dict:
+    def _transport(self, seed: int, items: list[tuple[int, int, int, int]], chunk: int) -> dict:
+        """items: (key_base, count, flat tally column, beamlet parameter index)."""
         arr = (nat.WorkItem * len(items))()
         total = 0
-        for k, (key_base, count, column) in enumerate(items):
+        for k, (key_base, count, column, beamlet) in enumerate(items):
             arr[k].key_base = int(key_base)
             arr[k].count = int(count)
             arr[k].column = int(column)
+            arr[k].beamlet = int(beamlet)
             total += int(count)
         ledger = nat.Ledger()
         self._ctx.call(
@@ -322,16 +334,22 @@ class HipEngine:
         )
         return ledger.as_dict()
 
-    def _fold(self, nhist: np.ndarray) -> int:
+    def _fold(self, n_slots: int, nhist: np.ndarray) -> int:
+        """Fold slots 0..n_slots-1 in order; nhist is [slot][column]."""
         total = nat.c_int64()
-        nh = np.ascontiguousarray(np.asarray(nhist, dtype=np.float64))
-        self._ctx.call("hmc_fold", nat.ptr(nh, nat.c_double), nat.ctypes.byref(total))
+        nh = np.ascontiguousarray(np.asarray(nhist, dtype=np.float64).reshape(-1))
+        self._ctx.call("hmc_fold", int(n_slots), nat.ptr(nh, nat.c_double), nat.ctypes.byref(total))
         return int(total.value)
 
     @staticmethod
     def _merge(books: dict, ledger: dict) -> None:
         for k, v in ledger.items():
-            books[k] = books.get(k, 0) + int(v)
+            books[k] = books.get(k, 0) + v
+
+    def set_profiling(self, enabled: bool) -> None:
+        """Time every transport launch with HIP events (per-kernel ``*_kernel_ms`` in
+        the result's ``counters``); a measurement hook, off by default."""
+        self._ctx.call("hmc_set_profiling", 1 if enabled else 0)
 
     # --------------------------------------------------------------------- run
 
@@ -380,18 +398,22 @@ class HipEngine:
         self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
         self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
         analytic, mono_energy = self._set_source(source, primary_kind)
-        self._set_scoring(scoring, 1)
+        slots = self._batch_slots(n_batches, 1, scoring.n_voxels)
+        self._set_scoring(scoring, 1, slots)
 
         per_batch = n_histories // n_batches
-        chunk = self._chunk(per_batch)
+        chunk = self._chunk(per_batch * slots)
         emitter = ProgressEmitter(progress, n_histories)
         books: dict = {}
         deposited_quanta = 0
-        for b in range(n_batches):
-            ledger = self._transport(seed, [(b * per_batch, per_batch, 0)], chunk)
+        for b0 in range(0, n_batches, slots):
+            block = range(b0, min(b0 + slots, n_batches))
+            items = [(b * per_batch, per_batch, b - b0, 0) for b in block]
+            ledger = self._transport(seed, items, chunk)
             self._merge(books, ledger)
-            deposited_quanta += self._fold(np.array([float(per_batch)]))
-            emitter.tick(per_batch)
+            deposited_quanta += self._fold(len(block), np.full(len(block), float(per_batch)))
+            for _ in block:
+                emitter.tick(per_batch)
         n_vox = scoring.n_voxels
         mean = np.empty(n_vox, dtype=np.float64)
         sigma = np.empty(n_vox, dtype=np.float64)
@@ -482,6 +504,7 @@ class HipEngine:
         per_batch = counts // n_batches
         offsets = np.concatenate(([0], np.cumsum(counts)[:-1]))
         if beamlet_group_size is None:
+            # float64 sums plus one int64 map per group column; batch slots come after.
             free = int(self._info.get("free_bytes", 0))
             fit = int(free * _GROUP_MEMORY_FRACTION) // (24 * n_vox) if free > 0 else 128
             beamlet_group_size = max(1, min(fit, n_beamlets, 4096))
@@ -498,19 +521,24 @@ class HipEngine:
         analytic, mono_energy = False, 0.0
         for g0 in range(0, n_beamlets, beamlet_group_size):
             group = range(g0, min(g0 + beamlet_group_size, n_beamlets))
+            n_cols = len(group)
             analytic, mono_energy = self._set_source(source, "photon", beamlets=group)
-            self._set_scoring(scoring, len(group))
+            slots = self._batch_slots(n_batches, n_cols, n_vox)
+            self._set_scoring(scoring, n_cols, slots)
             group_hist = int(counts[g0:group.stop].sum())
-            chunk = self._chunk(int(per_batch[g0:group.stop].sum()))
-            for b in range(n_batches):
+            chunk = self._chunk(int(per_batch[g0:group.stop].sum()) * slots)
+            for b0 in range(0, n_batches, slots):
+                block = range(b0, min(b0 + slots, n_batches))
                 items = []
-                for j in group:
-                    r0 = b * int(per_batch[j])
-                    key = r0 if correlated else int(offsets[j]) + r0
-                    items.append((key, int(per_batch[j]), j - g0))
+                for b in block:
+                    for j in group:
+                        r0 = b * int(per_batch[j])
+                        key = r0 if correlated else int(offsets[j]) + r0
+                        items.append((key, int(per_batch[j]), (b - b0) * n_cols + (j - g0), j - g0))
                 ledger = self._transport(seed, items, chunk)
                 self._merge(books, ledger)
-                deposited_quanta += self._fold(per_batch[g0:group.stop].astype(np.float64))
+                nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
+                deposited_quanta += self._fold(len(block), nhist)
             self._ctx.call("hmc_finalize", int(n_batches), None, None)
             kept = np.zeros(len(group), dtype=np.int64)
             total = nat.c_int64()

@@ -1,0 +1,397 @@
+"""GPU parity tests: the HIP engine against the oracle and the reference's golden doses.
+
+Statistical parity for transport (the batched chi-squared oracle and the per-voxel
+z-score gate of the acceptance criterion: at most 5 percent of voxels above 10 percent
+of max dose beyond |z| > 2), bit-exact parity for the integer and float64 reduction
+work (fold, finalize, truncation, CSC compaction) on synthetic quanta, and
+bit-reproducibility under any chunking, grouping or batch layout.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.stats import assert_z_consistent, batched_chi2_pvalue, north_star_gate
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = 0.01
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from pyradmc_amd import _native
+
+    _native.lib()  # fails loudly if the library is missing
+    return _native
+
+
+def _engine(grid, xs, **kw):
+    from pyradmc_amd import HipEngine
+
+    return HipEngine(grid, xs, **kw)
+
+
+def _balance(r) -> float:
+    return abs(r.energy_emitted - r.energy_deposited - r.energy_escaped - r.energy_unscored) / r.energy_emitted
+
+
+# ----------------------------------------------------------- reduction, bit exact
+
+
+def _synthetic(nv: int, cols: int, rng) -> np.ndarray:
+    q = rng.integers(0, 5_000_000_000, size=(cols, nv), dtype=np.int64)
+    q[rng.random((cols, nv)) < 0.6] = 0  # mostly empty, like a Dij map
+    return q
+
+
+def test_fold_finalize_truncate_bitexact_vs_oracle(hip):
+    from oracle import mc
+
+    rng = np.random.default_rng(7)
+    shape, cols, n_batches = (5, 7, 9), 3, 6
+    nv = int(np.prod(shape))
+    mass = rng.uniform(0.1, 2.0, size=nv)
+    mass[::11] = 0.0  # uncovered scoring voxels
+    ctx = hip.Context(0)
+    origin = np.zeros(3)
+    spacing = np.ones(3)
+    ctx.call("hmc_set_scoring", *shape, hip.ptr(origin, ctypes.c_double),
+             hip.ptr(spacing, ctypes.c_double), hip.ptr(mass, ctypes.c_double), cols, 1)
+    s1 = np.zeros((cols, nv))
+    s2 = np.zeros((cols, nv))
+    nhist = np.array([1000.0, 2500.0, 40.0])
+    for _ in range(n_batches):
+        q = _synthetic(nv, cols, rng)
+        ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+        total = ctypes.c_int64()
+        ctx.call("hmc_fold", 1, hip.ptr(nhist, ctypes.c_double), ctypes.byref(total))
+        assert total.value == mc.fold_quanta(q, mass, nhist, s1, s2)
+        left = np.empty_like(q)
+        ctx.call("hmc_get_quanta", hip.ptr(left, ctypes.c_int64))
+        assert not left.any()  # the fold consumes the map
+    mean = np.empty(cols * nv)
+    sigma = np.empty(cols * nv)
+    ctx.call("hmc_finalize", n_batches, hip.ptr(mean, ctypes.c_double), hip.ptr(sigma, ctypes.c_double))
+    m_ref, s_ref = mc.finalize(s1, s2, n_batches)
+    np.testing.assert_array_equal(mean.reshape(cols, nv), m_ref)
+    np.testing.assert_array_equal(sigma.reshape(cols, nv), s_ref)
+    for trunc in (0.0, 1e-4, 0.3, 1.0):
+        counts = np.zeros(cols, dtype=np.int64)
+        total = ctypes.c_int64()
+        ctx.call("hmc_truncate", trunc, hip.ptr(counts, ctypes.c_int64), ctypes.byref(total))
+        idx = np.empty(total.value, dtype=np.int64)
+        dose = np.empty(total.value)
+        sig = np.empty(total.value)
+        if total.value:
+            ctx.call("hmc_fetch_csc", hip.ptr(idx, ctypes.c_int64), hip.ptr(dose, ctypes.c_double),
+                     hip.ptr(sig, ctypes.c_double))
+        indptr, ridx, rdose, rsig = mc.truncate(m_ref, s_ref, trunc)
+        np.testing.assert_array_equal(np.concatenate(([0], np.cumsum(counts))), indptr)
+        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_array_equal(dose, rdose)
+        np.testing.assert_array_equal(sig, rsig)
+    ctx.close()
+
+
+def test_slot_fold_equals_sequential_batches(hip):
+    # Several batches transported side by side (one map slot each) and folded in slot
+    # order reproduce the one-batch-at-a-time fold bit for bit.
+    from oracle import mc
+
+    rng = np.random.default_rng(11)
+    shape, cols, slots = (6, 5, 4), 2, 3
+    nv = int(np.prod(shape))
+    mass = rng.uniform(0.2, 1.5, size=nv)
+    ctx = hip.Context(0)
+    ctx.call("hmc_set_scoring", *shape, hip.ptr(np.zeros(3), ctypes.c_double),
+             hip.ptr(np.ones(3), ctypes.c_double), hip.ptr(mass, ctypes.c_double), cols, slots)
+    q = np.stack([_synthetic(nv, cols, rng) for _ in range(slots)])  # [slot][col][voxel]
+    nhist = np.array([[100.0, 300.0], [100.0, 300.0], [100.0, 300.0]])
+    ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+    total = ctypes.c_int64()
+    ctx.call("hmc_fold", slots, hip.ptr(nhist, ctypes.c_double), ctypes.byref(total))
+    mean = np.empty(cols * nv)
+    sigma = np.empty(cols * nv)
+    ctx.call("hmc_finalize", slots, hip.ptr(mean, ctypes.c_double), hip.ptr(sigma, ctypes.c_double))
+    s1 = np.zeros((cols, nv))
+    s2 = np.zeros((cols, nv))
+    folded = sum(mc.fold_quanta(q[k], mass, nhist[k], s1, s2) for k in range(slots))
+    assert total.value == folded
+    m_ref, s_ref = mc.finalize(s1, s2, slots)
+    np.testing.assert_array_equal(mean.reshape(cols, nv), m_ref)
+    np.testing.assert_array_equal(sigma.reshape(cols, nv), s_ref)
+    ctx.close()
+
+
+def test_run_batch_slot_count_inert():
+    from pyradmc_amd import workloads
+
+    grid, xs, src, _ = workloads.slab_phantom()
+    a = _engine(grid, xs)
+    b = _engine(grid, xs)
+    b._batch_slots = lambda n_batches, n_columns, n_vox: 3  # blocks of 3 batches
+    ra = a.run(src, 300_000, 10, seed=4)
+    rb = b.run(src, 300_000, 10, seed=4)
+    np.testing.assert_array_equal(ra.dose, rb.dose)
+    np.testing.assert_array_equal(ra.dose_sigma, rb.dose_sigma)
+    assert ra.energy_deposited == rb.energy_deposited
+
+
+def test_truncation_large_column_ordering(hip):
+    # Several truncation chunks per column: CSC rows must come out ascending.
+    from oracle import mc
+
+    rng = np.random.default_rng(3)
+    shape, cols = (40, 40, 30), 2
+    nv = int(np.prod(shape))
+    mass = np.full(nv, 0.5)
+    ctx = hip.Context(0)
+    ctx.call("hmc_set_scoring", *shape, hip.ptr(np.zeros(3), ctypes.c_double),
+             hip.ptr(np.ones(3), ctypes.c_double), hip.ptr(mass, ctypes.c_double), cols, 1)
+    q = _synthetic(nv, cols, rng)
+    ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+    ctx.call("hmc_fold", 1, hip.ptr(np.array([10.0, 10.0]), ctypes.c_double), ctypes.byref(ctypes.c_int64()))
+    ctx.call("hmc_finalize", 1, None, None)
+    counts = np.zeros(cols, dtype=np.int64)
+    total = ctypes.c_int64()
+    ctx.call("hmc_truncate", 0.2, hip.ptr(counts, ctypes.c_int64), ctypes.byref(total))
+    idx = np.empty(total.value, dtype=np.int64)
+    dose = np.empty(total.value)
+    sig = np.empty(total.value)
+    ctx.call("hmc_fetch_csc", hip.ptr(idx, ctypes.c_int64), hip.ptr(dose, ctypes.c_double),
+             hip.ptr(sig, ctypes.c_double))
+    s1 = np.zeros((cols, nv))
+    s2 = np.zeros((cols, nv))
+    mc.fold_quanta(q, mass, [10.0, 10.0], s1, s2)
+    m, s = mc.finalize(s1, s2, 1)
+    indptr, ridx, rdose, _ = mc.truncate(m, s, 0.2)
+    np.testing.assert_array_equal(idx, ridx)
+    np.testing.assert_array_equal(dose, rdose)
+    ctx.close()
+
+
+# ------------------------------------------------------- transport vs reference
+
+
+@pytest.mark.parametrize("tag,units,te", [("c1_cm_kerma", "cm", False), ("c1_cm", "cm", True),
+                                          ("c1_mm", "mm", True)])
+def test_small_pencil_matches_reference_engine(golden_dir, tag, units, te):
+    from pyradmc_amd import workloads
+
+    grid, xs, src, kw = workloads.small_pencil(units)
+    eng = _engine(grid, xs)
+    r = eng.run(src, kw["n_histories"], kw["n_batches"], seed=31337, transport_electrons=te)
+    g = np.load(golden_dir / f"ref_run_{tag}.npz")
+    mask = (g["dose"] > 0.1 * g["dose"].max()) & ((r.dose_sigma > 0) | (g["dose_sigma"] > 0))
+    p, *_ = batched_chi2_pvalue(r.dose, r.dose_sigma, r.n_batches, g["dose"], g["dose_sigma"],
+                                int(g["n_batches"]), mask)
+    assert p > ALPHA
+    assert _balance(r) < 1e-6
+    eng.close()
+
+
+@pytest.fixture(scope="module")
+def slab_runs(oracle_lib):
+    """HIP and oracle on the heterogeneous slab phantom, scored on a coarse grid."""
+    from pyradmc_amd import ScoringGrid, workloads
+
+    grid, xs, src, _ = workloads.slab_phantom()
+    scoring = ScoringGrid.rebin(grid, (16, 16, 48), (0.8, 0.8, 0.4), (0.0, 0.0, 0.0))
+    eng = _engine(grid, xs)
+    hip_run = eng.run(src, 16_000_000, 40, seed=99, scoring_grid=scoring)
+    eng.close()
+    ora = oracle_lib.for_grid(grid, src, scoring=dict(shape=scoring.shape, spacing=scoring.spacing,
+                                                      origin=scoring.origin, mass=scoring.voxel_mass))
+    ora_run = ora.run(4_000_000, 40, seed=98, threads=16)
+    return hip_run, ora_run
+
+
+def test_slab_phantom_matches_oracle(slab_runs):
+    # Acceptance gate: voxels above 10 percent of max dose, z = dD / sqrt(s1^2 + s2^2)
+    # consistent with N(0, 1) - the 2-sigma exceedance count against its binomial null
+    # (k=2) and the batched chi-squared; 40 batches a side keep the batch-estimated
+    # sigmas close to exact.
+    hip_run, ora = slab_runs
+    mask = ora["dose"] > 0.1 * ora["dose"].max()
+    p, *_ = batched_chi2_pvalue(hip_run.dose, hip_run.dose_sigma, 40, ora["dose"], ora["dose_sigma"], 40, mask)
+    assert p > ALPHA
+    frac, n = north_star_gate(hip_run.dose, hip_run.dose_sigma, ora["dose"], ora["dose_sigma"])
+    assert n > 1000 and frac < 0.06
+    assert_z_consistent(hip_run.dose, hip_run.dose_sigma, ora["dose"], ora["dose_sigma"], k=2.0,
+                        mask=mask)
+    assert_z_consistent(hip_run.dose, hip_run.dose_sigma, ora["dose"], ora["dose_sigma"], k=3.0,
+                        mask=mask)
+    # Energy books per emitted MeV agree to statistical precision.
+    eb = ora["books"]
+    assert hip_run.energy_deposited / hip_run.energy_emitted == pytest.approx(
+        eb["deposited"] / eb["emitted"], rel=0.01)
+    assert _balance(hip_run) < 1e-6
+
+
+def test_slab_phantom_matches_reference_engine(golden_dir, slab_runs):
+    hi = golden_dir / "ref_hi_slab.npz"
+    if not hi.is_file():
+        pytest.skip("high-statistics reference slab fixture not generated")
+    g = np.load(hi)
+    hip_run, _ = slab_runs
+    mask = g["dose"] > 0.1 * g["dose"].max()
+    p, *_ = batched_chi2_pvalue(hip_run.dose, hip_run.dose_sigma, 40, g["dose"], g["dose_sigma"],
+                                int(g["n_batches"]), mask)
+    assert p > ALPHA
+    assert_z_consistent(hip_run.dose, hip_run.dose_sigma, g["dose"], g["dose_sigma"], k=2.0, mask=mask)
+
+
+# ------------------------------------------------------------ reproducibility
+
+
+def test_bit_reproducible_and_chunk_inert():
+    from pyradmc_amd import workloads
+
+    grid, xs, src, _ = workloads.slab_phantom()
+    a = _engine(grid, xs).run(src, 200_000, 4, seed=5)
+    b = _engine(grid, xs).run(src, 200_000, 4, seed=5)
+    c = _engine(grid, xs, chunk_size=7_919).run(src, 200_000, 4, seed=5)
+    for other in (b, c):
+        np.testing.assert_array_equal(a.dose, other.dose)
+        np.testing.assert_array_equal(a.dose_sigma, other.dose_sigma)
+        assert a.energy_escaped == other.energy_escaped
+        assert a.energy_deposited == other.energy_deposited
+
+
+def test_batch_layout_changes_only_the_fold():
+    # The same histories in 2 or 4 batches deposit the same total quanta.
+    from pyradmc_amd import workloads
+
+    grid, xs, src, _ = workloads.small_pencil("cm")
+    eng = _engine(grid, xs)
+    r2 = eng.run(src, 40_000, 2, seed=11)
+    r4 = eng.run(src, 40_000, 4, seed=11)
+    assert r2.energy_deposited == r4.energy_deposited
+    assert r2.energy_escaped == r4.energy_escaped
+
+
+# -------------------------------------------------------------------------- Dij
+
+
+def test_one_by_one_lattice_equals_open_field():
+    from pyradmc_amd import BeamletGridSource, ParallelBeamSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((8, 8, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    eng = _engine(grid, xs)
+    open_field = eng.run(ParallelBeamSource(2.0, -0.1, (1.0, 3.0), (1.0, 3.0)), 60_000, 6, seed=21)
+    for correlated in (True, False):
+        d = eng.run_dij(BeamletGridSource(2.0, -0.1, (1.0, 3.0), (1.0, 3.0), 1, 1), 60_000, 6, seed=21,
+                        truncation=0.0, correlated=correlated)
+        np.testing.assert_array_equal(d.column_dense(0), open_field.dose)
+        np.testing.assert_array_equal(d.sigma_dense(0), open_field.dose_sigma)
+
+
+def test_dij_grouping_and_chunking_bit_identical():
+    from pyradmc_amd import BeamletGridSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((8, 8, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    src = BeamletGridSource(2.0, -0.1, (1.0, 3.0), (1.0, 3.0), 2, 3)
+    counts = np.array([600, 1200, 300, 900, 1500, 600])
+    base = None
+    for group, chunk in ((None, None), (1, 257), (4, 1000), (6, 33)):
+        eng = _engine(grid, xs, chunk_size=chunk)
+        d = eng.run_dij(src, counts, 3, seed=8, truncation=1e-3, beamlet_group_size=group)
+        if base is None:
+            base = d
+            continue
+        for name in ("indptr", "indices", "dose", "sigma"):
+            np.testing.assert_array_equal(getattr(base, name), getattr(d, name))
+        assert base.energy_deposited == d.energy_deposited
+
+
+def test_dij_matches_reference_engine(golden_dir):
+    from pyradmc_amd import BeamletGridSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    g = np.load(golden_dir / "ref_dij_small.npz")
+    grid = VoxelGrid.uniform_water((8, 8, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    src = BeamletGridSource(2.0, -0.1, (1.0, 3.0), (1.0, 3.0), 2, 2)
+    eng = _engine(grid, xs)
+    for tag, corr in (("corr", True), ("indep", False)):
+        d = eng.run_dij(src, 30_000, 6, seed=77, correlated=corr, truncation=0.0)
+        for j in range(4):
+            lo, hi = int(g[f"{tag}_indptr"][j]), int(g[f"{tag}_indptr"][j + 1])
+            ref = np.zeros(d.n_voxels)
+            ref_s = np.zeros(d.n_voxels)
+            ref[g[f"{tag}_indices"][lo:hi]] = g[f"{tag}_dose"][lo:hi]
+            ref_s[g[f"{tag}_indices"][lo:hi]] = g[f"{tag}_sigma"][lo:hi]
+            ours = d.column_dense(j).ravel()
+            ours_s = d.sigma_dense(j).ravel()
+            mask = (ref > 0.1 * ref.max()) & ((ref_s > 0) | (ours_s > 0))
+            p, *_ = batched_chi2_pvalue(ours, ours_s, 6, ref, ref_s, 6, mask)
+            assert p > ALPHA, (tag, j, p)
+
+
+def test_uneven_dij_matches_oracle(oracle_lib):
+    from pyradmc_amd import SpectralBeamletSource, VoxelGrid, bremsstrahlung_6mv_like
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((16, 16, 20), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    centers = [(3.0, 4.0, 5.0), (4.0, 4.0, 5.0), (5.0, 4.0, 5.0)]
+    src = SpectralBeamletSource(bremsstrahlung_6mv_like(), (4.0, 4.0, -95.0), centers, 0.5, 0.5)
+    counts = np.array([200_000, 50_000, 120_000])
+    eng = _engine(grid, xs)
+    d = eng.run_dij(src, counts * 4, 8, seed=3, truncation=1e-4)
+    ora = oracle_lib.for_grid(grid, src).run_dij(counts, 8, seed=4, truncation=1e-4, threads=16)
+    for j in range(3):
+        ours, ours_s = d.column_dense(j).ravel(), d.sigma_dense(j).ravel()
+        ref, ref_s = ora["dense_dose"][j], ora["dense_sigma"][j]
+        mask = (ref > 0.1 * ref.max()) & ((ref_s > 0) | (ours_s > 0))
+        p, *_ = batched_chi2_pvalue(ours, ours_s, 8, ref, ref_s, 8, mask)
+        assert p > ALPHA, (j, p)
+    assert d.energy_emitted > 0
+    assert _balance(d) < 1e-6
+
+
+# ------------------------------------------------------------- failure paths
+
+
+def test_queue_overflow_raises():
+    # A 6 MeV pencil into a 40 cm water block: each photon Compton-scatters several
+    # times in one photon round, so a queue sized for one secondary per primary
+    # overflows; the books must fail loudly, never drop particles silently.
+    from pyradmc_amd import AnalyticCrossSections, PencilBeamSource, VoxelGrid
+
+    grid = VoxelGrid.uniform_water((20, 20, 20), (2.0, 2.0, 2.0))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    eng = _engine(grid, xs, chunk_size=1000, queue_factor=1)
+    with pytest.raises(RuntimeError, match="overflow"):
+        eng.run(PencilBeamSource(6.0, (20.0, 20.0, -1.0), (0.0, 0.0, 1.0)), 20_000, 2, seed=1)
+
+
+def test_majorant_violation_raises():
+    from pyradmc_amd import AnalyticCrossSections, workloads
+
+    grid, _, src, _ = workloads.slab_phantom()
+    too_low = AnalyticCrossSections(geometry_densities=((0, 1.0),))  # bone is 1.85
+    eng = _engine(grid, too_low)
+    with pytest.raises(RuntimeError, match="majorant"):
+        eng.run(src, 100_000, 2, seed=1)
+
+
+def test_argument_validation():
+    from pyradmc_amd import workloads
+
+    grid, xs, src, _ = workloads.small_pencil("cm")
+    eng = _engine(grid, xs)
+    with pytest.raises(ValueError):
+        eng.run(src, 10, 3, seed=1)
+    with pytest.raises(ValueError):
+        eng.run(src, 10, 2, seed=1, msc_model="x")
+    with pytest.raises(ValueError):
+        eng.run(src, 10, 2, seed=1, scoring_mode="dose_to_water", transport_electrons=False)
