@@ -15,8 +15,9 @@ launch - 48 B per particle popped, 48 B per queue push, 5 B per voxel fetch (flo
 density + uint8 material), 16 B per int64 tally atomic; tables and the GS grid are
 cache-resident and not counted - divided by that kernel's mean launch time from HIP
 events recorded on the engine's own stream during the timed steps. ``traffic`` is read
-from ``profiles/traffic_electron_round.json`` when a PMC measurement of the same
-workload has been committed there, else null.
+from ``profiles/traffic_electron_round.json``: HBM bytes per launch from two PMC runs of
+this workload (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter reads for a
+1 GiB device copy (``--copy-probe-mib 1024`` runs that copy first); null if absent.
 
 ``cpu_baseline`` times the CPU restatement of the reference algorithm (``oracle/``,
 double precision, one stream per history, pthreads) on this host's cores for a
@@ -71,14 +72,28 @@ def main() -> int:
     ap.add_argument("--histories", type=int, default=10_000_000)
     ap.add_argument("--batches", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=None,
-                    help="histories for the CPU baseline (default: ~15 s of CPU work)")
+                    help="histories for the CPU baseline (default: about 10 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run three device-to-device copies of this many MiB (a known-traffic "
+                         "kernel that calibrates the HBM byte counters of a PMC profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               "(histories shard across GPUs with no exchange for Dij columns; see the design notes)",
               file=sys.stderr)
         return 2
+
+    if args.copy_probe_mib > 0:
+        import torch
+
+        n = args.copy_probe_mib << 18  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     from pyradmc_amd import HipEngine, workloads
 
@@ -127,7 +142,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        sample = args.cpu_sample or 150_000 * threads
+        sample = args.cpu_sample or 2_500_000 * threads  # ~10 s at ~0.25M histories/s/thread
         cpu = _cpu_baseline(sample, threads)
 
     line = {

@@ -1145,7 +1145,10 @@ int hmc_device_info(int device, char* name, int name_len, int* cu_count, int64_t
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HMC_EHIP;
   if (name && name_len > 0) {
-    std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", prop.name, prop.gcnArchName);
+    // The marketing name can be empty where the ids table is not readable; the
+    // architecture and CU count always identify the part.
+    std::snprintf(name, static_cast<size_t>(name_len), "%s%s%s, %d CUs", prop.name,
+                  prop.name[0] ? " " : "", prop.gcnArchName, prop.multiProcessorCount);
   }
   if (cu_count) *cu_count = prop.multiProcessorCount;
   if (total_bytes) *total_bytes = static_cast<int64_t>(prop.totalGlobalMem);

@@ -209,6 +209,39 @@ def _dij_small() -> None:
     np.savez_compressed(OUT / "ref_dij_small.npz", **out)
 
 
+def _dij_fan() -> None:
+    """Divergent spectral beamlets (SpectralBeamletSource) through a parallelogram
+    aperture into a water block with a lung-density slab: pins the fan generator
+    (focal point, axes, spectrum draw order) and density scaling in a Dij."""
+    from pyradmc.data.analytic import AnalyticCrossSections
+    from pyradmc.geometry.grid import VoxelGrid
+    from pyradmc.geometry.source import SpectralBeamletSource
+    from pyradmc.geometry.spectrum import Spectrum
+
+    ReferenceEngine, HostRNG = _engine()
+    shape = (16, 16, 24)
+    density = np.ones(shape)
+    density[:, :, 6:12] = 0.3
+    grid = VoxelGrid(shape=shape, spacing=(0.5, 0.5, 0.5), density=density,
+                     material=np.zeros(shape, dtype=np.int32))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    engine = ReferenceEngine(grid, xs, HostRNG())
+    edges = np.linspace(0.1, 6.0, 61)
+    mid = 0.5 * (edges[:-1] + edges[1:])
+    spectrum = Spectrum(edges, mid * np.exp(-mid / 1.8), convention="number")
+    centers = [(3.5, 3.5, 4.0), (4.5, 3.5, 4.0), (3.5, 4.5, 4.0), (4.5, 4.8, 4.0)]
+    source = SpectralBeamletSource(spectrum, (4.0, 4.0, -96.0), centers, 1.0, 0.8,
+                                   u_axis=(1.0, 0.0, 0.0), v_axis=(0.3, 1.0, 0.0))
+    started = time.perf_counter()
+    d = engine.run_dij(source, 16000, 8, seed=20260712, correlated=True, truncation=1e-3)
+    out = {name: np.asarray(getattr(d, name)) for name in ("indptr", "indices", "dose", "sigma")}
+    for name in ("energy_emitted", "energy_deposited", "energy_escaped", "energy_unscored"):
+        out[name] = np.float64(getattr(d, name))
+    out["density"] = density
+    print(f"dij fan: {time.perf_counter() - started:.1f} s")
+    np.savez_compressed(OUT / "ref_dij_fan.npz", **out)
+
+
 def _spectral_parallel_source():
     """Parallel 10x10 cm field of 6 MV-like spectral photons: the reference's sanctioned
     extension point (a Source subclass); 4 uniforms: energy (2), then x, then y."""
@@ -280,10 +313,13 @@ def _run_hi(config: str, part: int, n: int) -> None:
     print(f"hi {config} part {part}: {n} in {wall:.1f} s ({n / wall:.1f} hist/s)")
 
 
-def _merge_hi(config: str) -> None:
-    """Pool the shards: mean of shard means; sigma from the pooled batch variance."""
-    parts = sorted((OUT.parent.parent / "_work").glob(f"hi_{config}_*.npz"))
-    runs = [np.load(p) for p in parts]
+def _merge_hi(config: str, parts: list[int]) -> None:
+    """Pool equal-size shards: mean of shard means; standard error from the shard
+    standard errors (each from 10 batches), i.e. 10 * shards batches in total."""
+    work = OUT.parent.parent / "_work"
+    runs = [np.load(work / f"hi_{config}_{p}.npz") for p in parts]
+    if len({int(r["n"]) for r in runs}) != 1:
+        raise ValueError("shards must hold equal history counts")
     k = len(runs)
     dose = sum(r["dose"] for r in runs) / k
     sigma = np.sqrt(sum(r["dose_sigma"] ** 2 for r in runs)) / k
@@ -303,6 +339,7 @@ TASKS = {
     "run_c1_cm": lambda: _run_c1(1.0, (8.0, 8.0, -1.0), True, "c1_cm"),
     "run_c1_mm": lambda: _run_c1(0.1, (0.8, 0.8, -0.1), True, "c1_mm"),
     "dij_small": _dij_small,
+    "dij_fan": _dij_fan,
 }
 
 
@@ -310,7 +347,7 @@ if __name__ == "__main__":
     if sys.argv[1] == "hi":
         _run_hi(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
     elif sys.argv[1] == "merge_hi":
-        _merge_hi(sys.argv[2])
+        _merge_hi(sys.argv[2], [int(v) for v in sys.argv[3:]])
     else:
         for task in sys.argv[1:]:
             TASKS[task]()

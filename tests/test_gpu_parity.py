@@ -14,7 +14,8 @@ import ctypes
 import numpy as np
 import pytest
 
-from tests.stats import assert_z_consistent, batched_chi2_pvalue, north_star_gate
+from tests.stats import (assert_within_replicates, assert_z_consistent, batched_chi2_pvalue,
+                         comparison_statistics, north_star_gate)
 
 pytestmark = pytest.mark.gpu
 
@@ -232,17 +233,58 @@ def test_slab_phantom_matches_oracle(slab_runs):
     assert _balance(hip_run) < 1e-6
 
 
+def _assert_matches_fixture(g, truth, mask, run, replicates: int = 16) -> None:
+    """A pooled reference fixture (equal shards of 10 batches of a few ten thousand
+    histories) against a precise HIP run, calibrated by HIP replicates with the
+    fixture's exact shard, batch and history structure; ``mask`` is independent of both
+    compared samples."""
+    shards = int(g["shards"])
+    per_shard = int(g["n_histories"]) // shards
+    batches = int(g["n_batches"]) // shards
+    nb = int(g["n_batches"])
+
+    def summary(d, s):
+        return comparison_statistics(truth.dose, truth.dose_sigma, truth.n_batches, d, s, nb, mask)
+
+    reps = []
+    for r in range(replicates):
+        runs = [run(per_shard, batches, 5000 + 10 * r + k) for k in range(shards)]
+        reps.append(summary(sum(x.dose for x in runs) / shards,
+                            np.sqrt(sum(x.dose_sigma**2 for x in runs)) / shards))
+    assert_within_replicates(summary(g["dose"], g["dose_sigma"]), reps)
+
+
 def test_slab_phantom_matches_reference_engine(golden_dir, slab_runs):
+    # Reference engine, 4 shards x 10 batches x 25k histories, against HIP at 16M.
     hi = golden_dir / "ref_hi_slab.npz"
     if not hi.is_file():
         pytest.skip("high-statistics reference slab fixture not generated")
-    g = np.load(hi)
-    hip_run, _ = slab_runs
-    mask = g["dose"] > 0.1 * g["dose"].max()
-    p, *_ = batched_chi2_pvalue(hip_run.dose, hip_run.dose_sigma, 40, g["dose"], g["dose_sigma"],
-                                int(g["n_batches"]), mask)
-    assert p > ALPHA
-    assert_z_consistent(hip_run.dose, hip_run.dose_sigma, g["dose"], g["dose_sigma"], k=2.0, mask=mask)
+    from pyradmc_amd import ScoringGrid, workloads
+
+    hip_run, ora = slab_runs
+    mask = ora["dose"] > 0.1 * ora["dose"].max()
+    grid, xs, src, _ = workloads.slab_phantom()
+    scoring = ScoringGrid.rebin(grid, (16, 16, 48), (0.8, 0.8, 0.4), (0.0, 0.0, 0.0))
+    eng = _engine(grid, xs)
+    _assert_matches_fixture(np.load(hi), hip_run, mask,
+                            lambda n, b, seed: eng.run(src, n, b, seed=seed, scoring_grid=scoring))
+    eng.close()
+
+
+def test_small_pencil_matches_high_statistics_reference(golden_dir):
+    # 6 MeV pencil in 16^3 water at 1 cm: reference engine 4 shards x 10 batches x 40k.
+    hi = golden_dir / "ref_hi_c1_cm.npz"
+    if not hi.is_file():
+        pytest.skip("high-statistics reference pencil fixture not generated")
+    from pyradmc_amd import workloads
+
+    grid, xs, src, _ = workloads.small_pencil("cm")
+    eng = _engine(grid, xs)
+    truth = eng.run(src, 16_000_000, 40, seed=77)
+    mask_run = eng.run(src, 4_000_000, 40, seed=78)
+    mask = mask_run.dose > 0.1 * mask_run.dose.max()
+    _assert_matches_fixture(np.load(hi), truth, mask, lambda n, b, seed: eng.run(src, n, b, seed=seed))
+    eng.close()
 
 
 # ------------------------------------------------------------ reproducibility
@@ -334,6 +376,42 @@ def test_dij_matches_reference_engine(golden_dir):
             mask = (ref > 0.1 * ref.max()) & ((ref_s > 0) | (ours_s > 0))
             p, *_ = batched_chi2_pvalue(ours, ours_s, 6, ref, ref_s, 6, mask)
             assert p > ALPHA, (tag, j, p)
+
+
+def test_fan_dij_matches_reference_engine(golden_dir):
+    # Divergent spectral beamlets through parallelogram apertures into a slab phantom:
+    # the reference's Dij (16k histories per beamlet, 8 batches, truncation 1e-3) against
+    # a precise HIP run, calibrated by HIP replicates of the same shape.
+    from tests.problems import FAN_BATCHES, FAN_HISTORIES, FAN_TRUNCATION, csc_dense, fan_dij_problem
+
+    g = np.load(golden_dir / "ref_dij_fan.npz")
+    grid, xs, src = fan_dij_problem(g)
+    eng = _engine(grid, xs)
+    n_cols = 4
+
+    def dense(d):
+        return (np.stack([d.column_dense(j).ravel() for j in range(n_cols)]),
+                np.stack([d.sigma_dense(j).ravel() for j in range(n_cols)]))
+
+    truth_d, truth_s = dense(eng.run_dij(src, FAN_HISTORIES * 100, 40, seed=101, truncation=0.0))
+    mask_d, _ = dense(eng.run_dij(src, FAN_HISTORIES * 40, 40, seed=102, truncation=0.0))
+    mask = mask_d > 0.1 * mask_d.max(axis=1, keepdims=True)
+
+    def summary(d, s, deposited, emitted):
+        out = comparison_statistics(truth_d, truth_s, 40, d, s, FAN_BATCHES, mask)
+        out["deposited_fraction"] = deposited / emitted
+        return out
+
+    reps = []
+    for r in range(16):
+        d = eng.run_dij(src, FAN_HISTORIES, FAN_BATCHES, seed=200 + r, truncation=FAN_TRUNCATION)
+        reps.append(summary(*dense(d), d.energy_deposited, d.energy_emitted))
+    eng.close()
+    nv = truth_d.shape[1]
+    ref = summary(csc_dense(g["indptr"], g["indices"], g["dose"], nv),
+                  csc_dense(g["indptr"], g["indices"], g["sigma"], nv),
+                  float(g["energy_deposited"]), float(g["energy_emitted"]))
+    assert_within_replicates(ref, reps)
 
 
 def test_uneven_dij_matches_oracle(oracle_lib):

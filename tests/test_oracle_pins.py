@@ -147,3 +147,106 @@ def test_oracle_transport_matches_reference_engine(golden_dir, oracle_lib, tag, 
     assert p > 0.01
     b = r["books"]
     assert abs(b["emitted"] - b["deposited"] - b["escaped"] - b["unscored"]) <= 1e-9 * b["emitted"]
+
+
+# ------------------------------------------------- scoring arithmetic of the oracle
+
+
+def test_fold_finalize_matches_two_pass_batch_statistics():
+    # The oracle's one-pass running sums against a two-pass computation from the
+    # per-batch doses themselves: catches a dropped n/(n-1), a sigma of the
+    # population instead of the mean, or a wrong unit conversion.
+    from oracle import mc
+
+    rng = np.random.default_rng(21)
+    cols, nv, n = 3, 50, 7
+    mass = rng.uniform(0.2, 3.0, size=nv)
+    mass[[4, 17]] = 0.0
+    nhist = np.array([1000.0, 250.0, 7.0])
+    s1 = np.zeros((cols, nv))
+    s2 = np.zeros((cols, nv))
+    batches = []
+    for _ in range(n):
+        q = rng.integers(0, 10**12, size=(cols, nv), dtype=np.int64)
+        q[rng.random((cols, nv)) < 0.3] = 0
+        mc.fold_quanta(q, mass, nhist, s1, s2)
+        dose = np.zeros((cols, nv))
+        for c in range(cols):
+            for v in range(nv):
+                if mass[v] > 0.0:
+                    dose[c, v] = (int(q[c, v]) / 1e9) / (mass[v] * nhist[c])  # MeV/g/history
+        batches.append(dose)
+    mean, sigma = mc.finalize(s1, s2, n)
+    stack = np.stack(batches)
+    np.testing.assert_allclose(mean, stack.mean(axis=0), rtol=1e-12, atol=0.0)
+    np.testing.assert_allclose(sigma, stack.std(axis=0, ddof=1) / math.sqrt(n), rtol=1e-6, atol=1e-300)
+    assert not mean[:, [4, 17]].any() and not sigma[:, [4, 17]].any()
+    one_mean, one_sigma = mc.finalize(s1, s2, 1)
+    assert not one_sigma.any()  # a single batch has no spread
+
+
+def test_truncation_known_answer():
+    from oracle import mc
+
+    mean = np.array([[0.0, 0.5, 1e-4, 0.25, 1.0],   # threshold 0.25 * 1.0, the tie is kept
+                     [0.0, 0.0, 0.0, 0.0, 0.0],      # empty column
+                     [4.0, 0.0, 4.0, 1e-9, 1.0]])    # threshold 0.25 * 4.0, the tie is kept
+    sigma = mean * 0.1
+    indptr, idx, dose, sig = mc.truncate(mean, sigma, 0.25)
+    np.testing.assert_array_equal(indptr, [0, 3, 3, 6])
+    np.testing.assert_array_equal(idx, [1, 3, 4, 0, 2, 4])
+    np.testing.assert_array_equal(dose, [0.5, 0.25, 1.0, 4.0, 4.0, 1.0])
+    np.testing.assert_array_equal(sig, 0.1 * dose)
+    # The threshold is the float64 product t * max: 0.001 * 3.2865069716518276 rounds
+    # above the stored value 0.0032865069716518274, which is dropped, although the
+    # quotient test value / max >= t would keep it.
+    mx, v = 3.2865069716518276, 0.0032865069716518274
+    assert v / mx >= 0.001 and not v >= 0.001 * mx
+    indptr, idx, _, _ = mc.truncate(np.array([[mx, v]]), np.zeros((1, 2)), 0.001)
+    np.testing.assert_array_equal(idx, [0])
+
+
+def test_reference_dij_golden_obeys_the_truncation_rule(golden_dir):
+    g = np.load(golden_dir / "ref_dij_small.npz")
+    for tag in ("corr", "indep"):
+        indptr, idx, dose = g[f"{tag}_indptr"], g[f"{tag}_indices"], g[f"{tag}_dose"]
+        for c in range(len(indptr) - 1):
+            d = dose[indptr[c]:indptr[c + 1]]
+            i = idx[indptr[c]:indptr[c + 1]]
+            assert np.all(np.diff(i) > 0) and np.all(d > 0.0)
+            assert np.all(d >= 1e-3 * d.max())
+
+
+def _column_mask(dense: np.ndarray, rel: float = 0.1) -> np.ndarray:
+    return dense > rel * dense.max(axis=1, keepdims=True)
+
+
+def test_oracle_fan_dij_matches_reference(golden_dir, oracle_lib):
+    # Divergent spectral beamlets through parallelogram apertures into a slab phantom:
+    # the reference's Dij (16k histories per beamlet, 8 batches) against a precise
+    # oracle run, calibrated by oracle replicates of the same shape.
+    from tests.problems import FAN_BATCHES, FAN_HISTORIES, FAN_TRUNCATION, csc_dense, fan_dij_problem
+    from tests.stats import assert_within_replicates, comparison_statistics
+
+    g = np.load(golden_dir / "ref_dij_fan.npz")
+    grid, _, src = fan_dij_problem(g)
+    ora = oracle_lib.for_grid(grid, src)
+    counts = np.full(4, FAN_HISTORIES)
+    truth = ora.run_dij(counts * 25, 40, seed=101, truncation=0.0, threads=8)
+    mask = _column_mask(ora.run_dij(counts * 10, 10, seed=102, truncation=0.0, threads=8)["dense_dose"])
+    nv = truth["dense_dose"].shape[1]
+
+    def summary(indptr, indices, dose, sigma, deposited, emitted):
+        out = comparison_statistics(truth["dense_dose"], truth["dense_sigma"], 40,
+                                    csc_dense(indptr, indices, dose, nv),
+                                    csc_dense(indptr, indices, sigma, nv), FAN_BATCHES, mask)
+        out["deposited_fraction"] = deposited / emitted
+        return out
+
+    reps = []
+    for r in range(16):
+        rep = ora.run_dij(counts, FAN_BATCHES, seed=200 + r, truncation=FAN_TRUNCATION, threads=8)
+        reps.append(summary(rep["indptr"], rep["indices"], rep["dose"], rep["sigma"],
+                            rep["books"]["deposited"], rep["books"]["emitted"]))
+    assert_within_replicates(summary(g["indptr"], g["indices"], g["dose"], g["sigma"],
+                                     float(g["energy_deposited"]), float(g["energy_emitted"])), reps)
