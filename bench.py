@@ -33,6 +33,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -42,6 +44,51 @@ PEAK_HBM_GBS = 8000.0
 def _algorithmic_bytes(c: dict, prefix: str) -> float:
     return (48.0 * c[f"{prefix}_pops"] + 48.0 * c[f"{prefix}_pushes"]
             + 5.0 * c[f"{prefix}_voxels"] + 16.0 * c[f"{prefix}_deposits"])
+
+
+def _per_beamlet_sigma(d) -> np.ndarray:
+    """Median over each column's voxels with D >= 0.5 max of sigma/D."""
+    out = np.zeros(d.n_beamlets)
+    for j in range(d.n_beamlets):
+        lo, hi = int(d.indptr[j]), int(d.indptr[j + 1])
+        dose = np.asarray(d.dose[lo:hi], dtype=np.float64)
+        sig = np.asarray(d.sigma[lo:hi], dtype=np.float64)
+        keep = dose >= 0.5 * dose.max() if hi > lo else np.zeros(0, bool)
+        out[j] = float(np.median(sig[keep] / dose[keep])) if keep.any() else np.nan
+    return out
+
+
+def _dij_leg(median: float) -> dict:
+    """The batched uneven Dij: 128x128x96 CT-like phantom at 2.5 mm, 100 spectral
+    beamlets of 5x5 mm from 100 cm SAD, lognormal per-beamlet counts, truncation 1e-4,
+    float32 CSC values - one complete run_dij, timed wall to wall (CSC on the host)."""
+    from pyradmc_amd import HipEngine, workloads
+
+    grid, xs, src, kw = workloads.ct_like_dij(median=median)
+    counts = kw["n_histories_per_beamlet"]
+    engine = HipEngine(grid, xs)
+    small = np.maximum(kw["n_batches"], counts // 100 // kw["n_batches"] * kw["n_batches"])
+    engine.run_dij(src, small, kw["n_batches"], seed=1, truncation=kw["truncation"])  # warm
+    started = time.perf_counter()
+    d = engine.run_dij(src, counts, kw["n_batches"], seed=kw["seed"], truncation=kw["truncation"],
+                       value_dtype=np.float32)
+    wall = time.perf_counter() - started
+    engine.close()
+    sig = _per_beamlet_sigma(d)
+    med = float(np.nanmedian(sig))
+    return {
+        "workload": "CT-like 128x128x96 @ 2.5 mm, 100 beamlets 5x5 mm, SAD 100, lognormal counts "
+                    f"(median {median:g}, sigma 0.8), truncation 1e-4, f32 CSC",
+        "histories": int(counts.sum()),
+        "wall_s": round(wall, 4),
+        "histories_per_s": round(float(counts.sum()) / wall, 1),
+        "nnz": int(d.indptr[-1]),
+        "per_beamlet_sigma_median": round(med, 5),
+        "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
+        # Sigma scales as N^-1/2 at fixed layout: projected wall time for every beamlet's
+        # median sigma at 2 percent, from the worst (90th percentile) column.
+        "wall_s_at_2pct_projected": round(wall * (float(np.nanpercentile(sig, 90)) / 0.02) ** 2, 4),
+    }
 
 
 def _cpu_baseline(sample_histories: int, threads: int) -> dict:
@@ -74,6 +121,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="histories for the CPU baseline (default: about 10 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dij", action="store_true", help="skip the batched Dij leg")
+    ap.add_argument("--dij-median", type=float, default=2e5,
+                    help="median per-beamlet history count of the Dij leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
                          "kernel that calibrates the HBM byte counters of a PMC profile)")
@@ -139,6 +189,8 @@ def main() -> int:
             counters["electron_kernel_ms"] / max(counters["electron_kernel_ms"] + counters["photon_kernel_ms"], 1e-9), 4),
     }
 
+    dij = None if args.no_dij else _dij_leg(args.dij_median)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -168,6 +220,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dij": dij,
         "events_per_history": {
             k: round(counters[k] / total, 3)
             for k in ("photon_steps", "electron_steps", "photon_voxels", "electron_voxels",

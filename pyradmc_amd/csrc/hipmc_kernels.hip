@@ -55,6 +55,62 @@ struct Queue {
   unsigned int capacity;
 };
 
+// Slot reservation for the lanes of one wave that want one: a single atomic per
+// wave, ranks by lane order. Must be reached by every active lane of the wave.
+__device__ inline unsigned int wave_reserve(unsigned int* counter, bool want) {
+  const unsigned long long mask = __ballot(want ? 1 : 0);
+  if (mask == 0ull) return 0u;
+  const int lane = static_cast<int>(__lane_id());
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned int base = 0u;
+  if (lane == leader) base = atomicAdd(counter, static_cast<unsigned int>(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ inline long long wave_sum(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Queue indices handed to the idle lanes of a wave. The wave claims blocks of
+// kRefillBlock indices from the queue head with one atomic per block and gives them
+// to its idle lanes in lane order, so the single head counter sees one atomic per
+// block instead of one per particle. Called by every lane still in the round loop, at
+// the loop head (converged); the pool state is therefore identical in all of them.
+// Which lane transports which particle does not change any result: each particle
+// carries its own random stream and the tallies are integer sums.
+constexpr unsigned int kRefillBlock = 64;
+
+struct RefillPool {
+  unsigned int next = 0, end = 0;
+};
+
+__device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, bool idle) {
+  const unsigned long long mask = __ballot(idle ? 1 : 0);
+  if (mask == 0ull) return 0u;
+  const unsigned int want = static_cast<unsigned int>(__popcll(mask));
+  const int lane = static_cast<int>(__lane_id());
+  const unsigned int rank =
+      static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+  const unsigned int avail = pool.end - pool.next;
+  unsigned int idx;
+  if (want <= avail) {
+    idx = pool.next + rank;
+    pool.next += want;
+  } else {
+    const unsigned long long present = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(present)) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(head, kRefillBlock);
+    base = __shfl(base, leader);
+    idx = rank < avail ? pool.next + rank : base + (rank - avail);
+    pool.next = base + (want - avail);
+    pool.end = base + kRefillBlock;
+  }
+  return idx;
+}
+
 enum LedgerSlot {
   L_ESCAPED = 0,
   L_UNSCORED,
@@ -63,16 +119,13 @@ enum LedgerSlot {
   L_VIOLATIONS,
   L_TRAPPED,
   // per-kernel event counts, photon round then electron round: voxel fetches,
-  // tally atomics, queue pushes, particles popped, steps
+  // tally atomics, steps (queue pushes and pops are counted by the host from the
+  // queue counts it reads between rounds)
   L_P_VOX,
   L_P_DEP,
-  L_P_PUSH,
-  L_P_POP,
   L_P_STEPS,
   L_E_VOX,
   L_E_DEP,
-  L_E_PUSH,
-  L_E_POP,
   L_E_STEPS,
   L_COUNT
 };
@@ -87,10 +140,12 @@ struct TransportParams {
   int transport_electrons, msc_gs, dose_to_water, max_steps;
 };
 
+// Per-lane books kept in registers and flushed once per lane at the end of a round;
+// event counts are 32-bit (a lane's count within one launch stays far below 2^32).
+// Rare events (majorant violations, trapped particles) go straight to the ledger.
 struct LaneTally {
   long long escaped = 0, unscored = 0, deposited = 0;
-  unsigned long long vox = 0, dep = 0, push = 0, pop = 0, steps = 0;
-  unsigned long long violations = 0, trapped = 0;
+  unsigned int vox = 0, dep = 0, steps = 0;
   // One pending tally cell per lane: consecutive deposits into the same cell are
   // summed here and written with one atomic when the cell changes (integer
   // addition is associative, so the maps are unchanged bit for bit).
@@ -107,26 +162,28 @@ struct LaneTally {
     pend_cell = -1;
   }
 
-  // base = L_P_VOX for the photon round, L_E_VOX for the electron round.
+  // base = L_P_VOX for the photon round, L_E_VOX for the electron round. Every lane
+  // of the wave must reach this (it is called after the lane loop): the books are
+  // summed over the wave and written by lane 0.
   __device__ void flush(unsigned long long* ledger, int base) const {
-    if (escaped) atomicAdd(ledger + L_ESCAPED, static_cast<unsigned long long>(escaped));
-    if (unscored) atomicAdd(ledger + L_UNSCORED, static_cast<unsigned long long>(unscored));
-    if (deposited) atomicAdd(ledger + L_DEPOSITED, static_cast<unsigned long long>(deposited));
-    if (violations) atomicAdd(ledger + L_VIOLATIONS, violations);
-    if (trapped) atomicAdd(ledger + L_TRAPPED, trapped);
-    if (vox) atomicAdd(ledger + base + 0, vox);
-    if (dep) atomicAdd(ledger + base + 1, dep);
-    if (push) atomicAdd(ledger + base + 2, push);
-    if (pop) atomicAdd(ledger + base + 3, pop);
-    if (steps) atomicAdd(ledger + base + 4, steps);
+    const long long books[6] = {escaped, unscored, deposited, static_cast<long long>(vox),
+                                static_cast<long long>(dep), static_cast<long long>(steps)};
+    const int slot[6] = {L_ESCAPED, L_UNSCORED, L_DEPOSITED, base, base + 1, base + 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const long long total = wave_sum(books[k]);
+      if (__lane_id() == 0 && total != 0)
+        atomicAdd(ledger + slot[k], static_cast<unsigned long long>(total));
+    }
   }
 };
 
-__device__ inline void push(const Queue& q, LaneTally& tal, uint32_t kind, uint32_t column,
-                            float e, float w, float x, float y, float z, float ux, float uy,
-                            float uz, uint64_t rng) {
-  const unsigned int idx = atomicAdd(q.count, 1u);
-  tal.push++;
+// Append one particle; the lanes of a wave that push at the same call site share one
+// atomic on the queue count (wave_reserve over the lanes executing the call).
+__device__ inline void push(const Queue& q, uint32_t kind, uint32_t column, float e, float w,
+                            float x, float y, float z, float ux, float uy, float uz,
+                            uint64_t rng) {
+  const unsigned int idx = wave_reserve(q.count, true);
   if (idx < q.capacity) {
     Particle p;
     p.x = x;
@@ -212,9 +269,9 @@ __device__ inline void annihilate(const TransportParams& P, LaneTally& tal, cons
   __sincosf(phi, &sp, &cp);
   const float ax = s * cp, ay = s * sp, az = c;
   const uint64_t r1 = spawn_stream(rng);
-  push(qp, tal, PHOTON, column, kElectronMass, w, x, y, z, ax, ay, az, r1);
+  push(qp, PHOTON, column, kElectronMass, w, x, y, z, ax, ay, az, r1);
   const uint64_t r2 = spawn_stream(rng);
-  push(qp, tal, PHOTON, column, kElectronMass, w, x, y, z, -ax, -ay, -az, r2);
+  push(qp, PHOTON, column, kElectronMass, w, x, y, z, -ax, -ay, -az, r2);
 }
 
 __device__ inline void load_particle(const Particle* data, unsigned int i, float& x, float& y,
@@ -255,7 +312,14 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 // Woodcock flight on the majorant, real/virtual test, channel selection; Compton
 // continues in-lane, photo and pair hand electrons to the electron queue
 // (pyradmc/transport/photon.py:161-282, kernels.py:1171-1529).
-__global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue qin,
+// Waves per SIMD the round kernels are register-budgeted for (a tuning constant:
+// more resident waves hide more of the dependent-load latency of a transport step).
+#ifndef HMC_ROUND_WAVES
+#define HMC_ROUND_WAVES 6
+#endif
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
+k_photon_round(TransportParams P, Queue qin,
                                                       unsigned int* head, Queue qe, Queue qp) {
   const unsigned int n_in = min(*qin.count, qin.capacity);
   LaneTally tal;
@@ -270,13 +334,13 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
   int steps = 0;
   const int n = P.t.n_points;
 
+  RefillPool pool;
   while (true) {
+    const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
-      const unsigned int i = atomicAdd(head, 1u);
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
-      tal.pop++;
       column = meta >> 3;
       base = static_cast<long long>(column) * P.s.n_voxels;
       if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
@@ -290,7 +354,7 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
     }
     tal.steps++;
     if (++steps > P.max_steps) {
-      tal.trapped++;
+      atomicAdd(P.ledger + L_TRAPPED, 1ull);
       escape(tal, w * e);
       active = false;
       continue;
@@ -319,7 +383,7 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
     const float mu_r = rho * c.w;
     const float mu_real = mu_c + mu_p + mu_pp + mu_r;
     if (mu_real > mu_maj) {
-      tal.violations++;
+      atomicAdd(P.ledger + L_VIOLATIONS, 1ull);
       escape(tal, w * e);
       active = false;
       continue;
@@ -346,7 +410,7 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
         float ex = ux, ey = uy, ez = uz;
         rotate(ex, ey, ez, ce, phi + kPi);
         const uint64_t child = spawn_stream(rng);
-        push(qe, tal, ELECTRON, column, recoil, w, x, y, z, ex, ey, ez, child);
+        push(qe, ELECTRON, column, recoil, w, x, y, z, ex, ey, ez, child);
       } else {
         deposit(P, tal, base, x, y, z, w * recoil,
                 spr_factor(P.t, P.dose_to_water, recoil, mat, P.ecut));
@@ -373,7 +437,7 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
     } else if (process == PHOTOELECTRIC) {
       if (P.transport_electrons && e > P.ecut) {
         const uint64_t child = spawn_stream(rng);
-        push(qe, tal, ELECTRON, column, e, w, x, y, z, ux, uy, uz, child);
+        push(qe, ELECTRON, column, e, w, x, y, z, ux, uy, uz, child);
       } else {
         deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
       }
@@ -387,14 +451,14 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
         const float share_p = (1.0f - fraction) * kinetic;
         if (share_e > P.ecut) {
           const uint64_t child = spawn_stream(rng);
-          push(qe, tal, ELECTRON, column, share_e, w, x, y, z, ux, uy, uz, child);
+          push(qe, ELECTRON, column, share_e, w, x, y, z, ux, uy, uz, child);
         } else {
           deposit(P, tal, base, x, y, z, w * share_e,
                   spr_factor(P.t, P.dose_to_water, share_e, mat, P.ecut));
         }
         if (share_p > P.ecut) {
           const uint64_t child = spawn_stream(rng);
-          push(qe, tal, POSITRON, column, share_p, w, x, y, z, ux, uy, uz, child);
+          push(qe, POSITRON, column, share_p, w, x, y, z, ux, uy, uz, child);
         } else {
           deposit(P, tal, base, x, y, z, w * share_p,
                   spr_factor(P.t, P.dose_to_water, share_p, mat, P.ecut));
@@ -418,7 +482,8 @@ __global__ void __launch_bounds__(256) k_photon_round(TransportParams P, Queue q
 // CSDA loss through the restricted range and its inverse, Bernoulli brems, random
 // hinge with GS (or Gaussian) deflection, two half-step deposits, Moller delta
 // (pyradmc/transport/electron.py:292-481, kernels.py:1532-1885).
-__global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue qin,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
+k_electron_round(TransportParams P, Queue qin,
                                                         unsigned int* head, Queue qp, Queue qe) {
   const unsigned int n_in = min(*qin.count, qin.capacity);
   LaneTally tal;
@@ -431,13 +496,13 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
   int steps = 0;
   const int n = P.t.n_points;
 
+  RefillPool pool;
   while (true) {
+    const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
-      const unsigned int i = atomicAdd(head, 1u);
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
-      tal.pop++;
       column = meta >> 3;
       positron = (meta & 3u) == POSITRON;
       base = static_cast<long long>(column) * P.s.n_voxels;
@@ -452,7 +517,7 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
     tal.steps++;
     const float latent = positron ? 2.0f * kElectronMass : 0.0f;
     if (++steps > P.max_steps) {
-      tal.trapped++;
+      atomicAdd(P.ledger + L_TRAPPED, 1ull);
       escape(tal, w * (e + latent));
       active = false;
       continue;
@@ -571,7 +636,7 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
         }
         if (photon_w > 0.0f) {
           const uint64_t child = spawn_stream(rng);
-          push(qp, tal, PHOTON, column, k, photon_w, x, y, z, ux, uy, uz, child);
+          push(qp, PHOTON, column, k, photon_w, x, y, z, ux, uy, uz, child);
         }
       } else {
         deposit_or_escape(P, tal, base, x, y, z, w * k,
@@ -599,7 +664,7 @@ __global__ void __launch_bounds__(256) k_electron_round(TransportParams P, Queue
       float dx = ux, dy = uy, dz = uz;
       rotate(dx, dy, dz, cd, mphi);
       const uint64_t child = spawn_stream(rng);
-      push(qe, tal, ELECTRON, column, delta, w, x, y, z, dx, dy, dz, child);
+      push(qe, ELECTRON, column, delta, w, x, y, z, dx, dy, dz, child);
       rotate(ux, uy, uz, cp, mphi + kPi);
       e -= delta;
     }
@@ -649,23 +714,6 @@ __device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
   return a + (S.edges[k + 1] - a) * u_within;
 }
 
-// Slot reservation for the lanes of one wave that want one: a single atomic per
-// wave, ranks by lane order. Must be reached by every active lane of the wave.
-__device__ inline unsigned int wave_reserve(unsigned int* counter, bool want) {
-  const unsigned long long mask = __ballot(want ? 1 : 0);
-  if (mask == 0ull) return 0u;
-  const int lane = static_cast<int>(__lane_id());
-  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-  unsigned int base = 0u;
-  if (lane == leader) base = atomicAdd(counter, static_cast<unsigned int>(__popcll(mask)));
-  base = __shfl(base, leader);
-  return base + static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
-}
-
-__device__ inline long long wave_sum(long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
 
 // Primaries [p0, p0 + n_gen) of the flattened work items. Single-kind sources write
 // fixed slots of the target queue (thread t -> slot t; the count is set once);
@@ -1396,6 +1444,7 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
     d.beamlet = ctx->d_beamlet;
     ctx->src_beamlets = s->n_beamlets;
   }
+  if (s->kind == HMC_SRC_UPLOADED) d.up_n = 0;  // primaries must be uploaded for this source
   ctx->has_source = true;
   return HMC_OK;
 }
@@ -1481,7 +1530,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
 
   const long long chunk_n = std::min<long long>(chunk, std::max<long long>(total, 1));
   const size_t capacity = static_cast<size_t>(chunk_n) * static_cast<size_t>(queue_factor);
-  if (capacity > 0xFFFFFFF0ull) return fail(ctx, HMC_EINVAL, "queue capacity exceeds 2^32");
+  // Queue heads overshoot the queue length by up to two refill blocks per wave.
+  if (capacity > (1ull << 31)) return fail(ctx, HMC_EINVAL, "queue capacity exceeds 2^31");
   if (capacity > ctx->q_capacity) {
     for (int q = 0; q < 4; ++q)
       if ((rc = dev_alloc(ctx, &ctx->d_q[q], capacity))) return rc;
@@ -1509,6 +1559,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   const unsigned int cap = static_cast<unsigned int>(ctx->q_capacity);
   const long long max_lanes = static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048;
   long long overflow = 0, rounds = 0, launches[2] = {0, 0};
+  // Pops and pushes per kernel, from the queue counts read between rounds.
+  long long pops[2] = {0, 0}, pushes[2] = {0, 0};
   double kernel_ms[2] = {0.0, 0.0};
   bool pending[2] = {false, false};
   auto harvest = [&](int k) -> int {  // after a stream sync
@@ -1531,6 +1583,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
                        ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
                        static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());
+    bool electron_ran = false;
+    unsigned int pb_after_photon = 0;
     for (int round = 0; round < 100000; ++round) {
       unsigned int host_counts[4];
       HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts),
@@ -1540,6 +1594,10 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       for (int q = 0; q < 4; ++q)
         if (host_counts[q] > cap) overflow = std::max<long long>(overflow, host_counts[q]);
       if (overflow) break;
+      if (electron_ran) {  // the last electron round's photons (beyond the photon round's) and electrons
+        pushes[1] += static_cast<long long>(host_counts[pa] - pb_after_photon) + host_counts[ea];
+        electron_ran = false;
+      }
       const unsigned int n_p = host_counts[pa], n_e = host_counts[ea];
       if (n_p == 0 && n_e == 0) break;
       ++rounds;
@@ -1559,15 +1617,19 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
         ++launches[0];
       }
       // Electron round: consume ea (including the photon round's pushes).
-      unsigned int n_e_now = 0;
-      HMC_HIP(ctx, hipMemcpyAsync(&n_e_now, counts + ea, sizeof(unsigned int),
-                                  hipMemcpyDeviceToHost, ctx->stream));
+      unsigned int mid_counts[4];
+      HMC_HIP(ctx, hipMemcpyAsync(mid_counts, counts, sizeof(mid_counts), hipMemcpyDeviceToHost,
+                                  ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
       if ((rc = harvest(0))) return rc;
-      if (n_e_now > cap) {
-        overflow = n_e_now;
+      const unsigned int n_e_now = mid_counts[ea];
+      if (n_e_now > cap || mid_counts[pb] > cap) {
+        overflow = std::max(n_e_now, mid_counts[pb]);
         break;
       }
+      pops[0] += n_p;
+      pushes[0] += static_cast<long long>(n_e_now - n_e) + mid_counts[pb];
+      pb_after_photon = mid_counts[pb];
       if (n_e_now) {
         const long long lanes = std::min<long long>(n_e_now, max_lanes);
         if (ctx->profile) HMC_HIP(ctx, hipEventRecord(ctx->ev[1][0], ctx->stream));
@@ -1579,6 +1641,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
           pending[1] = true;
         }
         ++launches[1];
+        pops[1] += n_e_now;
+        electron_ran = true;
       }
       // Reset the consumed queues (counts and heads), then swap roles.
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + pa, 1, 0u);
@@ -1606,13 +1670,13 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   ledger->rounds = rounds;
   ledger->photon_voxels = L(L_P_VOX);
   ledger->photon_deposits = L(L_P_DEP);
-  ledger->photon_pushes = L(L_P_PUSH);
-  ledger->photon_pops = L(L_P_POP);
+  ledger->photon_pushes = pushes[0];
+  ledger->photon_pops = pops[0];
   ledger->photon_steps = L(L_P_STEPS);
   ledger->electron_voxels = L(L_E_VOX);
   ledger->electron_deposits = L(L_E_DEP);
-  ledger->electron_pushes = L(L_E_PUSH);
-  ledger->electron_pops = L(L_E_POP);
+  ledger->electron_pushes = pushes[1];
+  ledger->electron_pops = pops[1];
   ledger->electron_steps = L(L_E_STEPS);
   ledger->photon_launches = launches[0];
   ledger->electron_launches = launches[1];

@@ -80,6 +80,22 @@ def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
+_BUILTIN_SOURCES = frozenset({
+    "PencilBeamSource", "ParallelBeamSource", "SpectralParallelBeamSource",
+    "SpectralBeamSource", "BeamletGridSource", "SpectralBeamletSource",
+})
+_BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source"})
+_UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
+
+
+def _in_kernel(source) -> bool:
+    """Exact-type routing (pyradmc/backends/warp/engine.py:523-531): only the built-in
+    source types themselves get the in-kernel generator; any other source, a subclass
+    included (it may override emit or sample_batch), is host pre-sampled."""
+    t = type(source)
+    return t.__name__ in _BUILTIN_SOURCES and t.__module__ in _BUILTIN_MODULES
+
+
 def _type_name(obj) -> str:
     return type(obj).__name__
 
@@ -242,7 +258,9 @@ class HipEngine:
         self._ctx.call("hmc_set_physics", nat.ctypes.byref(p))
 
     def _set_source(self, source, primary_kind: str, beamlets: range | None = None) -> tuple[bool, float]:
-        """Describe the source to the generator; returns (analytic_emitted, energy)."""
+        """Describe the source to the generator; returns (analytic_emitted, energy).
+        Sources without an in-kernel generator are set up for host pre-sampling: their
+        primaries are uploaded per transport call (``_upload_primaries``)."""
         d = nat.SourceDesc()
         keep = []
         name = _type_name(source)
@@ -250,6 +268,18 @@ class HipEngine:
         d.particle = particle
         analytic = False
         energy = 0.0
+        if not _in_kernel(source):
+            sampler = "sample_beamlet_batch" if beamlets is not None else "sample_batch"
+            if not callable(getattr(source, sampler, None)):
+                raise ValueError(
+                    f"source type {name} has no in-kernel generator in this engine and no "
+                    f"{sampler}() for host pre-sampling; in-kernel types: "
+                    + ", ".join(sorted(_BUILTIN_SOURCES))
+                )
+            d.kind = nat.SRC_UPLOADED
+            d.particle = 0
+            self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
+            return False, 0.0
 
         def spectrum_arrays(spectrum):
             edges = np.ascontiguousarray(np.asarray(spectrum.edges, dtype=np.float64))
@@ -316,6 +346,23 @@ class HipEngine:
             )
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
         return analytic, energy
+
+    def _upload_primaries(self, samples: list[dict]) -> None:
+        """Upload host-sampled primaries (``sample_batch`` columns, IAEA particle codes
+        1 photon, 2 electron, 3 positron), concatenated in work-item order."""
+        code = np.concatenate([np.asarray(b["particle_type"]).reshape(-1) for b in samples])
+        kind = np.full(code.shape, -1, dtype=np.int32)
+        for iaea, k in ((1, 0), (2, 1), (3, 2)):
+            kind[code == iaea] = k
+        if np.any(kind < 0):
+            bad = sorted(set(np.asarray(code[kind < 0]).tolist()))[:5]
+            raise ValueError(f"sampled particle_type codes {bad} are not photon (1), electron (2) or positron (3)")
+        cols = [np.ascontiguousarray(np.concatenate(
+            [np.asarray(b[c], dtype=np.float32).reshape(-1) for b in samples])) for c in _UPLOAD_COLUMNS]
+        if any(c.shape != kind.shape for c in cols):
+            raise ValueError("sampled primary columns differ in length")
+        self._ctx.call("hmc_set_uploaded", int(kind.size), nat.ptr(kind, nat.c_int32),
+                       *[nat.ptr(c, nat.ctypes.c_float) for c in cols])
 
     def _transport(self, seed: int, items: list[tuple[int, int, int, int]], chunk: int) -> dict:
         """items: (key_base, count, flat tally column, beamlet parameter index)."""
@@ -398,6 +445,7 @@ class HipEngine:
         self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
         self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
         analytic, mono_energy = self._set_source(source, primary_kind)
+        presample = not _in_kernel(source)
         slots = self._batch_slots(n_batches, 1, scoring.n_voxels)
         self._set_scoring(scoring, 1, slots)
 
@@ -409,6 +457,8 @@ class HipEngine:
         for b0 in range(0, n_batches, slots):
             block = range(b0, min(b0 + slots, n_batches))
             items = [(b * per_batch, per_batch, b - b0, 0) for b in block]
+            if presample:
+                self._upload_primaries([source.sample_batch(seed, key, n) for key, n, _, _ in items])
             ledger = self._transport(seed, items, chunk)
             self._merge(books, ledger)
             deposited_quanta += self._fold(len(block), np.full(len(block), float(per_batch)))
@@ -463,8 +513,16 @@ class HipEngine:
         devices=None,
         concurrent_batches: int = 1,
         progress=None,
+        value_dtype=np.float64,
     ) -> DijResult:
-        """Beamlet-resolved dose matrix over the source's beamlets, CSC by column."""
+        """Beamlet-resolved dose matrix over the source's beamlets, CSC by column.
+
+        ``value_dtype`` (an extension; float64 as in the reference by default) may be
+        ``np.float32`` for the CSC dose and sigma arrays, halving a large Dij's host
+        footprint; indices stay int64 and the device statistics stay float64."""
+        value_dtype = np.dtype(value_dtype)
+        if value_dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ValueError(f"value_dtype must be float64 or float32, got {value_dtype}")
         n_beamlets = int(source.n_beamlets)
         counts = np.asarray(n_histories_per_beamlet, dtype=np.int64)
         uneven = counts.ndim == 1
@@ -535,6 +593,9 @@ class HipEngine:
                         r0 = b * int(per_batch[j])
                         key = r0 if correlated else int(offsets[j]) + r0
                         items.append((key, int(per_batch[j]), (b - b0) * n_cols + (j - g0), j - g0))
+                if not _in_kernel(source):
+                    self._upload_primaries([source.sample_beamlet_batch(seed, key, n, g0 + local)
+                                            for key, n, _, local in items])
                 ledger = self._transport(seed, items, chunk)
                 self._merge(books, ledger)
                 nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
@@ -551,8 +612,8 @@ class HipEngine:
                 self._ctx.call("hmc_fetch_csc", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.c_double), nat.ptr(sig, nat.c_double))
             col_counts.append(kept)
             col_idx.append(idx)
-            col_dose.append(dose)
-            col_sigma.append(sig)
+            col_dose.append(dose.astype(value_dtype, copy=False))
+            col_sigma.append(sig.astype(value_dtype, copy=False))
             emitter.tick(group_hist)
 
         indptr = np.zeros(n_beamlets + 1, dtype=np.int64)
@@ -572,8 +633,8 @@ class HipEngine:
             truncation=float(truncation),
             indptr=indptr,
             indices=np.concatenate(col_idx) if col_idx else np.zeros(0, np.int64),
-            dose=np.concatenate(col_dose) if col_dose else np.zeros(0),
-            sigma=np.concatenate(col_sigma) if col_sigma else np.zeros(0),
+            dose=np.concatenate(col_dose) if col_dose else np.zeros(0, value_dtype),
+            sigma=np.concatenate(col_sigma) if col_sigma else np.zeros(0, value_dtype),
             energy_emitted=energy_emitted,
             energy_deposited=energy_deposited,
             energy_escaped=float(books.get("escaped", 0)) * ENERGY_QUANTUM_MEV,

@@ -162,7 +162,16 @@ def bremsstrahlung_6mv_like(n_bins: int = 60, e_lo: float = 0.1, e_hi: float = 6
 
 
 class Source:
-    """Open-field source base (``run``)."""
+    """Open-field source base (``run``).
+
+    The built-in types have in-kernel generators. Any other source (a subclass of a
+    built-in type too) is host pre-sampled, as in the reference
+    (``pyradmc/geometry/source.py:115-146``): it implements
+    ``sample_batch(seed, history_offset, n)`` returning ``particle_type`` (int32 IAEA
+    codes: 1 photon, 2 electron, 3 positron) and float32 columns ``energy, x, y, z,
+    ux, uy, uz, weight`` for histories ``history_offset .. history_offset + n - 1``.
+    History ``h`` then transports on the random stream keyed by ``h``.
+    """
 
     @property
     def max_energy(self) -> float:  # pragma: no cover - interface
@@ -170,7 +179,13 @@ class Source:
 
 
 class BeamletSource:
-    """Beamlet-resolved source base (``run_dij``)."""
+    """Beamlet-resolved source base (``run_dij``).
+
+    Without an in-kernel generator a beamlet source is host pre-sampled through
+    ``sample_beamlet_batch(seed, history_offset, n, beamlet)`` (columns as for
+    ``Source.sample_batch``); ``history_offset`` is the Dij key of the batch's first
+    history of that beamlet (``r`` correlated, ``offset_j + r`` independent).
+    """
 
     @property
     def max_energy(self) -> float:  # pragma: no cover - interface

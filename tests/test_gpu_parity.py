@@ -354,6 +354,31 @@ def test_dij_grouping_and_chunking_bit_identical():
         assert base.energy_deposited == d.energy_deposited
 
 
+def test_ct_dij_three_schedules_bit_identical():
+    # The batched uneven Dij at full phantom size (128x128x96, 100 spectral beamlets,
+    # lognormal counts, truncation 1e-4; counts scaled down 100x): beamlet groups of
+    # 1, 17 and 100 with chunks of 2^14 and 2^20 give the same CSC bit for bit.
+    from pyradmc_amd import HipEngine, workloads
+
+    grid, xs, src, kw = workloads.ct_like_dij(median=2e3)
+    counts = kw["n_histories_per_beamlet"]
+    runs = []
+    for group, chunk in ((1, 1 << 14), (17, 1 << 20), (100, 1 << 14)):
+        eng = HipEngine(grid, xs, chunk_size=chunk)
+        runs.append(eng.run_dij(src, counts, kw["n_batches"], seed=kw["seed"], truncation=kw["truncation"],
+                                beamlet_group_size=group, value_dtype=np.float32))
+        eng.close()
+    a = runs[0]
+    assert a.dose.dtype == np.float32 and a.indptr[-1] > 0
+    for b in runs[1:]:
+        np.testing.assert_array_equal(a.indptr, b.indptr)
+        np.testing.assert_array_equal(a.indices, b.indices)
+        np.testing.assert_array_equal(a.dose, b.dose)
+        np.testing.assert_array_equal(a.sigma, b.sigma)
+        assert a.energy_deposited == b.energy_deposited
+    assert _balance(a) < 1e-6
+
+
 def test_dij_matches_reference_engine(golden_dir):
     from pyradmc_amd import BeamletGridSource, VoxelGrid
     from pyradmc_amd.data.analytic import AnalyticCrossSections
@@ -436,6 +461,115 @@ def test_uneven_dij_matches_oracle(oracle_lib):
     assert _balance(d) < 1e-6
 
 
+# ------------------------------------------------------- host pre-sampled sources
+
+
+def _host_columns(n, rng, energy, x, y, z, kinds=None):
+    ones = np.ones(n, dtype=np.float32)
+    return {"particle_type": np.ones(n, dtype=np.int32) if kinds is None else kinds,
+            "energy": np.asarray(energy, np.float32) * ones, "x": np.asarray(x, np.float32),
+            "y": np.asarray(y, np.float32), "z": np.full(n, z, np.float32),
+            "ux": 0 * ones, "uy": 0 * ones, "uz": ones, "weight": ones}
+
+
+class _HostParallel:
+    """A user source with no in-kernel generator: a 2 MeV parallel field, host-sampled."""
+    max_energy = 2.0
+
+    def sample_batch(self, seed, history_offset, n):
+        rng = np.random.default_rng([seed, history_offset])
+        return _host_columns(n, rng, 2.0, 2.0 + 4.0 * rng.random(n), 2.0 + 4.0 * rng.random(n), -0.1)
+
+
+def test_presampled_source_matches_in_kernel_generator():
+    from pyradmc_amd import ParallelBeamSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((16, 16, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    eng = _engine(grid, xs)
+    a = eng.run(ParallelBeamSource(2.0, -0.1, (2.0, 6.0), (2.0, 6.0)), 2_000_000, 10, seed=5)
+    b = eng.run(_HostParallel(), 2_000_000, 10, seed=6)
+    eng.close()
+    pooled = 0.5 * (a.dose + b.dose)
+    mask = pooled > 0.1 * pooled.max()
+    p, *_ = batched_chi2_pvalue(a.dose, a.dose_sigma, 10, b.dose, b.dose_sigma, 10, mask)
+    assert p > ALPHA
+    assert b.energy_emitted == pytest.approx(2_000_000 * 2.0, rel=1e-12)  # booked on the device
+    assert _balance(b) < 1e-6
+
+
+class _MixedSource:
+    """Photons, electrons and positrons (IAEA codes 1, 2, 3) with weights."""
+    max_energy = 3.0
+
+    def sample_batch(self, seed, history_offset, n):
+        rng = np.random.default_rng([seed, history_offset, 7])
+        cols = _host_columns(n, rng, 1.0 + 2.0 * rng.random(n), 2.0 + 4.0 * rng.random(n),
+                             2.0 + 4.0 * rng.random(n), 0.25,
+                             kinds=(1 + np.arange(history_offset, history_offset + n) % 3).astype(np.int32))
+        cols["weight"] = rng.uniform(0.5, 1.5, n).astype(np.float32)
+        return cols
+
+
+def test_presampled_mixed_particles_close_the_ledger():
+    from pyradmc_amd import VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((16, 16, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    eng = _engine(grid, xs)
+    src = _MixedSource()
+    r = eng.run(src, 300_000, 6, seed=9)
+    again = eng.run(src, 300_000, 6, seed=9)
+    eng.close()
+    np.testing.assert_array_equal(r.dose, again.dose)
+    expected = 0.0
+    for b in range(6):
+        c = src.sample_batch(9, b * 50_000, 50_000)
+        latent = np.where(c["particle_type"] == 3, 2.0 * 0.51099895069, 0.0)
+        expected += float(np.sum(c["weight"].astype(np.float64) * (c["energy"] + latent)))
+    assert r.energy_emitted == pytest.approx(expected, rel=1e-6)
+    assert _balance(r) < 1e-6
+    assert r.energy_deposited > 0.1 * r.energy_emitted
+
+
+class _HostLattice:
+    """The 2 MeV 2x2 lattice of BeamletGridSource(2.0, -0.1, (1, 3), (1, 3), 2, 2),
+    host-sampled per beamlet."""
+    max_energy = 2.0
+    n_beamlets = 4
+
+    def sample_beamlet_batch(self, seed, history_offset, n, beamlet):
+        rng = np.random.default_rng([seed, history_offset, beamlet])
+        x0 = 1.0 + (beamlet // 2)
+        y0 = 1.0 + (beamlet % 2)
+        return _host_columns(n, rng, 2.0, x0 + rng.random(n), y0 + rng.random(n), -0.1)
+
+
+def test_presampled_beamlet_source_matches_lattice():
+    from pyradmc_amd import BeamletGridSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((8, 8, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    lattice = BeamletGridSource(2.0, -0.1, (1.0, 3.0), (1.0, 3.0), 2, 2)
+    eng = _engine(grid, xs)
+    a = eng.run_dij(lattice, 200_000, 10, seed=3, truncation=0.0)
+    b = eng.run_dij(_HostLattice(), 200_000, 10, seed=4, truncation=0.0)
+    eng.close()
+    for j in range(4):
+        x0, x1, y0, y1 = lattice.beamlet_bounds(j)
+        assert (x0, y0) == (1.0 + j // 2, 1.0 + j % 2)  # the host twin's column order
+        da, sa = a.column_dense(j).ravel(), a.sigma_dense(j).ravel()
+        db, sb = b.column_dense(j).ravel(), b.sigma_dense(j).ravel()
+        pooled = 0.5 * (da + db)
+        mask = pooled > 0.1 * pooled.max()
+        p, *_ = batched_chi2_pvalue(da, sa, 10, db, sb, 10, mask)
+        assert p > ALPHA, (j, p)
+    assert b.energy_emitted == pytest.approx(4 * 200_000 * 2.0, rel=1e-12)
+
+
 # ------------------------------------------------------------- failure paths
 
 
@@ -473,3 +607,20 @@ def test_argument_validation():
         eng.run(src, 10, 2, seed=1, msc_model="x")
     with pytest.raises(ValueError):
         eng.run(src, 10, 2, seed=1, scoring_mode="dose_to_water", transport_electrons=False)
+
+    class BadCodes(_HostParallel):
+        def sample_batch(self, seed, history_offset, n):
+            cols = super().sample_batch(seed, history_offset, n)
+            cols["particle_type"][::3] = 5  # not a photon, electron or positron code
+            return cols
+
+    with pytest.raises(ValueError, match="particle_type"):
+        eng.run(BadCodes(), 30, 2, seed=1)
+
+    class NoSampler:
+        max_energy = 2.0
+
+    with pytest.raises(ValueError, match="pre-sampling"):
+        eng.run(NoSampler(), 10, 2, seed=1)
+    eng.run(src, 10, 2, seed=1)  # the engine stays usable after rejected calls
+    eng.close()
