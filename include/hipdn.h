@@ -1,0 +1,81 @@
+/* hipdn: kernels of the 3D convolutional Dij-column denoiser on one MI355X (gfx950).
+ *
+ * The reference has no learned component: these entry points have no counterpart in
+ * it. They serve the denoised Dij builder (pyradmc_amd/denoise.py), which predicts
+ * full-statistics beamlet columns from low-history ones and falls back to plain Monte
+ * Carlo for any column that fails its consistency check.
+ *
+ * Conventions. Every pointer argument is device memory owned by the caller; `stream`
+ * is a hipStream_t (NULL = the null stream). Activations are bfloat16, position-major
+ * ("nd": [N][D][H][W][C], C a multiple of 16) with an optional channel-major copy
+ * ("cm": [C][N*D*H*W]). Convolutions are 3x3x3, stride 1, zero padding. Weights: FP32
+ * master [27][Cout][Cin]; BF16 forward copy [27][Cout][Cin]; BF16 data-gradient copy
+ * [27][Cin][Cout] with flipped taps. Calls return HDN_OK or an error code and are
+ * asynchronous on `stream`.
+ */
+#ifndef HIPDN_H
+#define HIPDN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDN_OK 0
+#define HDN_EINVAL -1
+#define HDN_EHIP -2
+
+/* y = act(bias + conv(x, w)) with act(v) = v >= 0 ? v : slope v (slope 1: identity);
+ * y_cm may be NULL; bias may be NULL. N*D*H*W % 64 == 0, cin % 16 == 0,
+ * cout % 16 == 0. With w = the data-gradient copy, x = dz and bias = NULL, slope = 1
+ * this is the data gradient of a convolution. */
+int hdn_conv3d_fwd(const void* x, const void* w, const float* bias, void* y_nd, void* y_cm, int n,
+                   int d, int h, int w_, int cin, int cout, float slope, void* stream);
+
+/* Weight-gradient partials part[splits][27][cout][cin] (FP32) from the channel-major
+ * layer input and output gradient; W % 8 == 0. Sum them with hdn_reduce_splits. */
+int hdn_conv3d_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d,
+                     int h, int w_, int cin, int cout, void* stream);
+
+/* out[i] = sum over s of part[s][i], in split order (deterministic). */
+int hdn_reduce_splits(const float* part, float* out, long long len, int splits, void* stream);
+
+/* Per-channel sums of a channel-major tensor over its P positions: part[splits][C]. */
+int hdn_channel_sum(const void* cm, float* part, long long P, int C, int splits, void* stream);
+
+/* dz = dy * act'(y) for the leaky activation (from the stored output y). */
+int hdn_act_bwd(const void* y, const void* dy, void* dz_nd, void* dz_cm, long long P, int C,
+                float slope, void* stream);
+
+/* y = a + b. */
+int hdn_add(const void* a, const void* b, void* y_nd, void* y_cm, long long P, int C, void* stream);
+
+/* 2x2x2 average pooling to the coarse dims (n, d, h, w) and its gradient (fine dims). */
+int hdn_pool_fwd(const void* x, void* y_nd, void* y_cm, int n, int d, int h, int w_, int C,
+                 void* stream);
+int hdn_pool_bwd(const void* dy, void* dx, int n, int d2, int h2, int w2, int C, void* stream);
+
+/* Nearest 2x upsampling to the fine dims and its gradient (coarse dims). */
+int hdn_up_fwd(const void* x, void* y_nd, void* y_cm, int n, int d2, int h2, int w2, int C,
+               void* stream);
+int hdn_up_bwd(const void* dy, void* dx, int n, int d, int h, int w_, int C, void* stream);
+
+/* Residual head and weighted squared error: pred = inp[:, 0] + out[:, 0];
+ * partial[block] = sum weight (pred - target)^2 (blocks of 256 positions);
+ * dout = 2 weight (pred - target) grad_scale in channel 0 (others 0). target, weight,
+ * dout_nd, dout_cm, pred and partial may each be NULL (prediction only). */
+int hdn_loss(const void* out, const void* inp, int c_out, int c_in, const float* target,
+             const float* weight, void* dout_nd, void* dout_cm, float* pred, float* partial,
+             long long P, float grad_scale, void* stream);
+
+/* BF16 forward and data-gradient copies of a FP32 master weight. */
+int hdn_prepare_weights(const float* master, void* wf, void* wd, int cin, int cout, void* stream);
+
+/* One Adam step (bias-corrected; step counts from 1) with decoupled L2 term
+ * weight_decay * param added to the gradient. */
+int hdn_adam(float* param, const float* grad, float* m, float* v, long long len, float lr,
+             float beta1, float beta2, float eps, int step, float weight_decay, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

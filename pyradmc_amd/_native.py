@@ -168,11 +168,14 @@ def lib():
 
 
 def exported_symbols() -> list[str]:
-    """Every ``hmc_*`` entry point the header declares."""
-    header = Path(__file__).resolve().parent.parent / "include" / "hipmc.h"
+    """Every ``hmc_*`` / ``hdn_*`` entry point the headers declare."""
     import re
 
-    return sorted(set(re.findall(r"\b(hmc_[a-z_]+)\s*\(", header.read_text())))
+    include = Path(__file__).resolve().parent.parent / "include"
+    names = set()
+    for header, prefix in (("hipmc.h", "hmc"), ("hipdn.h", "hdn")):
+        names |= set(re.findall(rf"\b({prefix}_[a-z0-9_]+)\s*\(", (include / header).read_text()))
+    return sorted(names)
 
 
 def ptr(array: np.ndarray, ctype):

@@ -1,9 +1,11 @@
 """Compile the native library ``libhipmc.so`` in-tree for gfx950.
 
-``hipmc_kernels.hip`` (device kernels and the C ABI) is compiled by hipcc for
-``--offload-arch=gfx950``; ``gs_build.cpp`` (host-only GS grid builder) by g++; both
-link into one shared library beside this file, which travels with the source tree.
-Packed-FP32 instruction selection is disabled for the device code.
+``hipmc_kernels.hip`` (transport kernels and their C ABI, ``include/hipmc.h``) and
+``denoise_kernels.hip`` (the denoiser's MFMA convolution kernels, ``include/hipdn.h``)
+are compiled by hipcc for ``--offload-arch=gfx950``; ``gs_build.cpp`` (host-only GS
+grid builder) by g++; all link into one shared library beside this file, which
+travels with the source tree. Packed-FP32 instruction selection is disabled for the
+device code.
 """
 
 from __future__ import annotations
@@ -57,10 +59,16 @@ def build(verbose: bool = False) -> Path:
     build_dir = HERE / "_build"
     build_dir.mkdir(exist_ok=True)
     kernels_o = build_dir / "hipmc_kernels.o"
+    denoise_o = build_dir / "denoise_kernels.o"
     gs_o = build_dir / "gs_build.o"
     _run(
         [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
          str(CSRC / "hipmc_kernels.hip"), "-o", str(kernels_o)],
+        verbose,
+    )
+    _run(
+        [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
+         str(CSRC / "denoise_kernels.hip"), "-o", str(denoise_o)],
         verbose,
     )
     _run(
@@ -71,7 +79,7 @@ def build(verbose: bool = False) -> Path:
     tmp = HERE / (LIB_NAME + ".tmp")
     _run(
         [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", str(kernels_o),
-         str(gs_o), "-o", str(tmp)],
+         str(denoise_o), str(gs_o), "-o", str(tmp)],
         verbose,
     )
     os.replace(tmp, LIB_PATH)

@@ -1,0 +1,653 @@
+// Kernels of the 3D convolutional Dij-column denoiser: BF16 MFMA convolutions
+// (forward, data gradient, weight gradient) with FP32 accumulation, and the
+// elementwise, pooling, loss, reduction and optimizer kernels of the network, behind
+// the C ABI of include/hipdn.h. The reference has no learned component; these
+// kernels serve the denoised Dij builder (pyradmc_amd/denoise.py).
+//
+// Layouts. Activations are BF16, position-major "nd" = [N][D][H][W][C] with C a
+// multiple of 16; a tensor that feeds a weight gradient also has a channel-major copy
+// "cm" = [C][N*D*H*W]. Weights: FP32 master [27][Cout][Cin] (tap = (kz*3+ky)*3+kx);
+// BF16 forward copy wf = [27][Cout][Cin]; BF16 data-gradient copy wd = [27][Cin][Cout]
+// with the taps flipped (tap -> 26 - tap), so the data gradient is a forward
+// convolution of the output gradient.
+//
+// MFMA: v_mfma_f32_16x16x32_bf16 (gfx950). For D = A x B with A 16x32, B 32x16:
+// lane l holds A[l % 16][8 (l / 16) + e], B[8 (l / 16) + e][l % 16] (e = 0..7), and
+// D[4 (l / 16) + r][l % 16] (r = 0..3). Every operand is one aligned 16-byte load.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/hipdn.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) z[e] = static_cast<__bf16>(0.0f);
+  return z;
+}
+
+__device__ inline float act(float v, float slope) { return v >= 0.0f ? v : v * slope; }
+
+// ------------------------------------------------------------------ convolution
+//
+// y[p][co] = act(bias[co] + sum_{tap, ci} x[p + off(tap)][ci] wt[tap][co][ci]) with
+// zero padding. One wave computes 16*TM output channels x 64 positions (4 blocks of
+// 16); a workgroup of 4 waves covers 256 consecutive positions (P % 64 == 0). K = 27*cin runs in
+// chunks of 32; each lane's 8 k values lie inside one tap since cin % 8 == 0.
+template <int TM>
+__global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
+                                                  const __bf16* __restrict__ wt,
+                                                  const float* __restrict__ bias,
+                                                  __bf16* __restrict__ y_nd,
+                                                  __bf16* __restrict__ y_cm, int n, int d, int h,
+                                                  int w, int cin, int cout, float slope) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  const long long p0 = (static_cast<long long>(blockIdx.x) * 4 + wave) * 64;
+  if (p0 >= P) return;  // whole wave past the end (P % 64 == 0); no barriers follow
+  const int co0 = blockIdx.y * 16 * TM;
+  const int col = lane & 15;
+  const int kg = lane >> 4;
+
+  int pn[4], pz[4], py[4], px[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    long long t = p0 + 16 * nb + col;
+    px[nb] = static_cast<int>(t % w);
+    t /= w;
+    py[nb] = static_cast<int>(t % h);
+    t /= h;
+    pz[nb] = static_cast<int>(t % d);
+    pn[nb] = static_cast<int>(t / d);
+  }
+
+  f32x4 acc[TM][4];
+#pragma unroll
+  for (int m = 0; m < TM; ++m)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[m][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  const int K = 27 * cin;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + 8 * kg;
+    bf16x8 a[TM], b[4];
+    if (k < K) {
+      const int tap = k / cin;
+      const int c = k - tap * cin;
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+      for (int m = 0; m < TM; ++m)
+        a[m] = load8(wt + (static_cast<long long>(tap) * cout + co0 + 16 * m + col) * cin + c);
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        const int zz = pz[nb] + dz, yy = py[nb] + dy, xx = px[nb] + dx;
+        if (zz >= 0 && zz < d && yy >= 0 && yy < h && xx >= 0 && xx < w) {
+          const long long q = ((static_cast<long long>(pn[nb]) * d + zz) * h + yy) * w + xx;
+          b[nb] = load8(x + q * cin + c);
+        } else {
+          b[nb] = zero8();
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < TM; ++m) a[m] = zero8();
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) b[nb] = zero8();
+    }
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) acc[m][nb] = mfma16(a[m], b[nb], acc[m][nb]);
+  }
+
+#pragma unroll
+  for (int m = 0; m < TM; ++m) {
+    const int cb = co0 + 16 * m + 4 * kg;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const long long p = p0 + 16 * nb + col;
+      __bf16 out[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = acc[m][nb][r] + (bias ? bias[cb + r] : 0.0f);
+        out[r] = static_cast<__bf16>(act(v, slope));
+      }
+      uint2 packed;
+      std::memcpy(&packed, out, 8);
+      *reinterpret_cast<uint2*>(y_nd + p * cout + cb) = packed;
+      if (y_cm) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y_cm[static_cast<long long>(cb + r) * P + p] = out[r];
+      }
+    }
+  }
+}
+
+// 8 channel-major elements x[c][row + x0 + dx .. + 7] of one W-row (w % 8 == 0,
+// x0 % 8 == 0), zero outside [0, w): the tap's x shift as a funnel shift of aligned
+// 16-byte loads.
+__device__ inline bf16x8 load_row8(const __bf16* row, int x0, int w, int dx) {
+  const u32x4 cur = *reinterpret_cast<const u32x4*>(row + x0);
+  u32x4 out;
+  if (dx == 0) {
+    out = cur;
+  } else if (dx > 0) {
+    const u32x4 nxt = x0 + 8 < w ? *reinterpret_cast<const u32x4*>(row + x0 + 8) : u32x4{0, 0, 0, 0};
+    out[0] = (cur[0] >> 16) | (cur[1] << 16);
+    out[1] = (cur[1] >> 16) | (cur[2] << 16);
+    out[2] = (cur[2] >> 16) | (cur[3] << 16);
+    out[3] = (cur[3] >> 16) | (nxt[0] << 16);
+  } else {
+    const u32x4 prv = x0 > 0 ? *reinterpret_cast<const u32x4*>(row + x0 - 8) : u32x4{0, 0, 0, 0};
+    out[0] = (cur[0] << 16) | (prv[3] >> 16);
+    out[1] = (cur[1] << 16) | (cur[0] >> 16);
+    out[2] = (cur[2] << 16) | (cur[1] >> 16);
+    out[3] = (cur[3] << 16) | (cur[2] >> 16);
+  }
+  bf16x8 r;
+  std::memcpy(&r, &out, 16);
+  return r;
+}
+
+// Weight-gradient partials: part[s][tap][co][ci] = sum over split s's positions of
+// dz[p][co] x[p + off(tap)][ci]. One wave owns one 16x16 (ci, co) block for all 27
+// taps (108 accumulators) and a contiguous range of positions, in chunks of 32.
+__global__ void __launch_bounds__(64) k_conv_wgrad(const __bf16* __restrict__ x_cm,
+                                                   const __bf16* __restrict__ dz_cm,
+                                                   float* __restrict__ part, int n, int d, int h,
+                                                   int w, int cin, int cout, long long per_split) {
+  const int lane = threadIdx.x;
+  const int col = lane & 15;
+  const int kg = lane >> 4;
+  const int ci0 = 16 * blockIdx.x;
+  const int co0 = 16 * blockIdx.y;
+  const int split = blockIdx.z;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  const long long pbeg = static_cast<long long>(split) * per_split;
+  const long long pend = pbeg + per_split < P ? pbeg + per_split : P;
+
+  f32x4 acc[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  const __bf16* xrow_c = x_cm + static_cast<long long>(ci0 + col) * P;
+  const __bf16* dz_c = dz_cm + static_cast<long long>(co0 + col) * P;
+  for (long long pb = pbeg; pb < pend; pb += 32) {
+    const long long p = pb + 8 * kg;  // this lane's 8 positions: one aligned W-row segment
+    const bf16x8 b = load8(dz_c + p);
+    long long t = p;
+    const int x0 = static_cast<int>(t % w);
+    t /= w;
+    const int y = static_cast<int>(t % h);
+    t /= h;
+    const int z = static_cast<int>(t % d);
+    const long long nn = t / d;
+#pragma unroll
+    for (int dzz = -1; dzz <= 1; ++dzz) {
+#pragma unroll
+      for (int dyy = -1; dyy <= 1; ++dyy) {
+        const int zz = z + dzz, yy = y + dyy;
+        const bool row_ok = zz >= 0 && zz < d && yy >= 0 && yy < h;
+        const long long row_off = row_ok ? ((nn * d + zz) * h + yy) * w : 0;
+#pragma unroll
+        for (int dxx = -1; dxx <= 1; ++dxx) {
+          const int tap = ((dzz + 1) * 3 + (dyy + 1)) * 3 + (dxx + 1);
+          const bf16x8 a = row_ok ? load_row8(xrow_c + row_off, x0, w, dxx) : zero8();
+          acc[tap] = mfma16(a, b, acc[tap]);
+        }
+      }
+    }
+  }
+  // D[i = ci][j = co]: lane holds ci = ci0 + 4 kg + r, co = co0 + col.
+  float* out = part + static_cast<long long>(split) * 27 * cout * cin;
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap) {
+    float4 v = make_float4(acc[tap][0], acc[tap][1], acc[tap][2], acc[tap][3]);
+    *reinterpret_cast<float4*>(out + (static_cast<long long>(tap) * cout + co0 + col) * cin + ci0 +
+                               4 * kg) = v;
+  }
+}
+
+// out[i] = sum_s part[s][i], in split order (deterministic).
+__global__ void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out,
+                                long long len, int splits) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    float s = 0.0f;
+    for (int k = 0; k < splits; ++k) s += part[static_cast<long long>(k) * len + i];
+    out[i] = s;
+  }
+}
+
+// Per-channel sums of a channel-major tensor: part[s][c] over split s of the positions.
+__global__ void __launch_bounds__(256) k_channel_sum(const __bf16* __restrict__ cm,
+                                                     float* __restrict__ part, long long P,
+                                                     long long per_split) {
+  const int c = blockIdx.x;
+  const int s = blockIdx.y;
+  const long long beg = static_cast<long long>(s) * per_split;
+  const long long end = beg + per_split < P ? beg + per_split : P;
+  float acc = 0.0f;
+  for (long long p = beg + threadIdx.x; p < end; p += blockDim.x)
+    acc += static_cast<float>(cm[static_cast<long long>(c) * P + p]);
+  __shared__ float red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (static_cast<int>(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[static_cast<long long>(s) * gridDim.x + c] = red[0];  // [split][C]
+}
+
+// ------------------------------------------------------------------- elementwise
+//
+// Thread = (position p, 8-channel group g); p runs fastest so the channel-major
+// writes of a wave are contiguous.
+
+__device__ inline void store_both(__bf16* nd, __bf16* cm, long long P, int C, long long p, int g,
+                                  const float* v) {
+  __bf16 o[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) o[e] = static_cast<__bf16>(v[e]);
+  bf16x8 packed;
+  std::memcpy(&packed, o, 16);
+  *reinterpret_cast<bf16x8*>(nd + p * C + 8 * g) = packed;
+  if (cm) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) cm[static_cast<long long>(8 * g + e) * P + p] = o[e];
+  }
+}
+
+// dz = dy * act'(y) (act'(y) = 1 for y > 0, slope otherwise), both layouts.
+__global__ void k_act_bwd(const __bf16* __restrict__ y, const __bf16* __restrict__ dy,
+                          __bf16* __restrict__ dz_nd, __bf16* __restrict__ dz_cm, long long P,
+                          int C, float slope) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  const bf16x8 yv = load8(y + p * C + 8 * g);
+  const bf16x8 gv = load8(dy + p * C + 8 * g);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float gy = static_cast<float>(gv[e]);
+    v[e] = static_cast<float>(yv[e]) > 0.0f ? gy : gy * slope;
+  }
+  store_both(dz_nd, dz_cm, P, C, p, g, v);
+}
+
+// y = a + b, both layouts.
+__global__ void k_add(const __bf16* __restrict__ a, const __bf16* __restrict__ b,
+                      __bf16* __restrict__ y_nd, __bf16* __restrict__ y_cm, long long P, int C) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  const bf16x8 av = load8(a + p * C + 8 * g);
+  const bf16x8 bv = load8(b + p * C + 8 * g);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = static_cast<float>(av[e]) + static_cast<float>(bv[e]);
+  store_both(y_nd, y_cm, P, C, p, g, v);
+}
+
+// 2x2x2 average pooling: x [n][2d][2h][2w][C] -> y [n][d][h][w][C] (nd and cm).
+__global__ void k_pool_fwd(const __bf16* __restrict__ x, __bf16* __restrict__ y_nd,
+                           __bf16* __restrict__ y_cm, int n, int d, int h, int w, int C) {
+  const long long P = static_cast<long long>(n) * d * h * w;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  long long t = p;
+  const int xx = static_cast<int>(t % w);
+  t /= w;
+  const int yy = static_cast<int>(t % h);
+  t /= h;
+  const int zz = static_cast<int>(t % d);
+  const long long nn = t / d;
+  float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) {
+        const long long q = ((nn * 2 * d + 2 * zz + a) * 2 * h + 2 * yy + b) * 2 * w + 2 * xx + c;
+        const bf16x8 s = load8(x + q * C + 8 * g);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] += static_cast<float>(s[e]);
+      }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] *= 0.125f;
+  store_both(y_nd, y_cm, P, C, p, g, v);
+}
+
+// Gradient of the pooling: dx[fine] = dy[coarse] / 8 (nd only). Dims are the fine
+// ones (2d, 2h, 2w).
+__global__ void k_pool_bwd(const __bf16* __restrict__ dy, __bf16* __restrict__ dx, int n, int d2,
+                           int h2, int w2, int C) {
+  const long long P = static_cast<long long>(n) * d2 * h2 * w2;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  long long t = p;
+  const int xx = static_cast<int>(t % w2);
+  t /= w2;
+  const int yy = static_cast<int>(t % h2);
+  t /= h2;
+  const int zz = static_cast<int>(t % d2);
+  const long long nn = t / d2;
+  const long long q = ((nn * (d2 / 2) + zz / 2) * (h2 / 2) + yy / 2) * (w2 / 2) + xx / 2;
+  const bf16x8 s = load8(dy + q * C + 8 * g);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = static_cast<float>(s[e]) * 0.125f;
+  store_both(dx, nullptr, P, C, p, g, v);
+}
+
+// Nearest 2x upsampling: x [n][d][h][w][C] -> y [n][2d][2h][2w][C]. Dims are the fine
+// ones.
+__global__ void k_up_fwd(const __bf16* __restrict__ x, __bf16* __restrict__ y_nd,
+                         __bf16* __restrict__ y_cm, int n, int d2, int h2, int w2, int C) {
+  const long long P = static_cast<long long>(n) * d2 * h2 * w2;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  long long t = p;
+  const int xx = static_cast<int>(t % w2);
+  t /= w2;
+  const int yy = static_cast<int>(t % h2);
+  t /= h2;
+  const int zz = static_cast<int>(t % d2);
+  const long long nn = t / d2;
+  const long long q = ((nn * (d2 / 2) + zz / 2) * (h2 / 2) + yy / 2) * (w2 / 2) + xx / 2;
+  const bf16x8 s = load8(x + q * C + 8 * g);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = static_cast<float>(s[e]);
+  store_both(y_nd, y_cm, P, C, p, g, v);
+}
+
+// Gradient of the upsampling: dx[coarse] = sum of dy over its 8 fine cells (nd only).
+// Dims are the coarse ones.
+__global__ void k_up_bwd(const __bf16* __restrict__ dy, __bf16* __restrict__ dx, int n, int d,
+                         int h, int w, int C) {
+  const long long P = static_cast<long long>(n) * d * h * w;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i % P;
+  const int g = static_cast<int>(i / P);
+  long long t = p;
+  const int xx = static_cast<int>(t % w);
+  t /= w;
+  const int yy = static_cast<int>(t % h);
+  t /= h;
+  const int zz = static_cast<int>(t % d);
+  const long long nn = t / d;
+  float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) {
+        const long long q = ((nn * 2 * d + 2 * zz + a) * 2 * h + 2 * yy + b) * 2 * w + 2 * xx + c;
+        const bf16x8 s = load8(dy + q * C + 8 * g);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] += static_cast<float>(s[e]);
+      }
+  store_both(dx, nullptr, P, C, p, g, v);
+}
+
+// Residual head and sigma-weighted squared error. pred = inp[p][0] + out[p][0];
+// per-block partial of sum weight * (pred - target)^2; gradient with respect to out
+// (channel 0; the other channels zero) = 2 weight (pred - target) * grad_scale, in
+// both layouts. Also writes pred (float) when pred_out is not null.
+__global__ void __launch_bounds__(256) k_loss(const __bf16* __restrict__ out,
+                                              const __bf16* __restrict__ inp, int c_out, int c_in,
+                                              const float* __restrict__ target,
+                                              const float* __restrict__ weight,
+                                              __bf16* __restrict__ dout_nd,
+                                              __bf16* __restrict__ dout_cm,
+                                              float* __restrict__ pred_out,
+                                              float* __restrict__ partial, long long P,
+                                              float grad_scale) {
+  const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (p < P) {
+    const float pred = static_cast<float>(inp[p * c_in]) + static_cast<float>(out[p * c_out]);
+    if (pred_out) pred_out[p] = pred;
+    if (target) {
+      const float diff = pred - target[p];
+      const float wt = weight ? weight[p] : 1.0f;
+      l = wt * diff * diff;
+      if (dout_nd) {
+        const __bf16 g = static_cast<__bf16>(2.0f * wt * diff * grad_scale);
+        const __bf16 zero = static_cast<__bf16>(0.0f);
+        for (int c = 0; c < c_out; ++c) {
+          dout_nd[p * c_out + c] = c == 0 ? g : zero;
+          if (dout_cm) dout_cm[static_cast<long long>(c) * P + p] = c == 0 ? g : zero;
+        }
+      }
+    }
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (static_cast<int>(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && partial) partial[blockIdx.x] = red[0];
+}
+
+// FP32 master -> BF16 forward [27][cout][cin] and flipped, transposed data-gradient
+// [27][cin][cout] copies.
+__global__ void k_prepare_weights(const float* __restrict__ master, __bf16* __restrict__ wf,
+                                  __bf16* __restrict__ wd, int cin, int cout) {
+  const long long total = 27LL * cin * cout;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int ci = static_cast<int>(i % cin);
+  const long long t = i / cin;
+  const int co = static_cast<int>(t % cout);
+  const int tap = static_cast<int>(t / cout);
+  const __bf16 v = static_cast<__bf16>(master[i]);
+  wf[i] = v;
+  wd[(static_cast<long long>(26 - tap) * cin + ci) * cout + co] = v;
+}
+
+// Adam (Kingma and Ba 2015) with bias correction; zero gradients leave padding zero.
+__global__ void k_adam(float* __restrict__ param, const float* __restrict__ grad,
+                       float* __restrict__ m, float* __restrict__ v, long long len, float lr,
+                       float beta1, float beta2, float eps, float corr1, float corr2,
+                       float weight_decay) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const float g = grad[i] + weight_decay * param[i];
+  const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  param[i] -= lr * (mi / corr1) / (sqrtf(vi / corr2) + eps);
+}
+
+unsigned int blocks_for(long long n, int threads) {
+  return static_cast<unsigned int>((n + threads - 1) / threads);
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int launch_status() { return hipGetLastError() == hipSuccess ? HDN_OK : HDN_EHIP; }
+
+bool shape_ok(int n, int d, int h, int w) { return n > 0 && d > 0 && h > 0 && w > 0; }
+
+}  // namespace
+
+extern "C" {
+
+int hdn_conv3d_fwd(const void* x, const void* wt, const float* bias, void* y_nd, void* y_cm, int n,
+                   int d, int h, int w, int cin, int cout, float slope, void* stream) {
+  if (!x || !wt || !y_nd || !shape_ok(n, d, h, w)) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  if (P % 64 != 0 || cin % 16 != 0 || cin < 16 || cout % 16 != 0 || cout < 16) return HDN_EINVAL;
+  const int tm = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+  const dim3 grid(static_cast<unsigned int>((P + 255) / 256), static_cast<unsigned int>(cout / (16 * tm)));
+  const __bf16* xb = static_cast<const __bf16*>(x);
+  const __bf16* wb = static_cast<const __bf16*>(wt);
+  __bf16* yb = static_cast<__bf16*>(y_nd);
+  __bf16* cb = static_cast<__bf16*>(y_cm);
+  if (tm == 4)
+    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
+                       n, d, h, w, cin, cout, slope);
+  else if (tm == 2)
+    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
+                       n, d, h, w, cin, cout, slope);
+  else
+    hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
+                       n, d, h, w, cin, cout, slope);
+  return launch_status();
+}
+
+int hdn_conv3d_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d,
+                     int h, int w, int cin, int cout, void* stream) {
+  if (!x_cm || !dz_cm || !part || splits < 1 || !shape_ok(n, d, h, w)) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  if (w % 8 != 0 || P % 32 != 0 || cin % 16 != 0 || cout % 16 != 0) return HDN_EINVAL;
+  long long per_split = (P + splits - 1) / splits;
+  per_split = (per_split + 31) / 32 * 32;
+  if (per_split * (splits - 1) >= P) return HDN_EINVAL;  // every split must own positions
+  const dim3 grid(static_cast<unsigned int>(cin / 16), static_cast<unsigned int>(cout / 16),
+                  static_cast<unsigned int>(splits));
+  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, as_stream(stream),
+                     static_cast<const __bf16*>(x_cm), static_cast<const __bf16*>(dz_cm), part, n, d,
+                     h, w, cin, cout, per_split);
+  return launch_status();
+}
+
+int hdn_reduce_splits(const float* part, float* out, long long len, int splits, void* stream) {
+  if (!part || !out || len < 1 || splits < 1) return HDN_EINVAL;
+  const unsigned int blocks = static_cast<unsigned int>(std::min<long long>(blocks_for(len, 256), 65535));
+  hipLaunchKernelGGL(k_reduce_splits, dim3(blocks), dim3(256), 0, as_stream(stream), part, out, len,
+                     splits);
+  return launch_status();
+}
+
+int hdn_channel_sum(const void* cm, float* part, long long P, int C, int splits, void* stream) {
+  if (!cm || !part || P < 1 || C < 1 || splits < 1) return HDN_EINVAL;
+  const long long per_split = (P + splits - 1) / splits;
+  hipLaunchKernelGGL(k_channel_sum, dim3(static_cast<unsigned int>(C), static_cast<unsigned int>(splits)),
+                     dim3(256), 0, as_stream(stream), static_cast<const __bf16*>(cm), part, P,
+                     per_split);
+  return launch_status();
+}
+
+int hdn_act_bwd(const void* y, const void* dy, void* dz_nd, void* dz_cm, long long P, int C,
+                float slope, void* stream) {
+  if (!y || !dy || !dz_nd || P < 1 || C % 8 != 0) return HDN_EINVAL;
+  hipLaunchKernelGGL(k_act_bwd, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(y), static_cast<const __bf16*>(dy),
+                     static_cast<__bf16*>(dz_nd), static_cast<__bf16*>(dz_cm), P, C, slope);
+  return launch_status();
+}
+
+int hdn_add(const void* a, const void* b, void* y_nd, void* y_cm, long long P, int C, void* stream) {
+  if (!a || !b || !y_nd || P < 1 || C % 8 != 0) return HDN_EINVAL;
+  hipLaunchKernelGGL(k_add, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(a), static_cast<const __bf16*>(b),
+                     static_cast<__bf16*>(y_nd), static_cast<__bf16*>(y_cm), P, C);
+  return launch_status();
+}
+
+int hdn_pool_fwd(const void* x, void* y_nd, void* y_cm, int n, int d, int h, int w, int C,
+                 void* stream) {
+  if (!x || !y_nd || !shape_ok(n, d, h, w) || C % 8 != 0) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(x), static_cast<__bf16*>(y_nd),
+                     static_cast<__bf16*>(y_cm), n, d, h, w, C);
+  return launch_status();
+}
+
+int hdn_pool_bwd(const void* dy, void* dx, int n, int d2, int h2, int w2, int C, void* stream) {
+  if (!dy || !dx || !shape_ok(n, d2, h2, w2) || C % 8 != 0 || d2 % 2 || h2 % 2 || w2 % 2)
+    return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d2 * h2 * w2;
+  hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(dy), static_cast<__bf16*>(dx), n, d2, h2, w2, C);
+  return launch_status();
+}
+
+int hdn_up_fwd(const void* x, void* y_nd, void* y_cm, int n, int d2, int h2, int w2, int C,
+               void* stream) {
+  if (!x || !y_nd || !shape_ok(n, d2, h2, w2) || C % 8 != 0 || d2 % 2 || h2 % 2 || w2 % 2)
+    return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d2 * h2 * w2;
+  hipLaunchKernelGGL(k_up_fwd, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(x), static_cast<__bf16*>(y_nd),
+                     static_cast<__bf16*>(y_cm), n, d2, h2, w2, C);
+  return launch_status();
+}
+
+int hdn_up_bwd(const void* dy, void* dx, int n, int d, int h, int w, int C, void* stream) {
+  if (!dy || !dx || !shape_ok(n, d, h, w) || C % 8 != 0) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  hipLaunchKernelGGL(k_up_bwd, dim3(blocks_for(P * (C / 8), 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(dy), static_cast<__bf16*>(dx), n, d, h, w, C);
+  return launch_status();
+}
+
+int hdn_loss(const void* out, const void* inp, int c_out, int c_in, const float* target,
+             const float* weight, void* dout_nd, void* dout_cm, float* pred, float* partial,
+             long long P, float grad_scale, void* stream) {
+  if (!out || !inp || P < 1 || c_out < 1 || c_in < 1) return HDN_EINVAL;
+  if ((dout_nd || partial) && !target) return HDN_EINVAL;
+  hipLaunchKernelGGL(k_loss, dim3(blocks_for(P, 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(out), static_cast<const __bf16*>(inp), c_out, c_in,
+                     target, weight, static_cast<__bf16*>(dout_nd), static_cast<__bf16*>(dout_cm),
+                     pred, partial, P, grad_scale);
+  return launch_status();
+}
+
+int hdn_prepare_weights(const float* master, void* wf, void* wd, int cin, int cout, void* stream) {
+  if (!master || !wf || !wd || cin < 1 || cout < 1) return HDN_EINVAL;
+  hipLaunchKernelGGL(k_prepare_weights, dim3(blocks_for(27LL * cin * cout, 256)), dim3(256), 0,
+                     as_stream(stream), master, static_cast<__bf16*>(wf), static_cast<__bf16*>(wd),
+                     cin, cout);
+  return launch_status();
+}
+
+int hdn_adam(float* param, const float* grad, float* m, float* v, long long len, float lr,
+             float beta1, float beta2, float eps, int step, float weight_decay, void* stream) {
+  if (!param || !grad || !m || !v || len < 1 || step < 1) return HDN_EINVAL;
+  const float corr1 = 1.0f - std::pow(beta1, static_cast<float>(step));
+  const float corr2 = 1.0f - std::pow(beta2, static_cast<float>(step));
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(len, 256)), dim3(256), 0, as_stream(stream), param, grad,
+                     m, v, len, lr, beta1, beta2, eps, corr1, corr2, weight_decay);
+  return launch_status();
+}
+
+}  // extern "C"
