@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -91,6 +92,71 @@ def _dij_leg(median: float) -> dict:
     }
 
 
+def _window_errors(d, ref, source, grid) -> np.ndarray:
+    """Per-column dose-weighted RMS error of Dij d against Dij ref inside each
+    beamlet's beam-aligned window."""
+    from pyradmc_amd import denoise
+
+    spacing = float(grid.spacing[0])
+    out = np.zeros(source.n_beamlets)
+    for j in range(source.n_beamlets):
+        cx, cy, _ = source.centers[j]
+        wx, wy = denoise.beam_window((int(cx / spacing), int(cy / spacing)), tuple(grid.shape))
+        out[j] = denoise.dose_weighted_rms(d.column_dense(j)[wx, wy, :], ref.column_dense(j)[wx, wy, :])
+    return out
+
+
+def _denoise_leg(model_path: Path, low: int, target: float) -> dict:
+    """The 100-beamlet CT-like Dij at a dose-weighted RMS error target against a
+    100x-history Monte Carlo reference: plain Monte Carlo (histories per beamlet set
+    from a measured error at 1e5 and sigma ~ N^-1/2, then run and timed) versus
+    low-history Monte Carlo plus the trained denoiser with plain Monte Carlo fallback.
+    Held-out phantom (a seed never used in training)."""
+    from pyradmc_amd import HipEngine, denoise, workloads
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = workloads.ct_like_grid(seed=20261111)
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    src = workloads.ct_like_beamlets()
+    engine = HipEngine(grid, xs)
+    model = denoise.load(model_path)
+    nb = 10
+    engine.run_dij(src, 10 * nb, nb, seed=1, truncation=1e-4)  # warm
+    ref = engine.run_dij(src, 100 * low, nb, seed=7, truncation=1e-4)
+    probe_n = 100_000
+    probe = engine.run_dij(src, probe_n, nb, seed=8, truncation=1e-4)
+    eps = float(np.percentile(_window_errors(probe, ref, src, grid), 90))
+    n_plain = int(math.ceil(probe_n * (eps / target) ** 2 / nb)) * nb
+    started = time.perf_counter()
+    plain = engine.run_dij(src, n_plain, nb, seed=9, truncation=1e-4, value_dtype=np.float32)
+    wall_plain = time.perf_counter() - started
+    plain_err = _window_errors(plain, ref, src, grid)
+    builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, tolerance=target,
+                                         fallback_histories=n_plain, n_batches=nb)
+    builder.build(src, seed=11)  # warm (kernels, allocator)
+    started = time.perf_counter()
+    den, report = builder.build(src, seed=12, truncation=1e-4)
+    wall_den = time.perf_counter() - started
+    den_err = _window_errors(den, ref, src, grid)
+    engine.close()
+    return {
+        "workload": "CT-like 128x128x96 @ 2.5 mm (held-out seed), 100 beamlets 5x5 mm, SAD 100, "
+                    f"reference {100 * low} histories per beamlet",
+        "target_error": target,
+        "plain_histories_per_beamlet": n_plain,
+        "plain_wall_s": round(wall_plain, 4),
+        "plain_error_median": round(float(np.median(plain_err)), 5),
+        "plain_error_p90": round(float(np.percentile(plain_err, 90)), 5),
+        "denoised_low_histories_per_beamlet": 2 * low,
+        "denoised_wall_s": round(wall_den, 4),
+        "denoised_seconds": {k: round(v, 4) for k, v in report.seconds.items()},
+        "denoised_fallback_columns": int(report.fallback.sum()),
+        "denoised_error_median": round(float(np.median(den_err)), 5),
+        "denoised_error_p90": round(float(np.percentile(den_err, 90)), 5),
+        "speedup": round(wall_plain / wall_den, 3) if wall_den > 0 else None,
+    }
+
+
 def _cpu_baseline(sample_histories: int, threads: int) -> dict:
     from oracle import mc
     from pyradmc_amd import workloads
@@ -124,6 +190,9 @@ def main() -> int:
     ap.add_argument("--no-dij", action="store_true", help="skip the batched Dij leg")
     ap.add_argument("--dij-median", type=float, default=2e5,
                     help="median per-beamlet history count of the Dij leg")
+    ap.add_argument("--denoise-model", default=str(ROOT / "pyradmc_amd" / "models" / "denoiser_ct.npz"),
+                    help="trained denoiser weights for the denoised-Dij leg (skipped if absent)")
+    ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
                          "kernel that calibrates the HBM byte counters of a PMC profile)")
@@ -190,6 +259,9 @@ def main() -> int:
     }
 
     dij = None if args.no_dij else _dij_leg(args.dij_median)
+    denoised = None
+    if not args.no_denoise and Path(args.denoise_model).is_file():
+        denoised = _denoise_leg(Path(args.denoise_model), 10_000, 0.02)
 
     cpu = None
     if not args.no_cpu:
@@ -221,6 +293,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dij": dij,
+        "denoised_dij": denoised,
         "events_per_history": {
             k: round(counters[k] / total, 3)
             for k in ("photon_steps", "electron_steps", "photon_voxels", "electron_voxels",

@@ -49,11 +49,18 @@ __host__ __device__ inline uint32_t pack_meta(uint32_t kind, uint32_t primary, u
   return kind | (primary << 2) | (column << 3);
 }
 
+// A particle queue: records, the shared count (reserved slots), capacity, and - in a
+// round kernel - this wave's chunk state in LDS ({next, end} of the slots the wave has
+// reserved but not yet filled; see chunk_reserve).
 struct Queue {
   Particle* data;
   unsigned int* count;
   unsigned int capacity;
+  volatile unsigned int* chunk = nullptr;
 };
+
+// Kind tag of a padding record: a slot a producer wave reserved but did not fill.
+constexpr uint32_t kEmptyKind = 3u;
 
 // Slot reservation for the lanes of one wave that want one: a single atomic per
 // wave, ranks by lane order. Must be reached by every active lane of the wave.
@@ -137,6 +144,7 @@ struct TransportParams {
   long long* edep;
   unsigned long long* ledger;
   float pcut, ecut, step_fraction, resolution;
+  float log_pcut;  // logf(pcut), for the bremsstrahlung spectrum
   int transport_electrons, msc_gs, dose_to_water, max_steps;
 };
 
@@ -162,28 +170,76 @@ struct LaneTally {
     pend_cell = -1;
   }
 
-  // base = L_P_VOX for the photon round, L_E_VOX for the electron round. Every lane
-  // of the wave must reach this (it is called after the lane loop): the books are
-  // summed over the wave and written by lane 0.
-  __device__ void flush(unsigned long long* ledger, int base) const {
+  // base = L_P_VOX for the photon round, L_E_VOX for the electron round. Every thread
+  // of the block must call this after its lane loop (it synchronises the block): the
+  // books are summed over the wave, then over the block in LDS (red: 6 x 4 slots),
+  // and written with one atomic per book per block.
+  __device__ void flush(unsigned long long* ledger, int base, long long* red) const {
     const long long books[6] = {escaped, unscored, deposited, static_cast<long long>(vox),
                                 static_cast<long long>(dep), static_cast<long long>(steps)};
-    const int slot[6] = {L_ESCAPED, L_UNSCORED, L_DEPOSITED, base, base + 1, base + 2};
+    const int wave = static_cast<int>(threadIdx.x >> 6);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const long long total = wave_sum(books[k]);
-      if (__lane_id() == 0 && total != 0)
-        atomicAdd(ledger + slot[k], static_cast<unsigned long long>(total));
+      if (__lane_id() == 0) red[k * 4 + wave] = total;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int slot[6] = {L_ESCAPED, L_UNSCORED, L_DEPOSITED, base, base + 1, base + 2};
+      long long total = 0;
+      for (int wv = 0; wv < static_cast<int>(blockDim.x >> 6); ++wv) total += red[threadIdx.x * 4 + wv];
+      if (total != 0)
+        atomicAdd(ledger + slot[threadIdx.x], static_cast<unsigned long long>(total));
     }
   }
 };
 
-// Append one particle; the lanes of a wave that push at the same call site share one
-// atomic on the queue count (wave_reserve over the lanes executing the call).
+// Queue slots for the lanes of a wave that push at one call site. A wave reserves
+// kPushChunk slots at a time with one atomic on the shared count and hands them out in
+// lane order, finishing its current chunk before starting the next, so the only
+// unfilled slots are each wave's tail at the end of the round (seal_chunk pads them).
+// The chunk state lives in LDS, so the lanes executing a divergent call site always
+// see the wave's current state; only the first active lane reads and writes it.
+constexpr unsigned int kPushChunk = 64;
+
+__device__ inline unsigned int chunk_reserve(const Queue& q) {
+  const unsigned long long mask = __ballot(1);
+  const int lane = static_cast<int>(__lane_id());
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  const unsigned int cnt = static_cast<unsigned int>(__popcll(mask));
+  const unsigned int rank =
+      static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+  unsigned int next = 0u, avail = 0u, fresh = 0u;
+  if (lane == leader) {
+    next = q.chunk[0];
+    avail = q.chunk[1] - next;
+    if (cnt <= avail) {
+      q.chunk[0] = next + cnt;
+    } else {
+      fresh = atomicAdd(q.count, kPushChunk);  // cnt - avail <= 64 = kPushChunk
+      q.chunk[0] = fresh + (cnt - avail);
+      q.chunk[1] = fresh + kPushChunk;
+    }
+  }
+  next = __shfl(next, leader);
+  avail = __shfl(avail, leader);
+  fresh = __shfl(fresh, leader);
+  return rank < avail ? next + rank : fresh + (rank - avail);
+}
+
+// Pad the wave's unfilled reserved slots with empty records. Every lane of the wave
+// calls this after its round loop.
+__device__ inline void seal_chunk(const Queue& q) {
+  const unsigned int next = q.chunk[0], end = q.chunk[1];
+  for (unsigned int sl = next + __lane_id(); sl < end; sl += 64u)
+    if (sl < q.capacity) q.data[sl].meta = kEmptyKind;
+}
+
+// Append one particle (chunk_reserve over the lanes executing the call).
 __device__ inline void push(const Queue& q, uint32_t kind, uint32_t column, float e, float w,
                             float x, float y, float z, float ux, float uy, float uz,
                             uint64_t rng) {
-  const unsigned int idx = wave_reserve(q.count, true);
+  const unsigned int idx = chunk_reserve(q);
   if (idx < q.capacity) {
     Particle p;
     p.x = x;
@@ -315,7 +371,7 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 // Waves per SIMD the round kernels are register-budgeted for (a tuning constant:
 // more resident waves hide more of the dependent-load latency of a transport step).
 #ifndef HMC_ROUND_WAVES
-#define HMC_ROUND_WAVES 6
+#define HMC_ROUND_WAVES 5
 #endif
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
@@ -334,6 +390,16 @@ k_photon_round(TransportParams P, Queue qin,
   int steps = 0;
   const int n = P.t.n_points;
 
+  // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
+  // ledger reduction slots.
+  __shared__ unsigned int chunk_state[4][2][2];
+  __shared__ long long ledger_red[6 * 4];
+  {
+    const int wv = static_cast<int>(threadIdx.x >> 6);
+    if (__lane_id() < 4) chunk_state[wv][__lane_id() >> 1][__lane_id() & 1] = 0u;
+    qe.chunk = chunk_state[wv][0];
+    qp.chunk = chunk_state[wv][1];
+  }
   RefillPool pool;
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
@@ -341,6 +407,7 @@ k_photon_round(TransportParams P, Queue qin,
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      if ((meta & 3u) == kEmptyKind) continue;  // padding left by a producer wave
       column = meta >> 3;
       base = static_cast<long long>(column) * P.s.n_voxels;
       if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
@@ -472,8 +539,10 @@ k_photon_round(TransportParams P, Queue qin,
       active = false;
     }
   }
+  seal_chunk(qe);
+  seal_chunk(qp);
   tal.flush_pending(P.edep);
-  tal.flush(P.ledger, L_P_VOX);
+  tal.flush(P.ledger, L_P_VOX, ledger_red);
 }
 
 // --------------------------------------------------------------- electron round
@@ -496,6 +565,16 @@ k_electron_round(TransportParams P, Queue qin,
   int steps = 0;
   const int n = P.t.n_points;
 
+  // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
+  // ledger reduction slots.
+  __shared__ unsigned int chunk_state[4][2][2];
+  __shared__ long long ledger_red[6 * 4];
+  {
+    const int wv = static_cast<int>(threadIdx.x >> 6);
+    if (__lane_id() < 4) chunk_state[wv][__lane_id() >> 1][__lane_id() & 1] = 0u;
+    qp.chunk = chunk_state[wv][0];
+    qe.chunk = chunk_state[wv][1];
+  }
   RefillPool pool;
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
@@ -503,6 +582,7 @@ k_electron_round(TransportParams P, Queue qin,
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
+      if ((meta & 3u) == kEmptyKind) continue;  // padding left by a producer wave
       column = meta >> 3;
       positron = (meta & 3u) == POSITRON;
       base = static_cast<long long>(column) * P.s.n_voxels;
@@ -544,7 +624,8 @@ k_electron_round(TransportParams P, Queue qin,
 
     int ei;
     float ew;
-    grid_locate(__logf(e), P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
+    const float log_e = __logf(e);
+    grid_locate(log_e, P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
     const float4 ea = lerp4(P.t.elec_a + row, ei, ew);  // csda, moller, r_range, radiative
 
     const float range_cm = ea.x / rho;
@@ -587,7 +668,7 @@ k_electron_round(TransportParams P, Queue qin,
     float local_brems = radiative_loss;
     if (e > P.pcut) {
       const float emitted_fraction = 1.0f - P.pcut / e;
-      const float mean_k = (e - P.pcut) / __logf(e / P.pcut);
+      const float mean_k = (e - P.pcut) / (log_e - P.log_pcut);
       emit_probability = radiative_loss * emitted_fraction / mean_k;
       local_brems = radiative_loss * (1.0f - emitted_fraction);
     }
@@ -611,7 +692,8 @@ k_electron_round(TransportParams P, Queue qin,
       continue;
     }
 
-    grid_locate(__logf(e), P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
+    const float log_e_mid = __logf(e);
+    grid_locate(log_e_mid, P.t.e_log_min, P.t.e_inv_dlog, n, ei, ew);
     const float4 eb = lerp4(P.t.elec_b + row, ei, ew);  // scattering power, log eta
     const float mean_square = eb.x * rho * s;
     float cos_hinge;
@@ -626,7 +708,7 @@ k_electron_round(TransportParams P, Queue qin,
     rotate(ux, uy, uz, cos_hinge, phi);
 
     if (emit_brems && e > P.pcut) {
-      float k = P.pcut * __expf(uniform(rng) * __logf(e / P.pcut));
+      float k = P.pcut * __expf(uniform(rng) * (log_e_mid - P.log_pcut));
       k = fminf(k, e - d2);
       if (k > P.pcut) {
         float photon_w = w;
@@ -669,8 +751,10 @@ k_electron_round(TransportParams P, Queue qin,
       e -= delta;
     }
   }
+  seal_chunk(qp);
+  seal_chunk(qe);
   tal.flush_pending(P.edep);
-  tal.flush(P.ledger, L_E_VOX);
+  tal.flush(P.ledger, L_E_VOX, ledger_red);
 }
 
 // -------------------------------------------------------------------- generate
@@ -715,116 +799,159 @@ __device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
 }
 
 
-// Primaries [p0, p0 + n_gen) of the flattened work items. Single-kind sources write
-// fixed slots of the target queue (thread t -> slot t; the count is set once);
-// host-sampled mixed-kind primaries reserve slots per wave in the queue of their
-// kind (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
+// Primaries [p0, p0 + n_gen) of the flattened work items, grid-stride (at most a few
+// thousand blocks). Single-kind sources write fixed slots of the target queue
+// (primary t -> slot t; the count is set once); host-sampled mixed-kind primaries
+// reserve slots in the queue of their kind with one atomic per queue per block and
+// pass. The emitted energy is reduced over the block and added with one atomic per
+// block (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
 __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* items, int n_items,
                                                   long long p0, long long n_gen, uint64_t seed,
                                                   Queue qp, Queue qe,
                                                   unsigned long long* ledger) {
-  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool live = t < n_gen;
-  long long emitted = 0;
-  Particle p;
-  uint32_t kind = static_cast<uint32_t>(S.particle);
-  if (live) {
-    const long long g = p0 + t;
-    int lo = 0, hi = n_items;  // last item with start <= g
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (items[mid].start <= g) lo = mid;
-      else hi = mid;
-    }
-    const DevItem item = items[lo];
-    const long long i = g - item.start;
-    const uint32_t column = static_cast<uint32_t>(item.column);
-    uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
-    float e = S.energy, w = 1.0f;
-    float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
-    bool book = false;
-    switch (S.kind) {
-      case HMC_SRC_PENCIL:
-        x = S.px; y = S.py; z = S.pz;
-        ux = S.dx; uy = S.dy; uz = S.dz;
-        break;
-      case HMC_SRC_PARALLEL:
-        x = S.x0 + S.xe * uniform(rng);
-        y = S.y0 + S.ye * uniform(rng);
-        z = S.z0;
-        break;
-      case HMC_SRC_PARALLEL_SPECTRAL:
-        e = sample_spectrum(S, rng);
-        x = S.x0 + S.xe * uniform(rng);
-        y = S.y0 + S.ye * uniform(rng);
-        z = S.z0;
-        kind = PHOTON;
-        book = true;
-        break;
-      case HMC_SRC_LATTICE: {
-        const float* b = S.beamlet + 4 * static_cast<long long>(item.beamlet);
-        x = b[0] + b[1] * uniform(rng);
-        y = b[2] + b[3] * uniform(rng);
-        z = S.z0;
-        kind = PHOTON;
-        break;
+  __shared__ unsigned int wave_count[2][4];
+  __shared__ unsigned int block_base[2];
+  __shared__ long long red[4];
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  const int lane = static_cast<int>(__lane_id());
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  const long long passes = (n_gen + stride - 1) / stride;
+  long long emitted_sum = 0;
+  for (long long pass = 0; pass < passes; ++pass) {
+    const long long t =
+        pass * stride + static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool live = t < n_gen;
+    long long emitted = 0;
+    Particle p;
+    uint32_t kind = static_cast<uint32_t>(S.particle);
+    if (live) {
+      const long long g = p0 + t;
+      int lo = 0, hi = n_items;  // last item with start <= g
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (items[mid].start <= g) lo = mid;
+        else hi = mid;
       }
-      case HMC_SRC_SPECTRAL_FAN: {
-        e = sample_spectrum(S, rng);
-        const float du = (uniform(rng) - 0.5f) * S.width_u;
-        const float dv = (uniform(rng) - 0.5f) * S.width_v;
-        const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
-        const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
-        const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
-        const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
-        const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-        x = S.fx; y = S.fy; z = S.fz;
-        ux = dx * inv; uy = dy * inv; uz = dz * inv;
-        kind = PHOTON;
-        book = true;
-        break;
+      const DevItem item = items[lo];
+      const long long i = g - item.start;
+      const uint32_t column = static_cast<uint32_t>(item.column);
+      uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
+      float e = S.energy, w = 1.0f;
+      float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
+      bool book = false;
+      switch (S.kind) {
+        case HMC_SRC_PENCIL:
+          x = S.px; y = S.py; z = S.pz;
+          ux = S.dx; uy = S.dy; uz = S.dz;
+          break;
+        case HMC_SRC_PARALLEL:
+          x = S.x0 + S.xe * uniform(rng);
+          y = S.y0 + S.ye * uniform(rng);
+          z = S.z0;
+          break;
+        case HMC_SRC_PARALLEL_SPECTRAL:
+          e = sample_spectrum(S, rng);
+          x = S.x0 + S.xe * uniform(rng);
+          y = S.y0 + S.ye * uniform(rng);
+          z = S.z0;
+          kind = PHOTON;
+          book = true;
+          break;
+        case HMC_SRC_LATTICE: {
+          const float* b = S.beamlet + 4 * static_cast<long long>(item.beamlet);
+          x = b[0] + b[1] * uniform(rng);
+          y = b[2] + b[3] * uniform(rng);
+          z = S.z0;
+          kind = PHOTON;
+          break;
+        }
+        case HMC_SRC_SPECTRAL_FAN: {
+          e = sample_spectrum(S, rng);
+          const float du = (uniform(rng) - 0.5f) * S.width_u;
+          const float dv = (uniform(rng) - 0.5f) * S.width_v;
+          const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
+          const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
+          const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
+          const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
+          const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+          x = S.fx; y = S.fy; z = S.fz;
+          ux = dx * inv; uy = dy * inv; uz = dz * inv;
+          kind = PHOTON;
+          book = true;
+          break;
+        }
+        default: {  // host-sampled primaries, indexed by the flat primary index
+          const long long j = g;
+          kind = static_cast<uint32_t>(S.up_particle[j]);
+          e = S.up[j];
+          x = S.up[S.up_n + j];
+          y = S.up[2 * S.up_n + j];
+          z = S.up[3 * S.up_n + j];
+          ux = S.up[4 * S.up_n + j];
+          uy = S.up[5 * S.up_n + j];
+          uz = S.up[6 * S.up_n + j];
+          w = S.up[7 * S.up_n + j];
+          book = true;
+          break;
+        }
       }
-      default: {  // host-sampled primaries, indexed by the flat primary index
-        const long long j = g;
-        kind = static_cast<uint32_t>(S.up_particle[j]);
-        e = S.up[j];
-        x = S.up[S.up_n + j];
-        y = S.up[2 * S.up_n + j];
-        z = S.up[3 * S.up_n + j];
-        ux = S.up[4 * S.up_n + j];
-        uy = S.up[5 * S.up_n + j];
-        uz = S.up[6 * S.up_n + j];
-        w = S.up[7 * S.up_n + j];
-        book = true;
-        break;
+      if (book) {
+        const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
+        emitted = quanta_round(static_cast<double>(w * (e + latent)));
       }
+      p.x = x; p.y = y; p.z = z; p.e = e;
+      p.ux = ux; p.uy = uy; p.uz = uz; p.w = w;
+      p.rng = rng;
+      p.meta = pack_meta(kind, 1u, column);
+      p.pad = 0u;
     }
-    if (book) {
-      const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
-      emitted = quanta_round(static_cast<double>(w * (e + latent)));
+    if (S.kind != HMC_SRC_UPLOADED) {
+      // One kind per launch: fixed slots, no atomics.
+      const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
+      if (live && t < q.capacity) q.data[t] = p;
+    } else {
+      // Block-level slot reservation: wave counts in LDS, one atomic per queue.
+      const bool to_p = live && kind == PHOTON;
+      const bool to_e = live && kind != PHOTON;
+      const unsigned long long mp = __ballot(to_p ? 1 : 0);
+      const unsigned long long me = __ballot(to_e ? 1 : 0);
+      if (lane == 0) {
+        wave_count[0][wave] = static_cast<unsigned int>(__popcll(mp));
+        wave_count[1][wave] = static_cast<unsigned int>(__popcll(me));
+      }
+      __syncthreads();
+      if (threadIdx.x < 2) {
+        unsigned int total = 0u;
+        for (int wv = 0; wv < static_cast<int>(blockDim.x >> 6); ++wv) total += wave_count[threadIdx.x][wv];
+        block_base[threadIdx.x] = total ? atomicAdd(threadIdx.x == 0 ? qp.count : qe.count, total) : 0u;
+      }
+      __syncthreads();
+      unsigned int off_p = block_base[0], off_e = block_base[1];
+      for (int wv = 0; wv < wave; ++wv) {
+        off_p += wave_count[0][wv];
+        off_e += wave_count[1][wv];
+      }
+      const unsigned long long below = (1ull << lane) - 1ull;
+      const unsigned int sp = off_p + static_cast<unsigned int>(__popcll(mp & below));
+      const unsigned int se = off_e + static_cast<unsigned int>(__popcll(me & below));
+      if (to_p && sp < qp.capacity) qp.data[sp] = p;
+      if (to_e && se < qe.capacity) qe.data[se] = p;
+      __syncthreads();  // wave_count and block_base are reused by the next pass
     }
-    p.x = x; p.y = y; p.z = z; p.e = e;
-    p.ux = ux; p.uy = uy; p.uz = uz; p.w = w;
-    p.rng = rng;
-    p.meta = pack_meta(kind, 1u, column);
-    p.pad = 0u;
+    emitted_sum += emitted;
   }
-  if (S.kind != HMC_SRC_UPLOADED) {
-    // One kind per launch: fixed slots, no atomics.
+  if (S.kind != HMC_SRC_UPLOADED && blockIdx.x == 0 && threadIdx.x == 0) {
     const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
-    if (live && t < q.capacity) q.data[t] = p;
-    if (t == 0) *q.count = static_cast<unsigned int>(n_gen);
-  } else {
-    const bool to_p = live && kind == PHOTON;
-    const bool to_e = live && kind != PHOTON;
-    const unsigned int sp = wave_reserve(qp.count, to_p);
-    const unsigned int se = wave_reserve(qe.count, to_e);
-    if (to_p && sp < qp.capacity) qp.data[sp] = p;
-    if (to_e && se < qe.capacity) qe.data[se] = p;
+    *q.count = static_cast<unsigned int>(n_gen);
   }
-  const long long total = wave_sum(emitted);
-  if (__lane_id() == 0 && total != 0)
-    atomicAdd(ledger + L_EMITTED, static_cast<unsigned long long>(total));
+  const long long wave_total = wave_sum(emitted_sum);
+  if (lane == 0) red[wave] = wave_total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long total = 0;
+    for (int wv = 0; wv < static_cast<int>(blockDim.x >> 6); ++wv) total += red[wv];
+    if (total != 0) atomicAdd(ledger + L_EMITTED, static_cast<unsigned long long>(total));
+  }
 }
 
 __global__ void k_fill_u32(unsigned int* a, int n, unsigned int v) {
@@ -1545,6 +1672,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   P.edep = ctx->d_edep;
   P.ledger = ctx->d_ledger;
   P.pcut = static_cast<float>(ctx->phys.pcut);
+  P.log_pcut = std::log(P.pcut);
   P.ecut = static_cast<float>(ctx->phys.ecut);
   P.step_fraction = static_cast<float>(ctx->phys.step_energy_fraction);
   P.resolution = static_cast<float>(ctx->phys.deposit_resolution_cm);
@@ -1579,7 +1707,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, 8, 0u);
     int pa = 0, ea = 1, pb = 2, eb = 3;  // queue roles
     Queue qpa{ctx->d_q[pa], counts + pa, cap}, qea{ctx->d_q[ea], counts + ea, cap};
-    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 1ll << 31)), dim3(256), 0,
+    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
                        ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
                        static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());

@@ -370,9 +370,19 @@ class Conv3dDenoiser:
 # ------------------------------------------------------------------------ data
 
 
+def to_network_axes(a: np.ndarray) -> np.ndarray:
+    """[N][x][y][z] windows -> [N][z][x][y]: the network's innermost axis is a 64-wide
+    lateral one (the convolutions are cubic, so this only relabels axes)."""
+    return np.ascontiguousarray(np.transpose(a, (0, 3, 1, 2)))
+
+
+def from_network_axes(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.transpose(a, (0, 2, 3, 1)))
+
+
 def pack_input(noisy: np.ndarray, density: np.ndarray, device: str = "cuda:0"):
-    """[N][D][H][W] scaled noisy dose and density -> padded bf16 nd tensor and its
-    channel-major copy."""
+    """[N][D][H][W] scaled noisy dose and density (network axes) -> padded bf16 nd
+    tensor and its channel-major copy."""
     torch = _torch()
     x = np.zeros(noisy.shape + (CIN_PAD,), dtype=np.float32)
     x[..., 0] = noisy
@@ -495,9 +505,10 @@ def train(model: Conv3dDenoiser, pairs: list[ColumnPair], epochs: int = 10, batc
                 ds.append(dd)
                 ts.append(t)
                 ws.append(1.0 / (s * s + SIGMA_FLOOR * SIGMA_FLOOR))
-            x, x_cm = pack_input(np.stack(xs), np.stack(ds) * 0.5, model.device)
-            target = torch.from_numpy(np.ascontiguousarray(np.stack(ts), dtype=np.float32)).to(model.device)
-            weight = np.stack(ws).astype(np.float32)
+            x, x_cm = pack_input(to_network_axes(np.stack(xs)), to_network_axes(np.stack(ds)) * 0.5,
+                                 model.device)
+            target = torch.from_numpy(to_network_axes(np.stack(ts).astype(np.float32))).to(model.device)
+            weight = to_network_axes(np.stack(ws).astype(np.float32))
             weight /= weight.mean()
             weight = torch.from_numpy(np.ascontiguousarray(weight)).to(model.device)
             out, cache = model.forward(x, x_cm, keep=True)
@@ -518,8 +529,9 @@ def predict(model: Conv3dDenoiser, noisy: np.ndarray, density: np.ndarray) -> np
     single = noisy.ndim == 3
     a = noisy[None] if single else noisy
     dd = density[None] if single else density
-    x, _ = pack_input(a.astype(np.float32), dd.astype(np.float32) * 0.5, model.device)
-    pred = model.predict_tensor(x).cpu().numpy()
+    x, _ = pack_input(to_network_axes(a.astype(np.float32)), to_network_axes(dd.astype(np.float32)) * 0.5,
+                      model.device)
+    pred = from_network_axes(model.predict_tensor(x).cpu().numpy())
     return pred[0] if single else pred
 
 
