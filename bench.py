@@ -214,6 +214,12 @@ def main() -> int:
         torch.cuda.synchronize()
         del a, b
 
+    denoise_leg = not args.no_denoise and Path(args.denoise_model).is_file()
+    if denoise_leg:
+        # The denoiser runs on torch's HIP runtime: load it before any engine exists.
+        print("bench: loading the denoiser (PyTorch)", file=sys.stderr, flush=True)
+        import pyradmc_amd.denoise  # noqa: F401
+
     from pyradmc_amd import HipEngine, workloads
 
     grid, xs, src, kw = workloads.slab_phantom(n_histories=args.histories, n_batches=args.batches)
@@ -259,9 +265,7 @@ def main() -> int:
     }
 
     dij = None if args.no_dij else _dij_leg(args.dij_median)
-    denoised = None
-    if not args.no_denoise and Path(args.denoise_model).is_file():
-        denoised = _denoise_leg(Path(args.denoise_model), 10_000, 0.02)
+    denoised = _denoise_leg(Path(args.denoise_model), 10_000, 0.02) if denoise_leg else None
 
     cpu = None
     if not args.no_cpu:

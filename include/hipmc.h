@@ -150,6 +150,20 @@ int hmc_finalize(hmc_ctx *ctx, int n_batches, double *mean, double *sigma);
  * host prefix (engine.py:1496-1557). counts[n_columns] out; entries fetched next. */
 int hmc_truncate(hmc_ctx *ctx, double truncation, int64_t *counts, int64_t *total);
 int hmc_fetch_csc(hmc_ctx *ctx, int64_t *indices, double *dose, double *sigma);
+
+/* Beam-aligned windows of the finalized columns, for the learned denoiser (no
+ * counterpart in the reference; the denoised Dij builder in pyradmc_amd/denoise.py).
+ * origins: host int32 [n_cols][2] (x0, y0), n_cols equal to the scoring column count;
+ * a window is wx x wy scoring voxels laterally and the full depth.
+ * hmc_gather_windows writes float32 out[c][z][x][y] (device pointer) from the means
+ * kept on the device by hmc_finalize(ctx, n, NULL, NULL).
+ * hmc_scatter_windows replaces every column's mean and sigma by the window values of
+ * mean / sigma (device float32 [c][z][x][y]; sigma may be NULL for zero) and zero
+ * elsewhere, ready for hmc_truncate. Both return after the device work is done. */
+int hmc_gather_windows(hmc_ctx *ctx, const int32_t *origins, int64_t n_cols, int wx, int wy,
+                       float *out);
+int hmc_scatter_windows(hmc_ctx *ctx, const int32_t *origins, int64_t n_cols, int wx, int wy,
+                        const float *mean, const float *sigma);
 /* Clear the dense maps and sums for a new group. */
 int hmc_reset_maps(hmc_ctx *ctx);
 /* Replace / read the dense int64 quanta maps [n_slots][n_columns][n_voxels]: the synthetic

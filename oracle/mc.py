@@ -8,7 +8,8 @@ batch, and scores exactly like the reference's ``BatchedDoseScorer`` /
 
 Also restates the device fold arithmetic on fixed-point quanta
 (``pyradmc/backends/warp/kernels.py:771-909``) and the truncation rule, used to check
-the HIP fold/finalize/truncation bit for bit on synthetic quanta maps.
+the HIP fold/finalize/truncation bit for bit on synthetic quanta maps, and the
+planning product ``dose_for_weights`` (``pyradmc/scoring/dij.py:251-260``).
 """
 
 from __future__ import annotations
@@ -335,3 +336,17 @@ def for_grid(grid, source, **kw) -> Oracle:
                   grid.max_density_by_material(), source_dict(source), **kw)
 
 
+def dose_for_weights(indptr: np.ndarray, indices: np.ndarray, dose: np.ndarray, weights: np.ndarray,
+                     n_voxels: int) -> np.ndarray:
+    """DijResult.dose_for_weights (pyradmc/scoring/dij.py:251-260) for one weight
+    vector [n_beamlets] or a matrix [n_beamlets][n_w]: float64 products of the float32
+    entries and the weights, added column by column in ascending order (the entries of
+    one column in CSC order). Returns [n_voxels] or [n_voxels][n_w] float64."""
+    w = np.asarray(weights, dtype=np.float64)
+    single = w.ndim == 1
+    w2 = w[:, None] if single else w
+    out = np.zeros((n_voxels, w2.shape[1]), dtype=np.float64)
+    for j in range(w2.shape[0]):
+        lo, hi = int(indptr[j]), int(indptr[j + 1])
+        np.add.at(out, indices[lo:hi], np.asarray(dose[lo:hi])[:, None] * w2[j][None, :])
+    return out[:, 0] if single else out

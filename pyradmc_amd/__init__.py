@@ -39,6 +39,10 @@ def __getattr__(name: str):
         from pyradmc_amd.engine import HipEngine
 
         return HipEngine
+    if name == "DeviceDij":
+        from pyradmc_amd.spmm import DeviceDij
+
+        return DeviceDij
     raise AttributeError(name)
 
 
@@ -46,6 +50,7 @@ __all__ = [
     "AnalyticCrossSections",
     "BeamletGridSource",
     "DIJ_TRUNCATION_RELATIVE",
+    "DeviceDij",
     "DijResult",
     "ECUT_MEV",
     "ELECTRON_MASS_MEV",

@@ -158,6 +158,8 @@ def lib():
         "hmc_set_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_get_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_build_gs_grid": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, _f64p]),
+        "hmc_gather_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p]),
+        "hmc_scatter_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p, c_void_p]),
     }
     for name, (restype, argtypes) in sig.items():
         fn = getattr(h, name)
@@ -168,12 +170,12 @@ def lib():
 
 
 def exported_symbols() -> list[str]:
-    """Every ``hmc_*`` / ``hdn_*`` entry point the headers declare."""
+    """Every ``hmc_*`` / ``hdn_*`` / ``hsp_*`` entry point the headers declare."""
     import re
 
     include = Path(__file__).resolve().parent.parent / "include"
     names = set()
-    for header, prefix in (("hipmc.h", "hmc"), ("hipdn.h", "hdn")):
+    for header, prefix in (("hipmc.h", "hmc"), ("hipdn.h", "hdn"), ("hipsp.h", "hsp")):
         names |= set(re.findall(rf"\b({prefix}_[a-z0-9_]+)\s*\(", (include / header).read_text()))
     return sorted(names)
 

@@ -1,9 +1,10 @@
 """Compile the native library ``libhipmc.so`` in-tree for gfx950.
 
-``hipmc_kernels.hip`` (transport kernels and their C ABI, ``include/hipmc.h``) and
+``hipmc_kernels.hip`` (transport kernels and their C ABI, ``include/hipmc.h``),
 ``denoise_kernels.hip`` (the denoiser's MFMA convolution kernels, ``include/hipdn.h``)
-are compiled by hipcc for ``--offload-arch=gfx950``; ``gs_build.cpp`` (host-only GS
-grid builder) by g++; all link into one shared library beside this file, which
+and ``spmm_kernels.hip`` (dose for many weight vectors, ``include/hipsp.h``) are
+compiled by hipcc for ``--offload-arch=gfx950``; ``gs_build.cpp`` (host-only GS grid
+builder) by g++; all link into one shared library beside this file, which
 travels with the source tree. Packed-FP32 instruction selection is disabled for the
 device code.
 """
@@ -60,6 +61,7 @@ def build(verbose: bool = False) -> Path:
     build_dir.mkdir(exist_ok=True)
     kernels_o = build_dir / "hipmc_kernels.o"
     denoise_o = build_dir / "denoise_kernels.o"
+    spmm_o = build_dir / "spmm_kernels.o"
     gs_o = build_dir / "gs_build.o"
     _run(
         [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
@@ -72,6 +74,11 @@ def build(verbose: bool = False) -> Path:
         verbose,
     )
     _run(
+        [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
+         str(CSRC / "spmm_kernels.hip"), "-o", str(spmm_o)],
+        verbose,
+    )
+    _run(
         ["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-I", str(ROOT / "include"), "-c",
          str(CSRC / "gs_build.cpp"), "-o", str(gs_o)],
         verbose,
@@ -79,7 +86,7 @@ def build(verbose: bool = False) -> Path:
     tmp = HERE / (LIB_NAME + ".tmp")
     _run(
         [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", str(kernels_o),
-         str(denoise_o), str(gs_o), "-o", str(tmp)],
+         str(denoise_o), str(spmm_o), str(gs_o), "-o", str(tmp)],
         verbose,
     )
     os.replace(tmp, LIB_PATH)

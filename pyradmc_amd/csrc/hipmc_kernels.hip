@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hipmc.h"
@@ -57,6 +58,7 @@ struct Queue {
   unsigned int* count;
   unsigned int capacity;
   volatile unsigned int* chunk = nullptr;
+  volatile long long* pushed = nullptr;  // this wave's push count (LDS), see chunk_reserve
 };
 
 // Kind tag of a padding record: a slot a producer wave reserved but did not fill.
@@ -93,7 +95,8 @@ struct RefillPool {
   unsigned int next = 0, end = 0;
 };
 
-__device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, bool idle) {
+__device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, unsigned int n_in,
+                                            bool idle) {
   const unsigned long long mask = __ballot(idle ? 1 : 0);
   if (mask == 0ull) return 0u;
   const unsigned int want = static_cast<unsigned int>(__popcll(mask));
@@ -109,7 +112,12 @@ __device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head
     const unsigned long long present = __ballot(1);
     const int leader = __ffsll(static_cast<long long>(present)) - 1;
     unsigned int base = 0u;
-    if (lane == leader) base = atomicAdd(head, kRefillBlock);
+    if (lane == leader) {
+      // An exhausted queue is read, not claimed: a round launched wider than its
+      // queue costs no atomics in the waves that find nothing left.
+      base = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (base < n_in) base = atomicAdd(head, kRefillBlock);
+    }
     base = __shfl(base, leader);
     idx = rank < avail ? pool.next + rank : base + (rank - avail);
     pool.next = base + (want - avail);
@@ -125,15 +133,19 @@ enum LedgerSlot {
   L_EMITTED,
   L_VIOLATIONS,
   L_TRAPPED,
+  L_OVERFLOW,  // largest queue count seen above capacity (atomicMax), 0 if none
   // per-kernel event counts, photon round then electron round: voxel fetches,
-  // tally atomics, steps (queue pushes and pops are counted by the host from the
-  // queue counts it reads between rounds)
+  // tally atomics, steps, particles popped (padding excluded), particles pushed
   L_P_VOX,
   L_P_DEP,
   L_P_STEPS,
+  L_P_POP,
+  L_P_PUSH,
   L_E_VOX,
   L_E_DEP,
   L_E_STEPS,
+  L_E_POP,
+  L_E_PUSH,
   L_COUNT
 };
 
@@ -172,7 +184,8 @@ struct LaneTally {
 
   // base = L_P_VOX for the photon round, L_E_VOX for the electron round. Every thread
   // of the block must call this after its lane loop (it synchronises the block): the
-  // books are summed over the wave, then over the block in LDS (red: 6 x 4 slots),
+  // books are summed over the wave, then over the block in LDS (red: 8 x 4 slots; the
+  // wave's pop and push counts are already in red[6*4 + wave] and red[7*4 + wave]),
   // and written with one atomic per book per block.
   __device__ void flush(unsigned long long* ledger, int base, long long* red) const {
     const long long books[6] = {escaped, unscored, deposited, static_cast<long long>(vox),
@@ -184,8 +197,9 @@ struct LaneTally {
       if (__lane_id() == 0) red[k * 4 + wave] = total;
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
-      const int slot[6] = {L_ESCAPED, L_UNSCORED, L_DEPOSITED, base, base + 1, base + 2};
+    if (threadIdx.x < 8) {
+      const int slot[8] = {L_ESCAPED, L_UNSCORED, L_DEPOSITED, base,
+                           base + 1,  base + 2,   base + 3,    base + 4};
       long long total = 0;
       for (int wv = 0; wv < static_cast<int>(blockDim.x >> 6); ++wv) total += red[threadIdx.x * 4 + wv];
       if (total != 0)
@@ -211,6 +225,7 @@ __device__ inline unsigned int chunk_reserve(const Queue& q) {
       static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
   unsigned int next = 0u, avail = 0u, fresh = 0u;
   if (lane == leader) {
+    q.pushed[0] += cnt;
     next = q.chunk[0];
     avail = q.chunk[1] - next;
     if (cnt <= avail) {
@@ -233,6 +248,15 @@ __device__ inline void seal_chunk(const Queue& q) {
   const unsigned int next = q.chunk[0], end = q.chunk[1];
   for (unsigned int sl = next + __lane_id(); sl < end; sl += 64u)
     if (sl < q.capacity) q.data[sl].meta = kEmptyKind;
+}
+
+// Add the lanes (of those executing the call) that popped a real particle to the
+// wave's pop count in LDS.
+__device__ inline void count_pops(volatile long long* wave_pops, bool real) {
+  const unsigned long long mask = __ballot(real ? 1 : 0);
+  const unsigned long long active = __ballot(1);
+  if (static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(active)) - 1)
+    wave_pops[0] += __popcll(mask);
 }
 
 // Append one particle (chunk_reserve over the lanes executing the call).
@@ -393,21 +417,27 @@ k_photon_round(TransportParams P, Queue qin,
   // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
   // ledger reduction slots.
   __shared__ unsigned int chunk_state[4][2][2];
-  __shared__ long long ledger_red[6 * 4];
+  __shared__ long long ledger_red[8 * 4];  // [book][wave]; books 6, 7 = pops, pushes
+  const int wave_in_block = static_cast<int>(threadIdx.x >> 6);
   {
-    const int wv = static_cast<int>(threadIdx.x >> 6);
-    if (__lane_id() < 4) chunk_state[wv][__lane_id() >> 1][__lane_id() & 1] = 0u;
-    qe.chunk = chunk_state[wv][0];
-    qp.chunk = chunk_state[wv][1];
+    if (__lane_id() < 4) chunk_state[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
+    if (__lane_id() < 2) ledger_red[(6 + __lane_id()) * 4 + wave_in_block] = 0;
+    qe.chunk = chunk_state[wave_in_block][0];
+    qp.chunk = chunk_state[wave_in_block][1];
+    qe.pushed = &ledger_red[7 * 4 + wave_in_block];
+    qp.pushed = &ledger_red[7 * 4 + wave_in_block];
   }
+  volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
   RefillPool pool;
   while (true) {
-    const unsigned int i = refill_index(pool, head, !active);
+    const unsigned int i = refill_index(pool, head, n_in, !active);
     if (!active) {
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
-      if ((meta & 3u) == kEmptyKind) continue;  // padding left by a producer wave
+      const bool real = (meta & 3u) != kEmptyKind;  // else padding left by a producer wave
+      count_pops(wave_pops, real);
+      if (!real) continue;
       column = meta >> 3;
       base = static_cast<long long>(column) * P.s.n_voxels;
       if (!enter_grid(P.g, x, y, z, ux, uy, uz)) {
@@ -568,21 +598,27 @@ k_electron_round(TransportParams P, Queue qin,
   // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
   // ledger reduction slots.
   __shared__ unsigned int chunk_state[4][2][2];
-  __shared__ long long ledger_red[6 * 4];
+  __shared__ long long ledger_red[8 * 4];  // [book][wave]; books 6, 7 = pops, pushes
+  const int wave_in_block = static_cast<int>(threadIdx.x >> 6);
   {
-    const int wv = static_cast<int>(threadIdx.x >> 6);
-    if (__lane_id() < 4) chunk_state[wv][__lane_id() >> 1][__lane_id() & 1] = 0u;
-    qp.chunk = chunk_state[wv][0];
-    qe.chunk = chunk_state[wv][1];
+    if (__lane_id() < 4) chunk_state[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
+    if (__lane_id() < 2) ledger_red[(6 + __lane_id()) * 4 + wave_in_block] = 0;
+    qp.chunk = chunk_state[wave_in_block][0];
+    qe.chunk = chunk_state[wave_in_block][1];
+    qp.pushed = &ledger_red[7 * 4 + wave_in_block];
+    qe.pushed = &ledger_red[7 * 4 + wave_in_block];
   }
+  volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
   RefillPool pool;
   while (true) {
-    const unsigned int i = refill_index(pool, head, !active);
+    const unsigned int i = refill_index(pool, head, n_in, !active);
     if (!active) {
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
-      if ((meta & 3u) == kEmptyKind) continue;  // padding left by a producer wave
+      const bool real = (meta & 3u) != kEmptyKind;  // else padding left by a producer wave
+      count_pops(wave_pops, real);
+      if (!real) continue;
       column = meta >> 3;
       positron = (meta & 3u) == POSITRON;
       base = static_cast<long long>(column) * P.s.n_voxels;
@@ -959,6 +995,63 @@ __global__ void k_fill_u32(unsigned int* a, int n, unsigned int v) {
   if (i < n) a[i] = v;
 }
 
+// Reset the two queues a round consumed (counts and heads), recording a count above
+// capacity (a dropped push) in the ledger's overflow slot first.
+__global__ void k_reset_consumed(unsigned int* counts, unsigned int* heads, int qa, int qb,
+                                 unsigned int capacity, unsigned long long* ledger) {
+  if (threadIdx.x == 0) {
+    const unsigned int ca = counts[qa], cb = counts[qb];
+    const unsigned int worst = ca > cb ? ca : cb;
+    if (worst > capacity) atomicMax(ledger + L_OVERFLOW, static_cast<unsigned long long>(worst));
+    counts[qa] = 0u;
+    counts[qb] = 0u;
+    heads[qa] = 0u;
+    heads[qb] = 0u;
+  }
+}
+
+// Beam-aligned windows of finalized columns, for the denoiser: out[c][z][x][y] =
+// mean[c][((x0_c + x) * ny + (y0_c + y)) * nz + z] as float (x < wx, y < wy, all z).
+__global__ void k_gather_windows(const double* __restrict__ mean, const int* __restrict__ origins,
+                                 long long n_cols, int wx, int wy, int ny, int nz, long long nv,
+                                 float* __restrict__ out) {
+  const long long total = n_cols * nz * wx * wy;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    long long t = i;
+    const int y = static_cast<int>(t % wy);
+    t /= wy;
+    const int x = static_cast<int>(t % wx);
+    t /= wx;
+    const int z = static_cast<int>(t % nz);
+    const long long c = t / nz;
+    const long long v = (static_cast<long long>(origins[2 * c] + x) * ny + origins[2 * c + 1] + y) * nz + z;
+    out[i] = static_cast<float>(mean[c * nv + v]);
+  }
+}
+
+// The inverse for denoised results: mean and sigma of each column become the window
+// values (zero outside the window; the caller cleared them first).
+__global__ void k_scatter_windows(double* __restrict__ mean, double* __restrict__ sigma,
+                                  const int* __restrict__ origins, long long n_cols, int wx, int wy,
+                                  int ny, int nz, long long nv, const float* __restrict__ in_mean,
+                                  const float* __restrict__ in_sigma) {
+  const long long total = n_cols * nz * wx * wy;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    long long t = i;
+    const int y = static_cast<int>(t % wy);
+    t /= wy;
+    const int x = static_cast<int>(t % wx);
+    t /= wx;
+    const int z = static_cast<int>(t % nz);
+    const long long c = t / nz;
+    const long long v = (static_cast<long long>(origins[2 * c] + x) * ny + origins[2 * c + 1] + y) * nz + z;
+    mean[c * nv + v] = static_cast<double>(in_mean[i]);
+    sigma[c * nv + v] = in_sigma ? static_cast<double>(in_sigma[i]) : 0.0;
+  }
+}
+
 __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v) {
   for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -1149,6 +1242,7 @@ struct hmc_ctx {
   std::string error;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  std::unordered_map<void*, size_t> alloc_bytes;  // dev_alloc capacities
 
   // transport grid
   DevGrid g{};
@@ -1166,6 +1260,7 @@ struct hmc_ctx {
   double* d_s1 = nullptr;
   double* d_s2 = nullptr;
   double* d_nhist = nullptr;
+  int* d_origins = nullptr;  // window origins for hmc_gather_windows / hmc_scatter_windows
   bool has_scoring = false;
   bool finalized = false;
 
@@ -1197,7 +1292,7 @@ struct hmc_ctx {
   DevItem* d_items = nullptr;
   size_t items_cap = 0;
   bool profile = false;
-  hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  std::vector<hipEvent_t> ev_pool;  // per-launch timing events (profiling)
 
   // truncation
   unsigned long long* d_colmax = nullptr;
@@ -1225,19 +1320,27 @@ int fail(hmc_ctx* ctx, int code, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(_e));                   \
   } while (0)
 
+// Device buffer of at least `count` elements: an existing allocation that is large
+// enough is kept (hipMalloc/hipFree synchronise the device, so repeated runs with the
+// same shapes must not reallocate), otherwise it is replaced.
 template <typename T>
 int dev_alloc(hmc_ctx* ctx, T** ptr, size_t count) {
+  if (count == 0) count = 1;
+  const size_t bytes = count * sizeof(T);
   if (*ptr) {
+    auto it = ctx->alloc_bytes.find(static_cast<void*>(*ptr));
+    if (it != ctx->alloc_bytes.end() && it->second >= bytes) return HMC_OK;
+    if (it != ctx->alloc_bytes.end()) ctx->alloc_bytes.erase(it);
     (void)hipFree(*ptr);
     *ptr = nullptr;
   }
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), bytes);
   if (e != hipSuccess) {
     *ptr = nullptr;
     return fail(ctx, e == hipErrorOutOfMemory ? HMC_ENOMEM : HMC_EHIP,
                 std::string("hipMalloc: ") + hipGetErrorString(e));
   }
+  ctx->alloc_bytes[static_cast<void*>(*ptr)] = bytes;
   return HMC_OK;
 }
 
@@ -1301,12 +1404,11 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_up,           ctx->d_q[0],        ctx->d_q[1],       ctx->d_q[2],
                   ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
                   ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
-                  ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma};
+                  ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
-  for (auto& pair : ctx->ev)
-    for (auto& e : pair)
-      if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_pool)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1608,11 +1710,7 @@ int hmc_set_physics(hmc_ctx* ctx, const hmc_physics* p) {
 int hmc_set_profiling(hmc_ctx* ctx, int enabled) {
   if (!ctx) return HMC_EINVAL;
   HMC_HIP(ctx, hipSetDevice(ctx->device));
-  if (enabled && !ctx->ev[0][0]) {
-    for (int k = 0; k < 2; ++k)
-      for (int j = 0; j < 2; ++j) HMC_HIP(ctx, hipEventCreate(&ctx->ev[k][j]));
-  }
-  ctx->profile = enabled != 0;
+  ctx->profile = enabled != 0;  // events are created by the first profiled transport
   return HMC_OK;
 }
 
@@ -1685,24 +1783,25 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
                      static_cast<long long>(L_COUNT), 0ull);
 
   const unsigned int cap = static_cast<unsigned int>(ctx->q_capacity);
-  const long long max_lanes = static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048;
+  // Rounds launch at full width: lanes of an exhausted queue exit at once, so the host
+  // never needs a queue's size to launch it, and it enqueues kRoundsPerSync photon /
+  // electron rounds between count reads.
+  const unsigned int round_blocks =
+      static_cast<unsigned int>(blocks_for(static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048));
+  constexpr int kRoundsPerSync = 4;
   long long overflow = 0, rounds = 0, launches[2] = {0, 0};
-  // Pops and pushes per kernel, from the queue counts read between rounds.
-  long long pops[2] = {0, 0}, pushes[2] = {0, 0};
   double kernel_ms[2] = {0.0, 0.0};
-  bool pending[2] = {false, false};
-  auto harvest = [&](int k) -> int {  // after a stream sync
-    if (!pending[k]) return HMC_OK;
-    float ms = 0.0f;
-    HMC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[k][0], ctx->ev[k][1]));
-    kernel_ms[k] += ms;
-    pending[k] = false;
-    return HMC_OK;
-  };
+  if (ctx->profile && ctx->ev_pool.size() < 4 * kRoundsPerSync) {
+    for (size_t k = ctx->ev_pool.size(); k < 4 * kRoundsPerSync; ++k) {
+      hipEvent_t e = nullptr;
+      HMC_HIP(ctx, hipEventCreate(&e));
+      ctx->ev_pool.push_back(e);
+    }
+  }
   unsigned int* counts = ctx->d_counts;
   unsigned int* heads = ctx->d_counts + 4;
 
-  for (long long p0 = 0; p0 < total; p0 += chunk_n) {
+  for (long long p0 = 0; p0 < total && !overflow; p0 += chunk_n) {
     const long long n_gen = std::min(chunk_n, total - p0);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, 8, 0u);
     int pa = 0, ea = 1, pb = 2, eb = 3;  // queue roles
@@ -1711,82 +1810,55 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
                        ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
                        static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());
-    bool electron_ran = false;
-    unsigned int pb_after_photon = 0;
-    for (int round = 0; round < 100000; ++round) {
-      unsigned int host_counts[4];
-      HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-      HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      if ((rc = harvest(1))) return rc;
-      for (int q = 0; q < 4; ++q)
-        if (host_counts[q] > cap) overflow = std::max<long long>(overflow, host_counts[q]);
-      if (overflow) break;
-      if (electron_ran) {  // the last electron round's photons (beyond the photon round's) and electrons
-        pushes[1] += static_cast<long long>(host_counts[pa] - pb_after_photon) + host_counts[ea];
-        electron_ran = false;
-      }
-      const unsigned int n_p = host_counts[pa], n_e = host_counts[ea];
-      if (n_p == 0 && n_e == 0) break;
-      ++rounds;
-      Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
-      Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
-      if (n_p) {
+    while (true) {
+      for (int r = 0; r < kRoundsPerSync; ++r) {
+        Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
+        Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
+        hipEvent_t* ev = ctx->profile ? &ctx->ev_pool[4 * r] : nullptr;
         // Photon round: consume pa; electrons append to ea, photons to pb.
-        const long long lanes = std::min<long long>(n_p, max_lanes);
-        if (ctx->profile) HMC_HIP(ctx, hipEventRecord(ctx->ev[0][0], ctx->stream));
-        hipLaunchKernelGGL(k_photon_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream, P,
-                           qa_p, heads + pa, qa_e, qb_p);
+        if (ev) HMC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+        hipLaunchKernelGGL(k_photon_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p,
+                           heads + pa, qa_e, qb_p);
         HMC_HIP(ctx, hipGetLastError());
-        if (ctx->profile) {
-          HMC_HIP(ctx, hipEventRecord(ctx->ev[0][1], ctx->stream));
-          pending[0] = true;
-        }
+        if (ev) HMC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+        // Electron round: consume ea (including the photon round's pushes).
+        if (ev) HMC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+        hipLaunchKernelGGL(k_electron_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
+                           heads + ea, qb_p, qb_e);
+        HMC_HIP(ctx, hipGetLastError());
+        if (ev) HMC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+        hipLaunchKernelGGL(k_reset_consumed, dim3(1), dim3(64), 0, ctx->stream, counts, heads, pa,
+                           ea, cap, ctx->d_ledger);
+        std::swap(pa, pb);
+        std::swap(ea, eb);
+        ++rounds;
         ++launches[0];
+        ++launches[1];
       }
-      // Electron round: consume ea (including the photon round's pushes).
-      unsigned int mid_counts[4];
-      HMC_HIP(ctx, hipMemcpyAsync(mid_counts, counts, sizeof(mid_counts), hipMemcpyDeviceToHost,
+      unsigned int host_counts[4];
+      HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts), hipMemcpyDeviceToHost,
                                   ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      if ((rc = harvest(0))) return rc;
-      const unsigned int n_e_now = mid_counts[ea];
-      if (n_e_now > cap || mid_counts[pb] > cap) {
-        overflow = std::max(n_e_now, mid_counts[pb]);
-        break;
-      }
-      pops[0] += n_p;
-      pushes[0] += static_cast<long long>(n_e_now - n_e) + mid_counts[pb];
-      pb_after_photon = mid_counts[pb];
-      if (n_e_now) {
-        const long long lanes = std::min<long long>(n_e_now, max_lanes);
-        if (ctx->profile) HMC_HIP(ctx, hipEventRecord(ctx->ev[1][0], ctx->stream));
-        hipLaunchKernelGGL(k_electron_round, dim3(blocks_for(lanes)), dim3(256), 0, ctx->stream,
-                           P, qa_e, heads + ea, qb_p, qb_e);
-        HMC_HIP(ctx, hipGetLastError());
-        if (ctx->profile) {
-          HMC_HIP(ctx, hipEventRecord(ctx->ev[1][1], ctx->stream));
-          pending[1] = true;
+      if (ctx->profile) {
+        for (int r = 0; r < kRoundsPerSync; ++r) {
+          for (int k = 0; k < 2; ++k) {
+            float ms = 0.0f;
+            HMC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_pool[4 * r + 2 * k], ctx->ev_pool[4 * r + 2 * k + 1]));
+            kernel_ms[k] += ms;
+          }
         }
-        ++launches[1];
-        pops[1] += n_e_now;
-        electron_ran = true;
       }
-      // Reset the consumed queues (counts and heads), then swap roles.
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + pa, 1, 0u);
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts + ea, 1, 0u);
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, heads + pa, 1, 0u);
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, heads + ea, 1, 0u);
-      std::swap(pa, pb);
-      std::swap(ea, eb);
+      for (int q = 0; q < 4; ++q)
+        if (host_counts[q] > cap) overflow = std::max<long long>(overflow, host_counts[q]);
+      if (overflow || (host_counts[pa] == 0 && host_counts[ea] == 0)) break;
+      if (rounds > 100000) return fail(ctx, HMC_EHIP, "transport did not drain in 100000 rounds");
     }
-    if (overflow) break;
   }
   unsigned long long host_ledger[L_COUNT];
   HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger),
                               hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if ((rc = harvest(0)) || (rc = harvest(1))) return rc;
+  overflow = std::max<long long>(overflow, static_cast<long long>(host_ledger[L_OVERFLOW]));
   auto L = [&](int slot) { return static_cast<int64_t>(host_ledger[slot]); };
   ledger->escaped = L(L_ESCAPED);
   ledger->unscored = L(L_UNSCORED);
@@ -1798,13 +1870,13 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   ledger->rounds = rounds;
   ledger->photon_voxels = L(L_P_VOX);
   ledger->photon_deposits = L(L_P_DEP);
-  ledger->photon_pushes = pushes[0];
-  ledger->photon_pops = pops[0];
+  ledger->photon_pushes = L(L_P_PUSH);
+  ledger->photon_pops = L(L_P_POP);
   ledger->photon_steps = L(L_P_STEPS);
   ledger->electron_voxels = L(L_E_VOX);
   ledger->electron_deposits = L(L_E_DEP);
-  ledger->electron_pushes = pushes[1];
-  ledger->electron_pops = pops[1];
+  ledger->electron_pushes = L(L_E_PUSH);
+  ledger->electron_pops = L(L_E_POP);
   ledger->electron_steps = L(L_E_STEPS);
   ledger->photon_launches = launches[0];
   ledger->electron_launches = launches[1];
@@ -1918,6 +1990,61 @@ int hmc_truncate(hmc_ctx* ctx, double truncation, int64_t* counts, int64_t* tota
                      ctx->d_chunk_offsets, ctx->d_out_idx, ctx->d_out_dose, ctx->d_out_sigma);
   HMC_HIP(ctx, hipGetLastError());
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+static int check_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, int wx, int wy) {
+  if (!ctx || !origins) return fail(ctx, HMC_EINVAL, "null");
+  if (!ctx->has_scoring) return fail(ctx, HMC_ESTATE, "scoring grid not set");
+  if (n_cols != ctx->n_columns) return fail(ctx, HMC_EINVAL, "window count must equal the column count");
+  if (wx < 1 || wy < 1 || wx > ctx->s.nx || wy > ctx->s.ny) return fail(ctx, HMC_EINVAL, "bad window size");
+  for (int64_t c = 0; c < n_cols; ++c) {
+    const int x0 = origins[2 * c], y0 = origins[2 * c + 1];
+    if (x0 < 0 || y0 < 0 || x0 + wx > ctx->s.nx || y0 + wy > ctx->s.ny)
+      return fail(ctx, HMC_EINVAL, "window outside the scoring grid");
+  }
+  return HMC_OK;
+}
+
+int hmc_gather_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, int wx, int wy,
+                       float* out) {
+  int rc = check_windows(ctx, origins, n_cols, wx, wy);
+  if (rc) return rc;
+  if (!out) return fail(ctx, HMC_EINVAL, "null output");
+  if (!ctx->finalized) return fail(ctx, HMC_ESTATE, "finalize before gathering windows");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  int* d_org = nullptr;
+  if ((rc = upload(ctx, &ctx->d_origins, origins, static_cast<size_t>(2 * n_cols)))) return rc;
+  d_org = ctx->d_origins;
+  const long long total = n_cols * ctx->s.nz * wx * wy;
+  hipLaunchKernelGGL(k_gather_windows, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream,
+                     ctx->d_s1, d_org, static_cast<long long>(n_cols), wx, wy, ctx->s.ny, ctx->s.nz,
+                     ctx->s.n_voxels, out);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+int hmc_scatter_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, int wx, int wy,
+                        const float* mean, const float* sigma) {
+  int rc = check_windows(ctx, origins, n_cols, wx, wy);
+  if (rc) return rc;
+  if (!mean) return fail(ctx, HMC_EINVAL, "null input");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  if ((rc = upload(ctx, &ctx->d_origins, origins, static_cast<size_t>(2 * n_cols)))) return rc;
+  const long long nvc = ctx->s.n_voxels * ctx->n_columns;
+  const int blocks = blocks_for(nvc, 256, 65536);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_s1), nvc, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_s2), nvc, 0ull);
+  const long long total = n_cols * ctx->s.nz * wx * wy;
+  hipLaunchKernelGGL(k_scatter_windows, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream,
+                     ctx->d_s1, ctx->d_s2, ctx->d_origins, static_cast<long long>(n_cols), wx, wy,
+                     ctx->s.ny, ctx->s.nz, ctx->s.n_voxels, mean, sigma);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->finalized = true;  // s1 / s2 now hold mean / sigma for hmc_truncate
   return HMC_OK;
 }
 

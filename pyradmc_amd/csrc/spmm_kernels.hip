@@ -1,0 +1,635 @@
+// Dij products on the device: the planning loop's D = Dij @ W for many weight vectors
+// at once (the reference's DijResult.dose_for_weights, pyradmc/scoring/dij.py:251-260,
+// forms one vector's dose on the host). C ABI in include/hipsp.h.
+//
+// Set-up (hsp_create) turns the CSC Dij into two row-major forms on the device:
+//  * CSR with every row in ascending column order (repeated entries in CSC order):
+//    the columns are scattered one launch per column, in column order, so rows come
+//    out sorted, and a per-row insertion pass settles ties by CSC position;
+//  * row blocks: for each block of 8 consecutive rows (voxels adjacent along the
+//    fastest grid axis, which see nearly the same beamlets), the union of the 8 rows'
+//    columns in ascending order, each with its 8 values (0 where a row has no entry).
+// Values stay in the type the caller gave (float64, as the reference's Dij holds them,
+// or float32); a float64 product reads either exactly.
+// A product walks the row blocks. One weight row W[j][.] loaded from cache serves up
+// to 8 voxels, which cuts the L2 traffic for the weights by the block's overlap. Every
+// voxel still adds its own entries in ascending column order: in float64 with the
+// product rounded before the add, this is the reference's sum bit for bit.
+//
+// Roofline (HBM): per product, 4 + value bytes per CSR nonzero (as row blocks: 4 + 8
+// values per block entry, read once per pass over 256 vectors), the output (8 or 4
+// bytes per (row, vector)) and the weights (n_cols * n_w elements, cache-resident).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hipsp.h"
+
+namespace {
+constexpr int kBlockRows = 8;  // rows per row block
+}
+
+struct hsp_matrix {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  long long n_rows = 0, n_cols = 0, nnz = 0;
+  int value_type = HSP_F64;      // element type of val / blk_val
+  size_t value_bytes = 8;
+  long long* row_ptr = nullptr;  // CSR [n_rows + 1]
+  int* col = nullptr;            // [nnz]
+  void* val = nullptr;           // [nnz]
+  long long n_blocks = 0;        // ceil(n_rows / 8)
+  long long n_block_entries = 0;
+  long long* blk_ptr = nullptr;  // [n_blocks + 1]
+  int* blk_col = nullptr;        // [n_block_entries]
+  void* blk_val = nullptr;       // [n_block_entries][8]
+  void* w = nullptr;             // weights on the device (capacity w_bytes)
+  size_t w_bytes = 0;
+  void* out = nullptr;           // internal product buffer (capacity out_bytes)
+  size_t out_bytes = 0;
+  long long out_r0 = 0, out_r1 = 0;
+  int out_nw = 0, out_precision = HSP_F64;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::string error;
+};
+
+namespace {
+
+int fail(hsp_matrix* m, int code, const std::string& msg) {
+  if (m) m->error = msg;
+  return code;
+}
+
+#define HSP_HIP(m, call)                                                                       \
+  do {                                                                                         \
+    hipError_t _e = (call);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return fail(m, _e == hipErrorOutOfMemory ? HSP_ENOMEM : HSP_EHIP,                        \
+                  std::string(#call) + ": " + hipGetErrorString(_e));                          \
+  } while (0)
+
+unsigned int grid_for(long long n, long long cap = 65536) {
+  long long b = (n + 255) / 256;
+  return static_cast<unsigned int>(std::max(1ll, std::min(b, cap)));
+}
+
+// ---------------------------------------------------------------------------------------
+// Set-up kernels.
+
+__global__ void k_count_rows(const long long* __restrict__ rows, long long nnz,
+                             unsigned int* __restrict__ count) {
+  for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < nnz;
+       e += static_cast<long long>(gridDim.x) * blockDim.x)
+    atomicAdd(count + rows[e], 1u);
+}
+
+// Exclusive scan of n counts into out[0..n] (out[n] = total): per-tile sums, a serial
+// scan of the tile sums, then a scan inside each 1024-entry tile.
+constexpr int kScanTile = 1024;
+
+__global__ void k_tile_sums(const unsigned int* __restrict__ count, long long n,
+                            long long* __restrict__ tile_sum) {
+  __shared__ long long red[256];
+  const long long base = static_cast<long long>(blockIdx.x) * kScanTile;
+  long long s = 0;
+  for (int i = threadIdx.x; i < kScanTile; i += blockDim.x)
+    if (base + i < n) s += count[base + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (static_cast<int>(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = red[0];
+}
+
+__global__ void k_scan_tile_sums(long long* __restrict__ tile_sum, long long tiles,
+                                 long long* __restrict__ total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    long long run = 0;
+    for (long long t = 0; t < tiles; ++t) {
+      const long long v = tile_sum[t];
+      tile_sum[t] = run;
+      run += v;
+    }
+    *total = run;
+  }
+}
+
+__global__ void k_tile_scan(const unsigned int* __restrict__ count, long long n,
+                            const long long* __restrict__ tile_off, long long* __restrict__ out) {
+  __shared__ long long vals[kScanTile];
+  const long long base = static_cast<long long>(blockIdx.x) * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += blockDim.x)
+    vals[i] = base + i < n ? count[base + i] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // serial scan of one tile (1024 adds)
+    long long run = tile_off[blockIdx.x];
+    for (int i = 0; i < kScanTile; ++i) {
+      const long long v = vals[i];
+      vals[i] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kScanTile; i += blockDim.x)
+    if (base + i < n) out[base + i] = vals[i];
+}
+
+// Entries [e_lo, e_hi) of CSC column c into their rows. One launch per column, in
+// column order, so each row's entries arrive in ascending column order.
+template <typename V>
+__global__ void k_scatter_column(long long e_lo, long long e_hi, int c, const long long* __restrict__ rows,
+                                 const V* __restrict__ values, const long long* __restrict__ row_ptr,
+                                 unsigned int* __restrict__ cursor, int* __restrict__ col,
+                                 V* __restrict__ val, long long* __restrict__ src) {
+  const long long e = e_lo + static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= e_hi) return;
+  const long long r = rows[e];
+  const long long pos = row_ptr[r] + atomicAdd(cursor + r, 1u);
+  col[pos] = c;
+  val[pos] = values[e];
+  src[pos] = e;
+}
+
+// Order every row by (column, CSC position). Rows arrive sorted by column, so this
+// only reorders repeated (row, column) entries; it is linear in the row length.
+template <typename V>
+__global__ void k_sort_rows(const long long* __restrict__ row_ptr, long long n_rows,
+                            int* __restrict__ col, V* __restrict__ val, long long* __restrict__ src) {
+  for (long long r = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_rows;
+       r += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long lo = row_ptr[r], hi = row_ptr[r + 1];
+    for (long long i = lo + 1; i < hi; ++i) {
+      const int c = col[i];
+      const V v = val[i];
+      const long long s = src[i];
+      long long j = i - 1;
+      while (j >= lo && (col[j] > c || (col[j] == c && src[j] > s))) {
+        col[j + 1] = col[j];
+        val[j + 1] = val[j];
+        src[j + 1] = src[j];
+        --j;
+      }
+      col[j + 1] = c;
+      val[j + 1] = v;
+      src[j + 1] = s;
+    }
+  }
+}
+
+// Merge of the (up to 8) sorted rows of block b: calls emit(column, values[8]) once
+// per union entry, in ascending column order. A repeated (row, column) pair becomes
+// a second union entry for the same column, so each row keeps its own order.
+template <typename V, typename Emit>
+__device__ void merge_block(const long long* __restrict__ row_ptr, const int* __restrict__ col,
+                            const V* __restrict__ val, long long n_rows, long long b, Emit emit) {
+  long long cur[kBlockRows], end[kBlockRows];
+  for (int i = 0; i < kBlockRows; ++i) {
+    const long long r = b * kBlockRows + i;
+    cur[i] = r < n_rows ? row_ptr[r] : 0;
+    end[i] = r < n_rows ? row_ptr[r + 1] : 0;
+  }
+  for (;;) {
+    int c_min = 0x7fffffff;
+    bool any = false;
+    for (int i = 0; i < kBlockRows; ++i)
+      if (cur[i] < end[i]) {
+        c_min = min(c_min, col[cur[i]]);
+        any = true;
+      }
+    if (!any) return;
+    V v[kBlockRows];
+    for (int i = 0; i < kBlockRows; ++i) {
+      v[i] = V(0);
+      if (cur[i] < end[i] && col[cur[i]] == c_min) v[i] = val[cur[i]++];
+    }
+    emit(c_min, v);
+  }
+}
+
+template <typename V>
+__global__ void k_block_count(const long long* __restrict__ row_ptr, const int* __restrict__ col,
+                              const V* __restrict__ val, long long n_rows, long long n_blocks,
+                              unsigned int* __restrict__ count) {
+  for (long long b = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; b < n_blocks;
+       b += static_cast<long long>(gridDim.x) * blockDim.x) {
+    unsigned int n = 0;
+    merge_block(row_ptr, col, val, n_rows, b, [&](int, const V*) { ++n; });
+    count[b] = n;
+  }
+}
+
+template <typename V>
+__global__ void k_block_fill(const long long* __restrict__ row_ptr, const int* __restrict__ col,
+                             const V* __restrict__ val, long long n_rows, long long n_blocks,
+                             const long long* __restrict__ blk_ptr, int* __restrict__ blk_col,
+                             V* __restrict__ blk_val) {
+  for (long long b = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; b < n_blocks;
+       b += static_cast<long long>(gridDim.x) * blockDim.x) {
+    long long u = blk_ptr[b];
+    merge_block(row_ptr, col, val, n_rows, b, [&](int c, const V* v) {
+      blk_col[u] = c;
+      V* dst = blk_val + u * kBlockRows;
+      for (int i = 0; i < kBlockRows; ++i) dst[i] = v[i];
+      ++u;
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Product kernels.
+
+// acc + v * w. float64: product rounded, then the add (the reference's two numpy
+// operations); float32: one fused multiply-add.
+__device__ __forceinline__ double madd(double acc, double v, double w) {
+#pragma clang fp contract(off)
+  const double p = v * w;
+  return acc + p;
+}
+__device__ __forceinline__ float madd(float acc, float v, float w) { return fmaf(v, w, acc); }
+
+// Workgroup index remapped so that each XCD (workgroups go round-robin over the 8)
+// takes one contiguous range of row blocks: its L2 then holds the weight rows of one
+// region of the grid.
+__device__ __forceinline__ long long xcd_local_group() {
+  const long long g = gridDim.x, bid = blockIdx.x;
+  const long long xcd = bid & 7, idx = bid >> 3;
+  long long start = 0;
+  for (long long x = 0; x < xcd; ++x) start += (g - x + 7) >> 3;
+  return start + idx;
+}
+
+// Many vectors: a wave per row block, lanes over vectors (64 * KL per pass).
+template <typename T, typename V, int KL>
+__global__ void __launch_bounds__(256) k_spmm_wide(const long long* __restrict__ blk_ptr,
+                                                   const int* __restrict__ blk_col,
+                                                   const V* __restrict__ blk_val,
+                                                   const T* __restrict__ w, int n_w, long long r0,
+                                                   long long r1, T* __restrict__ out) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  // The wave index is uniform; saying so keeps the block's columns and values in
+  // scalar registers and its empty-entry tests on the scalar unit.
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const long long b = r0 / kBlockRows + xcd_local_group() * 4 + wave;
+  if (b * kBlockRows >= r1) return;
+  const long long u0 = blk_ptr[b], u1 = blk_ptr[b + 1];
+  for (int k0 = 0; k0 < n_w; k0 += 64 * KL) {
+    T acc[kBlockRows][KL];
+#pragma unroll
+    for (int i = 0; i < kBlockRows; ++i)
+#pragma unroll
+      for (int q = 0; q < KL; ++q) acc[i][q] = T(0);
+    for (long long u = u0; u < u1; ++u) {
+      const long long j = blk_col[u];
+      V v[kBlockRows];
+#pragma unroll
+      for (int i = 0; i < kBlockRows; ++i) v[i] = blk_val[u * kBlockRows + i];
+      T wv[KL];
+#pragma unroll
+      for (int q = 0; q < KL; ++q) {
+        const int k = k0 + q * 64 + lane;
+        wv[q] = k < n_w ? w[j * n_w + k] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < kBlockRows; ++i)
+        if (v[i] != V(0)) {
+#pragma unroll
+          for (int q = 0; q < KL; ++q) acc[i][q] = madd(acc[i][q], static_cast<T>(v[i]), wv[q]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kBlockRows; ++i) {
+      const long long r = b * kBlockRows + i;
+      if (r < r0 || r >= r1) continue;
+      T* dst = out + (r - r0) * n_w;
+#pragma unroll
+      for (int q = 0; q < KL; ++q) {
+        const int k = k0 + q * 64 + lane;
+        if (k < n_w) dst[k] = acc[i][q];
+      }
+    }
+  }
+}
+
+// Few vectors (n_w <= 8): 8 lanes per row block, a lane per row.
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) k_spmm_narrow(const long long* __restrict__ blk_ptr,
+                                                     const int* __restrict__ blk_col,
+                                                     const V* __restrict__ blk_val,
+                                                     const T* __restrict__ w, int n_w, long long r0,
+                                                     long long r1, T* __restrict__ out) {
+  const int i = static_cast<int>(threadIdx.x & (kBlockRows - 1));
+  const long long b = r0 / kBlockRows + xcd_local_group() * (256 / kBlockRows) + (threadIdx.x / kBlockRows);
+  if (b * kBlockRows >= r1) return;
+  T acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = T(0);
+  const long long u1 = blk_ptr[b + 1];
+  for (long long u = blk_ptr[b]; u < u1; ++u) {
+    const V v = blk_val[u * kBlockRows + i];
+    if (v == V(0)) continue;
+    const T* wr = w + static_cast<long long>(blk_col[u]) * n_w;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < n_w) acc[q] = madd(acc[q], static_cast<T>(v), wr[q]);
+  }
+  const long long r = b * kBlockRows + i;
+  if (r < r0 || r >= r1) return;
+  T* dst = out + (r - r0) * n_w;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < n_w) dst[q] = acc[q];
+}
+
+template <typename T, typename V>
+void launch_product(hsp_matrix* m, const T* w, int n_w, long long r0, long long r1, T* out) {
+  const V* bv = static_cast<const V*>(m->blk_val);
+  const long long blocks = (r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows;
+  if (n_w <= 8) {
+    const long long groups = (blocks + 256 / kBlockRows - 1) / (256 / kBlockRows);
+    hipLaunchKernelGGL((k_spmm_narrow<T, V>), dim3(static_cast<unsigned int>(groups)), dim3(256), 0, m->stream,
+                       m->blk_ptr, m->blk_col, bv, w, n_w, r0, r1, out);
+    return;
+  }
+  const dim3 grid(static_cast<unsigned int>((blocks + 3) / 4));
+  if (n_w <= 64)
+    hipLaunchKernelGGL((k_spmm_wide<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
+                       r0, r1, out);
+  else if (n_w <= 128)
+    hipLaunchKernelGGL((k_spmm_wide<T, V, 2>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
+                       r0, r1, out);
+  else
+    hipLaunchKernelGGL((k_spmm_wide<T, V, 4>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
+                       r0, r1, out);
+}
+
+// Exclusive scan of n device counts into out[0..n]; returns the total.
+int scan_counts(hsp_matrix* m, const unsigned int* count, long long n, long long* out, long long* total) {
+  const long long tiles = std::max((n + kScanTile - 1) / kScanTile, 1ll);
+  long long* d_tile = nullptr;
+  HSP_HIP(m, hipMalloc(&d_tile, sizeof(long long) * tiles));
+  hipLaunchKernelGGL(k_tile_sums, dim3(static_cast<unsigned int>(tiles)), dim3(256), 0, m->stream, count, n,
+                     d_tile);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(64), 0, m->stream, d_tile, tiles, out + n);
+  hipLaunchKernelGGL(k_tile_scan, dim3(static_cast<unsigned int>(tiles)), dim3(256), 0, m->stream, count, n,
+                     d_tile, out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(total, out + n, sizeof(long long), hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  (void)hipFree(d_tile);
+  HSP_HIP(m, e);
+  return HSP_OK;
+}
+
+// Set-up after the matrix's arrays are allocated; temporaries freed on every path.
+template <typename V>
+int build_forms(hsp_matrix* m, const int64_t* indptr, const int64_t* indices, const V* values) {
+  V* val = static_cast<V*>(m->val);
+  const long long n_rows = m->n_rows, n_cols = m->n_cols, nnz = m->nnz;
+  const size_t nz = static_cast<size_t>(std::max(nnz, 1ll));
+  long long *d_rows = nullptr, *d_src = nullptr;
+  V* d_values = nullptr;
+  unsigned int* d_count = nullptr;
+  auto release = [&]() {
+    for (void* p : {static_cast<void*>(d_rows), static_cast<void*>(d_src), static_cast<void*>(d_values),
+                    static_cast<void*>(d_count)})
+      if (p) (void)hipFree(p);
+    d_rows = d_src = nullptr;
+    d_values = nullptr;
+    d_count = nullptr;
+  };
+  auto run = [&]() -> int {
+    HSP_HIP(m, hipMalloc(&d_rows, sizeof(long long) * nz));
+    HSP_HIP(m, hipMalloc(&d_src, sizeof(long long) * nz));
+    HSP_HIP(m, hipMalloc(&d_values, sizeof(V) * nz));
+    HSP_HIP(m, hipMalloc(&d_count, sizeof(unsigned int) * std::max(n_rows, m->n_blocks)));
+    if (nnz) {
+      HSP_HIP(m, hipMemcpy(d_rows, indices, sizeof(long long) * nnz, hipMemcpyHostToDevice));
+      HSP_HIP(m, hipMemcpy(d_values, values, sizeof(V) * nnz, hipMemcpyHostToDevice));
+    }
+    HSP_HIP(m, hipMemsetAsync(d_count, 0, sizeof(unsigned int) * n_rows, m->stream));
+    hipLaunchKernelGGL(k_count_rows, dim3(grid_for(nnz)), dim3(256), 0, m->stream, d_rows, nnz, d_count);
+    HSP_HIP(m, hipGetLastError());
+    long long total = 0;
+    if (int rc = scan_counts(m, d_count, n_rows, m->row_ptr, &total)) return rc;
+    if (total != nnz) return fail(m, HSP_EHIP, "row count scan disagrees with nnz");
+    HSP_HIP(m, hipMemsetAsync(d_count, 0, sizeof(unsigned int) * n_rows, m->stream));  // cursors
+    for (long long c = 0; c < n_cols; ++c) {
+      const long long lo = indptr[c], hi = indptr[c + 1];
+      if (hi == lo) continue;
+      hipLaunchKernelGGL(k_scatter_column<V>, dim3(static_cast<unsigned int>((hi - lo + 255) / 256)), dim3(256),
+                         0, m->stream, lo, hi, static_cast<int>(c), d_rows, d_values, m->row_ptr, d_count, m->col,
+                         val, d_src);
+    }
+    hipLaunchKernelGGL(k_sort_rows<V>, dim3(grid_for(n_rows)), dim3(256), 0, m->stream, m->row_ptr, n_rows, m->col,
+                       val, d_src);
+    HSP_HIP(m, hipGetLastError());
+    HSP_HIP(m, hipMemsetAsync(d_count, 0, sizeof(unsigned int) * m->n_blocks, m->stream));
+    hipLaunchKernelGGL(k_block_count<V>, dim3(grid_for(m->n_blocks)), dim3(256), 0, m->stream, m->row_ptr,
+                       m->col, val, n_rows, m->n_blocks, d_count);
+    HSP_HIP(m, hipGetLastError());
+    if (int rc = scan_counts(m, d_count, m->n_blocks, m->blk_ptr, &m->n_block_entries)) return rc;
+    const size_t ne = static_cast<size_t>(std::max(m->n_block_entries, 1ll));
+    HSP_HIP(m, hipMalloc(&m->blk_col, sizeof(int) * ne));
+    HSP_HIP(m, hipMalloc(&m->blk_val, sizeof(V) * kBlockRows * ne));
+    hipLaunchKernelGGL(k_block_fill<V>, dim3(grid_for(m->n_blocks)), dim3(256), 0, m->stream, m->row_ptr,
+                       m->col, val, n_rows, m->n_blocks, m->blk_ptr, m->blk_col, static_cast<V*>(m->blk_val));
+    HSP_HIP(m, hipGetLastError());
+    HSP_HIP(m, hipStreamSynchronize(m->stream));
+    return HSP_OK;
+  };
+  const int rc = run();
+  release();
+  return rc;
+}
+
+int apply_on_device(hsp_matrix* m, const void* w, int n_w, long long r0, long long r1, int precision,
+                    void* out_device, double* kernel_ms) {
+  const size_t elem = precision == HSP_F64 ? sizeof(double) : sizeof(float);
+  if (!out_device) {
+    const size_t need = static_cast<size_t>(r1 - r0) * static_cast<size_t>(n_w) * elem;
+    if (need > m->out_bytes) {
+      if (m->out) (void)hipFree(m->out);
+      m->out = nullptr;
+      m->out_bytes = 0;
+      HSP_HIP(m, hipMalloc(&m->out, need));
+      m->out_bytes = need;
+    }
+    out_device = m->out;
+    m->out_r0 = r0;
+    m->out_r1 = r1;
+    m->out_nw = n_w;
+    m->out_precision = precision;
+  }
+  if (kernel_ms && !m->ev[0]) {
+    HSP_HIP(m, hipEventCreate(&m->ev[0]));
+    HSP_HIP(m, hipEventCreate(&m->ev[1]));
+  }
+  if (kernel_ms) HSP_HIP(m, hipEventRecord(m->ev[0], m->stream));
+  const bool v64 = m->value_type == HSP_F64;
+  if (precision == HSP_F64) {
+    const auto* wd = static_cast<const double*>(w);
+    auto* od = static_cast<double*>(out_device);
+    if (v64)
+      launch_product<double, double>(m, wd, n_w, r0, r1, od);
+    else
+      launch_product<double, float>(m, wd, n_w, r0, r1, od);
+  } else {
+    const auto* wf = static_cast<const float*>(w);
+    auto* of = static_cast<float*>(out_device);
+    if (v64)
+      launch_product<float, double>(m, wf, n_w, r0, r1, of);
+    else
+      launch_product<float, float>(m, wf, n_w, r0, r1, of);
+  }
+  HSP_HIP(m, hipGetLastError());
+  if (kernel_ms) HSP_HIP(m, hipEventRecord(m->ev[1], m->stream));
+  HSP_HIP(m, hipStreamSynchronize(m->stream));
+  if (kernel_ms) {
+    float ms = 0.0f;
+    HSP_HIP(m, hipEventElapsedTime(&ms, m->ev[0], m->ev[1]));
+    *kernel_ms = ms;
+  }
+  return HSP_OK;
+}
+
+int check_apply(hsp_matrix* m, int n_w, long long r0, long long r1, int precision) {
+  if (n_w < 1) return fail(m, HSP_EINVAL, "n_w must be at least 1");
+  if (precision != HSP_F64 && precision != HSP_F32) return fail(m, HSP_EINVAL, "unknown precision");
+  if (r0 < 0 || r1 > m->n_rows || r0 >= r1) return fail(m, HSP_EINVAL, "row range outside the matrix");
+  if ((r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows > 4ll * 0x7fffffffll)
+    return fail(m, HSP_EINVAL, "row range too large for one launch");
+  return HSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hsp_last_error(hsp_matrix* m) { return m ? m->error.c_str() : "null matrix"; }
+
+int hsp_create(int device, long long n_rows, long long n_cols, const int64_t* indptr,
+               const int64_t* indices, const void* values, int value_type, hsp_matrix** out) {
+  if (!out || !indptr || n_rows < 1 || n_cols < 1) return HSP_EINVAL;
+  if (value_type != HSP_F64 && value_type != HSP_F32) return HSP_EINVAL;
+  *out = nullptr;
+  if (n_cols >= (1ll << 31)) return HSP_EINVAL;  // column indices are int32 on the device
+  if (indptr[0] != 0) return HSP_EINVAL;
+  for (long long c = 0; c < n_cols; ++c)
+    if (indptr[c + 1] < indptr[c]) return HSP_EINVAL;
+  const long long nnz = indptr[n_cols];
+  if (nnz > 0 && (!indices || !values)) return HSP_EINVAL;
+  for (long long e = 0; e < nnz; ++e)
+    if (indices[e] < 0 || indices[e] >= n_rows) return HSP_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return HSP_EHIP;
+  auto* m = new hsp_matrix();
+  m->device = device;
+  m->n_rows = n_rows;
+  m->n_cols = n_cols;
+  m->nnz = nnz;
+  m->value_type = value_type;
+  m->value_bytes = value_type == HSP_F64 ? sizeof(double) : sizeof(float);
+  m->n_blocks = (n_rows + kBlockRows - 1) / kBlockRows;
+  const size_t nz = static_cast<size_t>(std::max(nnz, 1ll));
+  int rc = HSP_OK;
+  auto step = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HSP_OK) {
+      rc = e == hipErrorOutOfMemory ? HSP_ENOMEM : HSP_EHIP;
+      m->error = hipGetErrorString(e);
+    }
+  };
+  step(hipStreamCreate(&m->stream));
+  step(hipMalloc(&m->row_ptr, sizeof(long long) * (n_rows + 1)));
+  step(hipMalloc(&m->col, sizeof(int) * nz));
+  step(hipMalloc(&m->val, m->value_bytes * nz));
+  step(hipMalloc(&m->blk_ptr, sizeof(long long) * (m->n_blocks + 1)));
+  if (rc == HSP_OK)
+    rc = value_type == HSP_F64 ? build_forms(m, indptr, indices, static_cast<const double*>(values))
+                               : build_forms(m, indptr, indices, static_cast<const float*>(values));
+  if (rc != HSP_OK) {
+    hsp_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return HSP_OK;
+}
+
+void hsp_destroy(hsp_matrix* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  for (void* p : {static_cast<void*>(m->row_ptr), static_cast<void*>(m->col), static_cast<void*>(m->val),
+                  static_cast<void*>(m->blk_ptr), static_cast<void*>(m->blk_col), m->blk_val, m->w, m->out})
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : m->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int hsp_apply(hsp_matrix* m, const double* weights, int n_w, long long row_begin, long long row_end,
+              int precision, void* out_device, double* kernel_ms) {
+  if (!m) return HSP_EINVAL;
+  if (!weights) return fail(m, HSP_EINVAL, "null weights");
+  if (int rc = check_apply(m, n_w, row_begin, row_end, precision)) return rc;
+  HSP_HIP(m, hipSetDevice(m->device));
+  const size_t count = static_cast<size_t>(m->n_cols) * static_cast<size_t>(n_w);
+  const size_t bytes = count * (precision == HSP_F64 ? sizeof(double) : sizeof(float));
+  if (bytes > m->w_bytes) {
+    if (m->w) (void)hipFree(m->w);
+    m->w = nullptr;
+    m->w_bytes = 0;
+    HSP_HIP(m, hipMalloc(&m->w, bytes));
+    m->w_bytes = bytes;
+  }
+  if (precision == HSP_F64) {
+    HSP_HIP(m, hipMemcpy(m->w, weights, bytes, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> w32(weights, weights + count);
+    HSP_HIP(m, hipMemcpy(m->w, w32.data(), bytes, hipMemcpyHostToDevice));
+  }
+  return apply_on_device(m, m->w, n_w, row_begin, row_end, precision, out_device, kernel_ms);
+}
+
+int hsp_apply_device(hsp_matrix* m, const void* weights_device, int n_w, long long row_begin,
+                     long long row_end, int precision, void* out_device, double* kernel_ms) {
+  if (!m) return HSP_EINVAL;
+  if (!weights_device) return fail(m, HSP_EINVAL, "null weights");
+  if (int rc = check_apply(m, n_w, row_begin, row_end, precision)) return rc;
+  HSP_HIP(m, hipSetDevice(m->device));
+  return apply_on_device(m, weights_device, n_w, row_begin, row_end, precision, out_device, kernel_ms);
+}
+
+int hsp_fetch(hsp_matrix* m, long long row_begin, long long row_end, void* host) {
+  if (!m) return HSP_EINVAL;
+  if (!host) return fail(m, HSP_EINVAL, "null host buffer");
+  if (!m->out || row_begin < m->out_r0 || row_end > m->out_r1 || row_begin >= row_end)
+    return fail(m, HSP_EINVAL, "rows not in the last internal product");
+  HSP_HIP(m, hipSetDevice(m->device));
+  const size_t elem = m->out_precision == HSP_F64 ? sizeof(double) : sizeof(float);
+  const size_t off = static_cast<size_t>(row_begin - m->out_r0) * m->out_nw * elem;
+  HSP_HIP(m, hipMemcpy(host, static_cast<char*>(m->out) + off,
+                       elem * static_cast<size_t>(row_end - row_begin) * m->out_nw, hipMemcpyDeviceToHost));
+  return HSP_OK;
+}
+
+int hsp_csr(hsp_matrix* m, int64_t* row_ptr, int32_t* col, void* val) {
+  if (!m) return HSP_EINVAL;
+  if (!row_ptr) return fail(m, HSP_EINVAL, "null row_ptr");
+  HSP_HIP(m, hipSetDevice(m->device));
+  HSP_HIP(m, hipMemcpy(row_ptr, m->row_ptr, sizeof(long long) * (m->n_rows + 1), hipMemcpyDeviceToHost));
+  if (m->nnz) {
+    if (!col || !val) return fail(m, HSP_EINVAL, "null col or val");
+    HSP_HIP(m, hipMemcpy(col, m->col, sizeof(int) * m->nnz, hipMemcpyDeviceToHost));
+    HSP_HIP(m, hipMemcpy(val, m->val, m->value_bytes * m->nnz, hipMemcpyDeviceToHost));
+  }
+  return HSP_OK;
+}
+
+long long hsp_block_entries(hsp_matrix* m) { return m ? m->n_block_entries : -1; }
+
+}  // extern "C"

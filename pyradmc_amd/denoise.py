@@ -25,9 +25,18 @@ from __future__ import annotations
 
 import ctypes
 import math
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
+
+# The denoiser's buffers are torch tensors and its kernels run on torch's stream, so
+# the process must use one HIP runtime: torch's. torch is therefore imported here, at
+# module import, before the native library can be loaded by anything else.
+try:
+    import torch as _torch_module
+except ImportError:  # pragma: no cover - the denoiser then refuses to run
+    _torch_module = None
 
 from pyradmc_amd import _native
 
@@ -71,10 +80,15 @@ def _lib():
 
 
 def _torch():
-    import torch
-
+    torch = _torch_module
+    if torch is None:
+        raise RuntimeError("the denoiser needs PyTorch (ROCm) for its device buffers")
     if not torch.cuda.is_available():
-        raise RuntimeError("the denoiser needs a HIP device (torch.cuda is unavailable)")
+        hint = ""
+        if _native._lib is not None and "pyradmc_amd.denoise" in sys.modules:
+            hint = (" - the native library was loaded before torch initialised its HIP runtime;"
+                    " import pyradmc_amd.denoise (or torch) before creating a HipEngine")
+        raise RuntimeError("the denoiser needs a HIP device (torch.cuda is unavailable)" + hint)
     return torch
 
 
@@ -381,15 +395,12 @@ def from_network_axes(a: np.ndarray) -> np.ndarray:
 
 
 def pack_input(noisy: np.ndarray, density: np.ndarray, device: str = "cuda:0"):
-    """[N][D][H][W] scaled noisy dose and density (network axes) -> padded bf16 nd
-    tensor and its channel-major copy."""
+    """[N][D][H][W] scaled noisy dose and halved density (network axes, host arrays)
+    -> padded bf16 nd tensor and its channel-major copy."""
     torch = _torch()
-    x = np.zeros(noisy.shape + (CIN_PAD,), dtype=np.float32)
-    x[..., 0] = noisy
-    x[..., 1] = density
-    t = torch.from_numpy(x).to(device=device, dtype=torch.bfloat16)
-    t_cm = t.reshape(-1, CIN_PAD).t().contiguous()
-    return t, t_cm
+    a = torch.from_numpy(np.ascontiguousarray(noisy, dtype=np.float32)).to(device)
+    d = torch.from_numpy(np.ascontiguousarray(density, dtype=np.float32)).to(device)
+    return _pack_device(a, 2.0 * d)
 
 
 def column_scale(noisy_window: np.ndarray) -> float:
@@ -477,60 +488,89 @@ def generate_pairs(n_phantoms: int, beamlets_per_phantom: int, low: int = 10_000
 # ------------------------------------------------------------------ training
 
 
-def train(model: Conv3dDenoiser, pairs: list[ColumnPair], epochs: int = 10, batch: int = 4,
-          lr: float = 1e-3, crop: int = 64, seed: int = 0, log=None) -> list[float]:
-    """Adam on random crop x crop x crop sub-windows (random x/y flips); returns the
-    mean loss of each epoch."""
+def _pack_device(noisy, density):
+    """[N][D][H][W] float32 device tensors (network axes) -> padded bf16 nd input and
+    its channel-major copy (device-side plumbing; the density channel is halved)."""
+    torch = _torch()
+    x = torch.zeros(tuple(noisy.shape) + (CIN_PAD,), dtype=torch.bfloat16, device=noisy.device)
+    x[..., 0] = noisy.to(torch.bfloat16)
+    x[..., 1] = (density * 0.5).to(torch.bfloat16)
+    return x, x.reshape(-1, CIN_PAD).t().contiguous()
+
+
+def _stack_device(pairs: list[ColumnPair], device: str):
+    torch = _torch()
+
+    def up(name):
+        return torch.from_numpy(np.ascontiguousarray(np.stack([getattr(p, name) for p in pairs]),
+                                                     dtype=np.float32)).to(device)
+
+    noisy, dens, target = up("noisy"), up("density"), up("target")
+    sig = up("target_sigma")
+    weight = 1.0 / (sig * sig + SIGMA_FLOOR * SIGMA_FLOOR)
+    del sig
+    return noisy, dens, target, weight
+
+
+def train(model: Conv3dDenoiser, pairs: list[ColumnPair], epochs: int = 10, batch: int = 8,
+          lr: float = 1e-3, lr_final: float = 3e-5, crop: int = 64, seed: int = 0, log=None) -> list[float]:
+    """Adam with cosine learning-rate decay on random crop^3 sub-windows (random
+    x/y flips); the pairs are held on the device for the whole run. Returns the mean
+    loss of each epoch."""
     torch = _torch()
     rng = np.random.default_rng(seed)
-    history = []
+    noisy, dens, target, weight = _stack_device(pairs, model.device)
     n = len(pairs)
-    depth = pairs[0].noisy.shape[2]
+    depth = noisy.shape[3]
+    steps_per_epoch = max(n // batch, 1)
+    total = epochs * steps_per_epoch
+    history = []
+    step = 0
     for epoch in range(epochs):
         order = rng.permutation(n)
-        total, count = 0.0, 0
-        for b0 in range(0, n - batch + 1, batch):
+        loss_sum = torch.zeros((), dtype=torch.float64, device=model.device)
+        for b0 in range(0, steps_per_epoch * batch, batch):
             idx = order[b0:b0 + batch]
-            xs, ds, ts, ws = [], [], [], []
+            crops = []
             for j in idx:
-                p = pairs[j]
                 z0 = int(rng.integers(0, depth - crop + 1))
-                sl = (slice(0, crop), slice(0, crop), slice(z0, z0 + crop))
-                a, dd, t, s = p.noisy[sl], p.density[sl], p.target[sl], p.target_sigma[sl]
-                if rng.random() < 0.5:
-                    a, dd, t, s = a[::-1], dd[::-1], t[::-1], s[::-1]
-                if rng.random() < 0.5:
-                    a, dd, t, s = a[:, ::-1], dd[:, ::-1], t[:, ::-1], s[:, ::-1]
-                xs.append(a)
-                ds.append(dd)
-                ts.append(t)
-                ws.append(1.0 / (s * s + SIGMA_FLOOR * SIGMA_FLOOR))
-            x, x_cm = pack_input(to_network_axes(np.stack(xs)), to_network_axes(np.stack(ds)) * 0.5,
-                                 model.device)
-            target = torch.from_numpy(to_network_axes(np.stack(ts).astype(np.float32))).to(model.device)
-            weight = to_network_axes(np.stack(ws).astype(np.float32))
-            weight /= weight.mean()
-            weight = torch.from_numpy(np.ascontiguousarray(weight)).to(model.device)
+                fx, fy = rng.random() < 0.5, rng.random() < 0.5
+                parts = []
+                for arr in (noisy, dens, target, weight):
+                    c = arr[int(j), :crop, :crop, z0:z0 + crop]
+                    if fx:
+                        c = torch.flip(c, (0,))
+                    if fy:
+                        c = torch.flip(c, (1,))
+                    parts.append(c)
+                crops.append(parts)
+            # [N][x][y][z] -> network axes [N][z][x][y]
+            a, d, t, w = (torch.stack([c[k] for c in crops]).permute(0, 3, 1, 2).contiguous() for k in range(4))
+            w = w / w.mean()
+            x, x_cm = _pack_device(a, d)
             out, cache = model.forward(x, x_cm, keep=True)
-            partial, dout, dout_cm = model.loss_and_grad(out, x, target, weight, 1.0 / target.numel())
+            partial, dout, dout_cm = model.loss_and_grad(out, x, t, w, 1.0 / t.numel())
             model.backward(cache, dout, dout_cm)
-            model.adam_step(lr)
-            total += float(partial.sum().item()) / target.numel()
-            count += 1
-        history.append(total / max(count, 1))
+            rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * step / max(total, 1)))
+            model.adam_step(rate)
+            loss_sum += partial.sum().double() / t.numel()
+            step += 1
+        history.append(float(loss_sum.item()) / steps_per_epoch)
         if log:
-            log(f"epoch {epoch}: loss {history[-1]:.6g}")
+            log(f"epoch {epoch}: loss {history[-1]:.6g} lr {rate:.3g}")
     return history
 
 
 def predict(model: Conv3dDenoiser, noisy: np.ndarray, density: np.ndarray) -> np.ndarray:
     """Denoised window(s), same scaling as the input: [N][64][64][depth] or a single
     [64][64][depth] window."""
+    torch = _torch()
     single = noisy.ndim == 3
     a = noisy[None] if single else noisy
     dd = density[None] if single else density
-    x, _ = pack_input(to_network_axes(a.astype(np.float32)), to_network_axes(dd.astype(np.float32)) * 0.5,
-                      model.device)
+    at = torch.from_numpy(to_network_axes(np.asarray(a, dtype=np.float32))).to(model.device)
+    dt = torch.from_numpy(to_network_axes(np.asarray(dd, dtype=np.float32))).to(model.device)
+    x, _ = _pack_device(at, dt)
     pred = from_network_axes(model.predict_tensor(x).cpu().numpy())
     return pred[0] if single else pred
 
@@ -593,84 +633,139 @@ class DenoisedDijBuilder:
         self.fallback = int(fallback_histories)
         self.n_batches = int(n_batches)
 
-    def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 8):
+    def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 8,
+              group_size: int | None = None):
+        """(DijResult, DenoiseReport) for the source's beamlets. Columns go through the
+        engine in groups; each group's low-history means stay on the device, its
+        windows are gathered straight into the network's input buffers, and the
+        denoised means and sigmas are written back for the engine's own truncation
+        and CSC compaction."""
         import time
 
+        from pyradmc_amd import SpectralBeamletSource
         from pyradmc_amd.scoring import DijResult
 
+        torch = _torch()
         t0 = time.perf_counter()
-        eng = self.engine
+        eng, model = self.engine, self.model
         grid = eng.grid
         shape = tuple(grid.shape)
-        a = eng.run_dij(source, self.low, self.n_batches, seed=seed, truncation=0.0)
-        b = eng.run_dij(source, self.low, self.n_batches, seed=seed + 1, truncation=0.0)
-        t1 = time.perf_counter()
-        dens = np.asarray(grid.density, dtype=np.float32)
+        depth = shape[2]
         n_cols = int(source.n_beamlets)
         windows = _windows_for(grid, source, range(n_cols))
-        cols = []
+        origins = np.ascontiguousarray([[wx.start, wy.start] for wx, wy in windows], dtype=np.int32)
+        scoring = eng._default_scoring()
+        eng._prepare(source)  # default cuts and physics, as run_dij's defaults
+        if group_size is None:
+            free = int(_native.device_info(eng._index)["free_bytes"])
+            per_col = 8 * scoring.n_voxels * (self.n_batches + 3)
+            group_size = max(1, min(n_cols, int(0.25 * free) // max(per_col, 1)))
+        dens = torch.from_numpy(np.ascontiguousarray(np.transpose(np.asarray(grid.density, np.float32),
+                                                                  (2, 0, 1)))).to(model.device)  # [z][x][y]
+        counts = np.full(n_cols, self.low, dtype=np.int64)
+        books: dict = {}
+        deposited_quanta = 0  # dose-to-medium: the fold's quanta (the ledger books only escapes)
         disagreement = np.zeros(n_cols)
-        for c0 in range(0, n_cols, batch):
-            idx = list(range(c0, min(c0 + batch, n_cols)))
-            ins, scales, dd = [], [], []
-            for j in idx:
-                wx, wy = windows[j]
-                for run in (a, b):
-                    win = run.column_dense(j)[wx, wy, :].astype(np.float32)
-                    s = column_scale(win)
-                    ins.append(win / s)
-                    scales.append(s)
-                    dd.append(dens[wx, wy, :])
-            pred = predict(self.model, np.stack(ins), np.stack(dd))
-            for k, j in enumerate(idx):
-                pa = np.maximum(pred[2 * k], 0.0) * scales[2 * k]
-                pb = np.maximum(pred[2 * k + 1], 0.0) * scales[2 * k + 1]
-                mean = 0.5 * (pa + pb)
-                # rms(pa - mean) = rms(pa - pb) / 2: the error of the average, were the
-                # two estimates' errors independent
-                disagreement[j] = dose_weighted_rms(pa, mean) if mean.max() > 0 else 0.0
-                full = 0.5 * (a.column_dense(j) + b.column_dense(j))
-                sig = 0.5 * np.sqrt(a.sigma_dense(j) ** 2 + b.sigma_dense(j) ** 2)
-                wx, wy = windows[j]
-                full[wx, wy, :] = mean
-                sig[wx, wy, :] = 0.5 * np.abs(pa - pb)
-                cols.append((full.ravel(), sig.ravel()))
-        t2 = time.perf_counter()
+        col_counts, col_idx, col_dose, col_sigma = [], [], [], []
+        seconds = {"low_mc": 0.0, "denoise": 0.0, "fallback_mc": 0.0, "assemble": 0.0}
+        for g0 in range(0, n_cols, group_size):
+            group = range(g0, min(g0 + group_size, n_cols))
+            ng = len(group)
+            org = np.ascontiguousarray(origins[g0:group.stop])
+            org_p = _native.ptr(org, ctypes.c_int32)
+            wins = torch.empty((2, ng, depth, WINDOW_XY, WINDOW_XY), dtype=torch.float32, device=model.device)
+            ta = time.perf_counter()
+            for k, s_run in enumerate((seed, seed + 1)):
+                _, _, quanta = eng._dij_group(source, group, counts, self.n_batches, s_run, True, scoring, books)
+                deposited_quanta += quanta
+                torch.cuda.synchronize()
+                eng._ctx.call("hmc_gather_windows", org_p, ng, WINDOW_XY, WINDOW_XY, wins[k].data_ptr())
+            tb = time.perf_counter()
+            flat = wins.reshape(2, ng, -1)
+            scale = torch.topk(flat, 64, dim=2).values.mean(dim=2).clamp_min(1e-30)  # [2][ng]
+            noisy = (wins / scale[:, :, None, None, None]).reshape(2 * ng, depth, WINDOW_XY, WINDOW_XY)
+            dwin = torch.stack([dens[:, windows[j][0], windows[j][1]] for j in group])  # [ng][z][x][y]
+            dwin2 = torch.cat([dwin, dwin], dim=0)
+            preds = []
+            for b0 in range(0, 2 * ng, batch):
+                x, _ = _pack_device(noisy[b0:b0 + batch].contiguous(), dwin2[b0:b0 + batch].contiguous())
+                preds.append(model.predict_tensor(x))
+            pred = torch.cat(preds).reshape(2, ng, depth, WINDOW_XY, WINDOW_XY).clamp_min(0.0)
+            pred = pred * scale[:, :, None, None, None]
+            mean = 0.5 * (pred[0] + pred[1])
+            sigma = 0.5 * (pred[0] - pred[1]).abs()
+            # per column: dose-weighted RMS of (estimate - mean) over mean > 0.1 max, / max
+            m = mean.reshape(ng, -1)
+            d = (pred[0] - mean).reshape(ng, -1)
+            mmax = m.max(dim=1).values.clamp_min(1e-30)
+            sel = (m > 0.1 * mmax[:, None]).to(m.dtype) * m
+            rms = torch.sqrt((sel * d * d).sum(dim=1) / sel.sum(dim=1).clamp_min(1e-30)) / mmax
+            disagreement[g0:group.stop] = rms.double().cpu().numpy()
+            torch.cuda.synchronize()
+            tc = time.perf_counter()
+            eng._ctx.call("hmc_scatter_windows", org_p, ng, WINDOW_XY, WINDOW_XY, mean.contiguous().data_ptr(),
+                          sigma.contiguous().data_ptr())
+            kept = np.zeros(ng, dtype=np.int64)
+            total = _native.c_int64()
+            eng._ctx.call("hmc_truncate", float(truncation), _native.ptr(kept, _native.c_int64),
+                          ctypes.byref(total))
+            mtot = int(total.value)
+            idx = np.empty(mtot, dtype=np.int64)
+            dose = np.empty(mtot, dtype=np.float64)
+            sig = np.empty(mtot, dtype=np.float64)
+            if mtot:
+                eng._ctx.call("hmc_fetch_csc", _native.ptr(idx, _native.c_int64), _native.ptr(dose, ctypes.c_double),
+                              _native.ptr(sig, ctypes.c_double))
+            col_counts.append(kept)
+            col_idx.append(idx)
+            col_dose.append(dose)
+            col_sigma.append(sig)
+            td = time.perf_counter()
+            seconds["low_mc"] += tb - ta
+            seconds["denoise"] += tc - tb
+            seconds["assemble"] += td - tc
+        kept_all = np.concatenate(col_counts)
+        indptr = np.zeros(n_cols + 1, dtype=np.int64)
+        np.cumsum(kept_all, out=indptr[1:])
+        indices, dose, sigma = np.concatenate(col_idx), np.concatenate(col_dose), np.concatenate(col_sigma)
         fallback = disagreement > self.tolerance
+        te = time.perf_counter()
         if fallback.any():
-            from pyradmc_amd import SpectralBeamletSource
-
             redo = np.flatnonzero(fallback).tolist()
             sub = SpectralBeamletSource(source.spectrum, source.focal_point, [source.centers[j] for j in redo],
                                         source.width_u, source.width_v, source.u_axis, source.v_axis)
-            mc = eng.run_dij(sub, self.fallback, self.n_batches, seed=seed + 2, truncation=0.0)
-            for k, j in enumerate(redo):
-                cols[j] = (mc.column_dense(k).ravel(), mc.sigma_dense(k).ravel())
-        t3 = time.perf_counter()
-        # truncation and CSC, as the engine's device compaction does
-        indptr = np.zeros(n_cols + 1, dtype=np.int64)
-        idx_l, dose_l, sig_l = [], [], []
-        for j, (d, s) in enumerate(cols):
-            cmax = float(d.max(initial=0.0))
-            keep = np.flatnonzero((d > 0.0) & (d >= truncation * cmax))
-            idx_l.append(keep.astype(np.int64))
-            dose_l.append(d[keep])
-            sig_l.append(s[keep])
-            indptr[j + 1] = indptr[j] + keep.size
-        t4 = time.perf_counter()
+            mc = eng.run_dij(sub, self.fallback, self.n_batches, seed=seed + 2, truncation=truncation)
+            parts_i, parts_d, parts_s, new_counts = [], [], [], kept_all.copy()
+            slot = {j: k for k, j in enumerate(redo)}
+            for j in range(n_cols):
+                if j in slot:
+                    k = slot[j]
+                    lo, hi = int(mc.indptr[k]), int(mc.indptr[k + 1])
+                    parts_i.append(mc.indices[lo:hi]); parts_d.append(mc.dose[lo:hi]); parts_s.append(mc.sigma[lo:hi])
+                    new_counts[j] = hi - lo
+                else:
+                    lo, hi = int(indptr[j]), int(indptr[j + 1])
+                    parts_i.append(indices[lo:hi]); parts_d.append(dose[lo:hi]); parts_s.append(sigma[lo:hi])
+            indices, dose, sigma = np.concatenate(parts_i), np.concatenate(parts_d), np.concatenate(parts_s)
+            np.cumsum(new_counts, out=indptr[1:])
+            self._merge_books(books, mc)
+        seconds["fallback_mc"] = time.perf_counter() - te
+        seconds["total"] = time.perf_counter() - t0
+        q = 1.0e-9  # ledger quanta -> MeV
         result = DijResult(
             grid_shape=shape, n_beamlets=n_cols, n_histories_per_beamlet=2 * self.low, n_batches=self.n_batches,
-            truncation=float(truncation), indptr=indptr, indices=np.concatenate(idx_l),
-            dose=np.concatenate(dose_l), sigma=np.concatenate(sig_l),
-            energy_emitted=a.energy_emitted + b.energy_emitted,
-            energy_deposited=a.energy_deposited + b.energy_deposited,
-            energy_escaped=a.energy_escaped + b.energy_escaped,
-            energy_unscored=a.energy_unscored + b.energy_unscored,
-            correlated=True, scoring_mode="dose_to_medium", provenance=a.provenance, counters={})
-        report = DenoiseReport(disagreement=disagreement, fallback=fallback,
-                               seconds={"low_mc": t1 - t0, "denoise": t2 - t1, "fallback_mc": t3 - t2,
-                                        "assemble": t4 - t3, "total": t4 - t0})
-        return result, report
+            truncation=float(truncation), indptr=indptr, indices=indices, dose=dose, sigma=sigma,
+            energy_emitted=float(books.get("emitted", 0)) * q,
+            energy_deposited=float(deposited_quanta + books.get("fallback_deposited", 0)) * q,
+            energy_escaped=float(books.get("escaped", 0)) * q, energy_unscored=float(books.get("unscored", 0)) * q,
+            correlated=True, scoring_mode="dose_to_medium", provenance=None, counters=books)
+        return result, DenoiseReport(disagreement=disagreement, fallback=fallback, seconds=seconds)
+
+    @staticmethod
+    def _merge_books(books: dict, dij) -> None:
+        for name in ("emitted", "escaped", "unscored"):
+            books[name] = books.get(name, 0) + int(round(getattr(dij, "energy_" + name) / 1.0e-9))
+        books["fallback_deposited"] = books.get("fallback_deposited", 0) + int(round(dij.energy_deposited / 1.0e-9))
 
 
 def _main(argv=None) -> int:
@@ -690,7 +785,7 @@ def _main(argv=None) -> int:
     tr.add_argument("--low", type=int, default=10_000)
     tr.add_argument("--high", type=int, default=1_000_000)
     tr.add_argument("--epochs", type=int, default=10)
-    tr.add_argument("--batch", type=int, default=4)
+    tr.add_argument("--batch", type=int, default=8)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--seed", type=int, default=20260801)
     args = ap.parse_args(argv)

@@ -40,8 +40,8 @@ from pyradmc_amd.data.gs import build_gs_grid, gs_window_from_tables
 from pyradmc_amd.progress import ProgressEmitter
 from pyradmc_amd.scoring import DijResult, RunProvenance, ScoringGrid, TransportResult
 
-_CHUNK_HISTORIES_PER_CU = 16384
-_CHUNK_SIZE_CAP = 8 * 1024 * 1024
+_CHUNK_HISTORIES_PER_CU = 65536
+_CHUNK_SIZE_CAP = 32 * 1024 * 1024
 _CHUNK_MEMORY_FRACTION = 0.25
 _QUEUE_BYTES_PER_SLOT = 4 * 48
 _GROUP_MEMORY_FRACTION = 0.35
@@ -115,6 +115,8 @@ class HipEngine:
     _grid_uploaded: bool = field(default=False, init=False, repr=False)
     _uploaded_tables: object = field(default=None, init=False, repr=False)
     _info: dict | None = field(default=None, init=False, repr=False)
+    _scoring_default: object = field(default=None, init=False, repr=False)
+    _scoring_last: object = field(default=None, init=False, repr=False)
 
     def __post_init__(self) -> None:
         if self.queue_factor < 1:
@@ -158,7 +160,8 @@ class HipEngine:
         if self.chunk_size is not None:
             return int(min(self.chunk_size, max(total, 1)))
         target = min(_CHUNK_HISTORIES_PER_CU * max(self._info["cu_count"], 1), _CHUNK_SIZE_CAP)
-        free = int(self._info.get("free_bytes", 0))
+        # Queue memory is budgeted against what is free now (other engines may be alive).
+        free = int(nat.device_info(self._index).get("free_bytes", 0))
         if free > 0:
             target = min(target, int(free * _CHUNK_MEMORY_FRACTION) // (self.queue_factor * _QUEUE_BYTES_PER_SLOT))
         return int(max(1, min(target, max(total, 1))))
@@ -236,7 +239,19 @@ class HipEngine:
         fit = int(free * _SLOT_MEMORY_FRACTION) // per_slot if free > 0 else 1
         return max(1, min(n_batches, fit))
 
+    def _default_scoring(self) -> ScoringGrid:
+        if self._scoring_default is None:
+            self._scoring_default = ScoringGrid.for_grid(self.grid)
+        return self._scoring_default
+
     def _set_scoring(self, scoring, n_columns: int, n_slots: int = 1) -> None:
+        # Same grid object and layout as the previous call: only clear the maps (the
+        # voxel masses and buffers on the device are already right). Grids and scoring
+        # grids are treated as immutable once handed to the engine.
+        key = (n_columns, n_slots)
+        if self._scoring_last is not None and self._scoring_last[0] is scoring and self._scoring_last[1] == key:
+            self._ctx.call("hmc_reset_maps")
+            return
         mass = np.ascontiguousarray(np.asarray(scoring.voxel_mass, dtype=np.float64))
         origin = np.asarray(scoring.origin, dtype=np.float64)
         spacing = np.asarray(scoring.spacing, dtype=np.float64)
@@ -245,6 +260,7 @@ class HipEngine:
             nat.ptr(origin, nat.c_double), nat.ptr(spacing, nat.c_double),
             nat.ptr(mass, nat.c_double), int(n_columns), int(n_slots),
         )
+        self._scoring_last = (scoring, key)
 
     def _set_physics(self, pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w) -> None:
         p = nat.Physics()
@@ -439,7 +455,7 @@ class HipEngine:
             raise ValueError(f"deposit resolution must be positive, got {deposit_resolution_cm}")
         resolution = 0.0 if deposit_resolution_cm is None else float(deposit_resolution_cm)
 
-        scoring = scoring_grid if scoring_grid is not None else ScoringGrid.for_grid(self.grid)
+        scoring = scoring_grid if scoring_grid is not None else self._default_scoring()
         with self._lock:
             self._upload_grid()
         self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
@@ -492,6 +508,50 @@ class HipEngine:
         )
 
     # ----------------------------------------------------------------- run_dij
+
+    def _prepare(self, source, pcut: float = PCUT_MEV, ecut: float = ECUT_MEV, step: float | None = None,
+                 resolution: float = 0.0, transport_electrons: bool = True, msc_gs: int = 1,
+                 d2w: bool = False) -> None:
+        """Grid, cross-section tables (and GS grid) and physics on the device for a
+        source: what every transport needs before its first chunk."""
+        if step is None:
+            step = default_step_energy_fraction("gs" if msc_gs else "gaussian")
+        with self._lock:
+            self._upload_grid()
+        self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
+        self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
+
+    def _dij_group(self, source, group: range, counts: np.ndarray, n_batches: int, seed: int,
+                   correlated: bool, scoring, books: dict) -> tuple[bool, float, int]:
+        """Transport, fold and finalize one group of beamlet columns; the means and
+        sigmas stay on the device (for truncation, or for the denoiser's windows).
+        Returns (analytic emitted energy?, mono energy, deposited quanta)."""
+        per_batch = counts // n_batches
+        offsets = np.concatenate(([0], np.cumsum(counts)[:-1]))
+        g0 = group.start
+        n_cols = len(group)
+        analytic, mono_energy = self._set_source(source, "photon", beamlets=group)
+        slots = self._batch_slots(n_batches, n_cols, scoring.n_voxels)
+        self._set_scoring(scoring, n_cols, slots)
+        chunk = self._chunk(int(per_batch[g0:group.stop].sum()) * slots)
+        deposited = 0
+        for b0 in range(0, n_batches, slots):
+            block = range(b0, min(b0 + slots, n_batches))
+            items = []
+            for b in block:
+                for j in group:
+                    r0 = b * int(per_batch[j])
+                    key = r0 if correlated else int(offsets[j]) + r0
+                    items.append((key, int(per_batch[j]), (b - b0) * n_cols + (j - g0), j - g0))
+            if not _in_kernel(source):
+                self._upload_primaries([source.sample_beamlet_batch(seed, key, n, g0 + local)
+                                        for key, n, _, local in items])
+            ledger = self._transport(seed, items, chunk)
+            self._merge(books, ledger)
+            nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
+            deposited += self._fold(len(block), nhist)
+        self._ctx.call("hmc_finalize", int(n_batches), None, None)
+        return analytic, mono_energy, deposited
 
     def run_dij(
         self,
@@ -557,20 +617,15 @@ class HipEngine:
             raise ValueError(f"deposit resolution must be positive, got {deposit_resolution_cm}")
         resolution = 0.0 if deposit_resolution_cm is None else float(deposit_resolution_cm)
 
-        scoring = scoring_grid if scoring_grid is not None else ScoringGrid.for_grid(self.grid)
+        scoring = scoring_grid if scoring_grid is not None else self._default_scoring()
         n_vox = scoring.n_voxels
-        per_batch = counts // n_batches
-        offsets = np.concatenate(([0], np.cumsum(counts)[:-1]))
         if beamlet_group_size is None:
             # float64 sums plus one int64 map per group column; batch slots come after.
             free = int(self._info.get("free_bytes", 0))
             fit = int(free * _GROUP_MEMORY_FRACTION) // (24 * n_vox) if free > 0 else 128
             beamlet_group_size = max(1, min(fit, n_beamlets, 4096))
 
-        with self._lock:
-            self._upload_grid()
-        self._upload_tables(float(source.max_energy), pcut, ecut, with_gs=msc_gs == 1)
-        self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
+        self._prepare(source, pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
 
         emitter = ProgressEmitter(progress, int(counts.sum()))
         books: dict = {}
@@ -579,28 +634,10 @@ class HipEngine:
         analytic, mono_energy = False, 0.0
         for g0 in range(0, n_beamlets, beamlet_group_size):
             group = range(g0, min(g0 + beamlet_group_size, n_beamlets))
-            n_cols = len(group)
-            analytic, mono_energy = self._set_source(source, "photon", beamlets=group)
-            slots = self._batch_slots(n_batches, n_cols, n_vox)
-            self._set_scoring(scoring, n_cols, slots)
             group_hist = int(counts[g0:group.stop].sum())
-            chunk = self._chunk(int(per_batch[g0:group.stop].sum()) * slots)
-            for b0 in range(0, n_batches, slots):
-                block = range(b0, min(b0 + slots, n_batches))
-                items = []
-                for b in block:
-                    for j in group:
-                        r0 = b * int(per_batch[j])
-                        key = r0 if correlated else int(offsets[j]) + r0
-                        items.append((key, int(per_batch[j]), (b - b0) * n_cols + (j - g0), j - g0))
-                if not _in_kernel(source):
-                    self._upload_primaries([source.sample_beamlet_batch(seed, key, n, g0 + local)
-                                            for key, n, _, local in items])
-                ledger = self._transport(seed, items, chunk)
-                self._merge(books, ledger)
-                nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
-                deposited_quanta += self._fold(len(block), nhist)
-            self._ctx.call("hmc_finalize", int(n_batches), None, None)
+            analytic, mono_energy, q = self._dij_group(source, group, counts, n_batches, seed,
+                                                      correlated, scoring, books)
+            deposited_quanta += q
             kept = np.zeros(len(group), dtype=np.int64)
             total = nat.c_int64()
             self._ctx.call("hmc_truncate", float(truncation), nat.ptr(kept, nat.c_int64), nat.ctypes.byref(total))

@@ -156,6 +156,8 @@ class DijResult:
         return out.reshape(self.grid_shape)
 
     def dose_for_weights(self, weights: np.ndarray) -> np.ndarray:
+        """Host form of the container's product, as the reference's
+        (pyradmc/scoring/dij.py:251-260). For many products use ``to_device()``."""
         w = np.asarray(weights, dtype=np.float64)
         if w.shape != (self.n_beamlets,):
             raise ValueError(f"weights shape {w.shape} != ({self.n_beamlets},)")
@@ -164,6 +166,13 @@ class DijResult:
             lo, hi = int(self.indptr[j]), int(self.indptr[j + 1])
             np.add.at(out, self.indices[lo:hi], w[j] * self.dose[lo:hi])
         return out.reshape(self.grid_shape)
+
+    def to_device(self, device: int = 0):
+        """This matrix on a HIP device (``pyradmc_amd.spmm.DeviceDij``), for products
+        with many weight vectors."""
+        from pyradmc_amd.spmm import DeviceDij
+
+        return DeviceDij(self, device)
 
     @staticmethod
     def _unit_factor(unit: str) -> float:

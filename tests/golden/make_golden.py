@@ -4,13 +4,15 @@ Run only in the build container, where the reference checkout is present at
 /root/reference. It imports the reference package (pure Python + NumPy) and records
 input/output vectors for the functions that closed forms and invariants alone cannot
 pin to rounding (the flattened cross-section tables, individual Goudsmit-Saunderson
-nodes, and end-to-end ReferenceEngine doses). The fixtures are data: inputs and the
+nodes, end-to-end ReferenceEngine doses, and the summation order of
+DijResult.dose_for_weights). The fixtures are data: inputs and the
 reference's outputs, nothing of its source.
 
 Usage::
 
     HOME=<scratch> python tests/golden/make_golden.py tables gs_nodes
     HOME=<scratch> python tests/golden/make_golden.py run_c1_kerma run_c1_cm ...
+    HOME=<scratch> python tests/golden/make_golden.py dose_for_weights
 
 HOME is redirected by the caller because the reference persists its GS grid under
 ``~/.cache``.
@@ -242,6 +244,48 @@ def _dij_fan() -> None:
     np.savez_compressed(OUT / "ref_dij_fan.npz", **out)
 
 
+def _dose_for_weights() -> None:
+    """DijResult.dose_for_weights on (a) the fan-beamlet Dij the reference engine wrote
+    (ref_dij_fan.npz) and (b) a synthetic CSC whose float32 entries and weights span
+    sixteen decades, with repeated (voxel, beamlet) entries, so that a sum in any order
+    other than the reference's (ascending beamlet, CSC order inside a column, product
+    rounded before the add) shows in the low bits."""
+    from pyradmc.scoring.dij import DijResult
+
+    def product(shape, indptr, indices, dose, weights):
+        d = DijResult(grid_shape=shape, n_beamlets=len(indptr) - 1, n_histories_per_beamlet=1,
+                      n_batches=1, truncation=0.0, indptr=indptr, indices=indices, dose=dose,
+                      sigma=np.zeros_like(dose), energy_emitted=0.0, energy_deposited=0.0,
+                      energy_escaped=0.0)
+        return np.stack([d.dose_for_weights(w).ravel() for w in weights], axis=1)
+
+    out = {}
+    fan = np.load(OUT / "ref_dij_fan.npz")
+    rng = np.random.default_rng(20261201)
+    w_fan = rng.random((5, len(fan["indptr"]) - 1))
+    out["fan_weights"] = w_fan
+    out["fan_dose"] = product((16, 16, 24), fan["indptr"], fan["indices"], fan["dose"], w_fan)
+
+    n_rows, n_cols = 203, 37
+    indptr, indices = [0], []
+    for _ in range(n_cols):
+        k = int(rng.integers(0, 60))
+        rows = np.sort(rng.choice(n_rows, size=k, replace=False))
+        rows = np.concatenate([rows, rng.choice(rows, size=min(3, k), replace=False)]) if k else rows
+        indices.append(rng.permutation(rows))
+        indptr.append(indptr[-1] + len(rows))
+    indices = np.concatenate(indices).astype(np.int64)
+    indptr = np.asarray(indptr, dtype=np.int64)
+    dose = (rng.random(len(indices)) * 10.0 ** rng.integers(-8, 8, len(indices))).astype(np.float32)
+    w_syn = rng.random((4, n_cols)) * 10.0 ** rng.integers(-8, 8, (4, n_cols))
+    w_syn[1] *= np.where(rng.random(n_cols) < 0.3, -1.0, 1.0)
+    out.update(syn_indptr=indptr, syn_indices=indices, syn_dose=dose, syn_weights=w_syn,
+               syn_n_rows=np.int64(n_rows),
+               syn_out=product((n_rows, 1, 1), indptr, indices, dose, w_syn))
+    np.savez_compressed(OUT / "ref_dose_for_weights.npz", **out)
+    print("dose_for_weights: fan", out["fan_dose"].shape, "synthetic", out["syn_out"].shape)
+
+
 def _spectral_parallel_source():
     """Parallel 10x10 cm field of 6 MV-like spectral photons: the reference's sanctioned
     extension point (a Source subclass); 4 uniforms: energy (2), then x, then y."""
@@ -340,6 +384,7 @@ TASKS = {
     "run_c1_mm": lambda: _run_c1(0.1, (0.8, 0.8, -0.1), True, "c1_mm"),
     "dij_small": _dij_small,
     "dij_fan": _dij_fan,
+    "dose_for_weights": _dose_for_weights,
 }
 
 

@@ -240,3 +240,83 @@ def test_network_gradients_match_oracle(dn):
         cos = float(np.sum(got * ref) / (np.linalg.norm(got) * nr))
         err = float(np.linalg.norm(got - ref) / nr)
         assert cos > 0.99 and err < 0.15, (name, cos, err)
+
+
+def _small_ct(dn):
+    from pyradmc_amd import SpectralBeamletSource, VoxelGrid, bremsstrahlung_6mv_like
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    rng = np.random.default_rng(12)
+    shape = (72, 70, 24)
+    density = np.where(rng.random(shape) < 0.1, 0.3, 1.0)
+    grid = VoxelGrid(shape=shape, spacing=(0.25, 0.25, 0.25), density=density,
+                     material=np.zeros(shape, dtype=np.int32))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    centers = [(8.0, 8.0, 3.0), (9.5, 8.5, 3.0), (7.0, 9.0, 3.0)]
+    src = SpectralBeamletSource(bremsstrahlung_6mv_like(), (8.5, 8.5, -97.0), centers, 0.5, 0.5)
+    return grid, xs, src
+
+
+def test_window_gather_scatter_round_trip(dn):
+    import ctypes
+
+    import torch
+
+    from pyradmc_amd import HipEngine, _native
+
+    grid, xs, src = _small_ct(dn)
+    eng = HipEngine(grid, xs)
+    ref = eng.run_dij(src, 20_000, 4, seed=3, truncation=0.0)
+    windows = dn._windows_for(grid, src, range(3))
+    origins = np.ascontiguousarray([[wx.start, wy.start] for wx, wy in windows], dtype=np.int32)
+    counts = np.full(3, 20_000, dtype=np.int64)
+    books = {}
+    eng._dij_group(src, range(3), counts, 4, 3, True, eng._default_scoring(), books)
+    out = torch.empty((3, 24, 64, 64), dtype=torch.float32, device="cuda:0")
+    eng._ctx.call("hmc_gather_windows", _native.ptr(origins, ctypes.c_int32), 3, 64, 64, out.data_ptr())
+    got = out.cpu().numpy()
+    for j, (wx, wy) in enumerate(windows):
+        expect = np.transpose(ref.column_dense(j)[wx, wy, :], (2, 0, 1)).astype(np.float32)
+        np.testing.assert_array_equal(got[j], expect)  # same seed, same device arithmetic
+    # scatter the windows back (sigma = the mean, to see it land in the sigma output too)
+    eng._ctx.call("hmc_scatter_windows", _native.ptr(origins, ctypes.c_int32), 3, 64, 64, out.data_ptr(),
+                  out.data_ptr())
+    kept = np.zeros(3, dtype=np.int64)
+    total = _native.c_int64()
+    eng._ctx.call("hmc_truncate", 0.0, _native.ptr(kept, _native.c_int64), ctypes.byref(total))
+    n = int(total.value)
+    idx, dose, sig = np.empty(n, np.int64), np.empty(n), np.empty(n)
+    eng._ctx.call("hmc_fetch_csc", _native.ptr(idx, _native.c_int64), _native.ptr(dose, ctypes.c_double),
+                  _native.ptr(sig, ctypes.c_double))
+    eng.close()
+    lo = 0
+    for j, (wx, wy) in enumerate(windows):
+        dense = np.zeros(grid.shape)
+        dense.reshape(-1)[idx[lo:lo + kept[j]]] = dose[lo:lo + kept[j]]
+        lo += int(kept[j])
+        expect = np.zeros(grid.shape)
+        expect[wx, wy, :] = ref.column_dense(j)[wx, wy, :].astype(np.float32)
+        np.testing.assert_array_equal(dense, expect)
+    np.testing.assert_array_equal(sig, dose)
+
+
+def test_denoised_builder_falls_back_to_plain_monte_carlo(dn):
+    # An untrained network and a zero error budget: every column is re-simulated, and
+    # the result is the plain Monte Carlo Dij bit for bit.
+    from pyradmc_amd import HipEngine
+
+    grid, xs, src = _small_ct(dn)
+    eng = HipEngine(grid, xs)
+    model = dn.Conv3dDenoiser(seed=5)
+    builder = dn.DenoisedDijBuilder(eng, model, low_histories=4_000, tolerance=0.0,
+                                    fallback_histories=40_000, n_batches=4)
+    dij, report = builder.build(src, seed=21, truncation=1e-4)
+    plain = eng.run_dij(src, 40_000, 4, seed=23, truncation=1e-4)
+    eng.close()
+    assert report.fallback.all()
+    np.testing.assert_array_equal(dij.indptr, plain.indptr)
+    np.testing.assert_array_equal(dij.indices, plain.indices)
+    np.testing.assert_array_equal(dij.dose, plain.dose)
+    np.testing.assert_array_equal(dij.sigma, plain.sigma)
+    assert abs(dij.energy_emitted - dij.energy_deposited - dij.energy_escaped - dij.energy_unscored) \
+        < 1e-6 * dij.energy_emitted

@@ -250,3 +250,45 @@ def test_oracle_fan_dij_matches_reference(golden_dir, oracle_lib):
                             rep["books"]["deposited"], rep["books"]["emitted"]))
     assert_within_replicates(summary(g["indptr"], g["indices"], g["dose"], g["sigma"],
                                      float(g["energy_deposited"]), float(g["energy_emitted"])), reps)
+
+
+# ------------------------------------------------------------ planning product
+
+
+def test_dose_for_weights_matches_reference_bitwise(golden_dir):
+    """The oracle's dose_for_weights equals the reference's DijResult.dose_for_weights
+    bit for bit: on the engine-written fan Dij and on a synthetic CSC with entries and
+    weights over sixteen decades and repeated (voxel, beamlet) entries."""
+    from oracle import mc
+
+    g = np.load(golden_dir / "ref_dose_for_weights.npz")
+    fan = np.load(golden_dir / "ref_dij_fan.npz")
+    got = mc.dose_for_weights(fan["indptr"], fan["indices"], fan["dose"], g["fan_weights"].T, 16 * 16 * 24)
+    assert np.array_equal(got, g["fan_dose"])
+    got = mc.dose_for_weights(g["syn_indptr"], g["syn_indices"], g["syn_dose"], g["syn_weights"].T,
+                              int(g["syn_n_rows"]))
+    assert np.array_equal(got, g["syn_out"])
+    one = mc.dose_for_weights(g["syn_indptr"], g["syn_indices"], g["syn_dose"], g["syn_weights"][2],
+                              int(g["syn_n_rows"]))
+    assert np.array_equal(one, g["syn_out"][:, 2])
+
+
+def test_dose_for_weights_golden_pins_the_summation_order(golden_dir):
+    """The synthetic fixture tells summation orders apart: the same sums in descending
+    beamlet order, or through a float64 dense product, miss it in the low bits while
+    agreeing to rounding."""
+    g = np.load(golden_dir / "ref_dose_for_weights.npz")
+    indptr, indices, dose, w = g["syn_indptr"], g["syn_indices"], g["syn_dose"], g["syn_weights"].T
+    n = int(g["syn_n_rows"])
+    reversed_order = np.zeros((n, w.shape[1]))
+    for j in reversed(range(len(indptr) - 1)):
+        lo, hi = indptr[j], indptr[j + 1]
+        np.add.at(reversed_order, indices[lo:hi], dose[lo:hi, None].astype(np.float64) * w[j][None, :])
+    dense = np.zeros((n, len(indptr) - 1))
+    for j in range(len(indptr) - 1):
+        np.add.at(dense[:, j], indices[indptr[j]:indptr[j + 1]], dose[indptr[j]:indptr[j + 1]])
+    matmul = dense @ w
+    for other in (reversed_order, matmul):
+        assert not np.array_equal(other, g["syn_out"])
+        scale = np.abs(dense) @ np.abs(w)
+        assert np.all(np.abs(other - g["syn_out"]) <= 1e-13 * scale + 1e-300)
