@@ -157,6 +157,91 @@ def _denoise_leg(model_path: Path, low: int, target: float) -> dict:
     }
 
 
+def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[dict, object]:
+    """The full plan: 256x256x128 CT-like phantom at 2 mm, five gantry beams of 15x15
+    spectral beamlets (1125 columns). A probe Dij at ``probe_n`` histories per beamlet
+    gives each beamlet's sigma; sigma ~ N^-1/2 then sets the histories that bring the
+    90th-percentile beamlet to ``target``, and that plain Dij is run and timed wall to
+    wall (five run_dij calls, float32 CSC on the host). Then the planning product for
+    ``n_vectors`` seeded U[0,1) weight vectors on the device, float64 (bit-exact with
+    dose_for_weights) and float32, timed with HIP events around the kernel."""
+    from pyradmc_amd import HipEngine, spmm, workloads
+
+    grid, xs, beams, kw = workloads.full_plan()
+    nb, trunc = kw["n_batches"], kw["truncation"]
+    engine = HipEngine(grid, xs)
+    engine.run_dij(beams[0], 10 * nb, nb, seed=1, truncation=trunc)  # warm
+
+    def plan(n: int, seed: int):
+        started = time.perf_counter()
+        parts = [engine.run_dij(b, n, nb, seed=seed + i, truncation=trunc, value_dtype=np.float32)
+                 for i, b in enumerate(beams)]
+        d = workloads.hstack_dij(parts)
+        return d, time.perf_counter() - started
+
+    probe, probe_wall = plan(probe_n, kw["seed"])
+    p90 = float(np.nanpercentile(_per_beamlet_sigma(probe), 90))
+    del probe
+    n_2pct = int(math.ceil(probe_n * (p90 / target) ** 2 / nb)) * nb
+    n_run = min(n_2pct, cap)
+    d, wall = plan(n_run, kw["seed"] + 100)
+    engine.close()
+    sig = _per_beamlet_sigma(d)
+    out = {
+        "workload": "CT-like 256x256x128 @ 2 mm; 5 beams at gantry 0/72/144/216/288 deg about y through "
+                    "the isocentre, 15x15 beamlets of 5x5 mm each from 100 cm SAD (1125 columns), 6 MV-like "
+                    "spectrum, truncation 1e-4, f32 CSC",
+        "probe_histories_per_beamlet": probe_n,
+        "probe_wall_s": round(probe_wall, 3),
+        "probe_sigma_p90": round(p90, 5),
+        "histories_per_beamlet": n_run,
+        "histories_per_beamlet_for_target": n_2pct,
+        "target_sigma": target,
+        "wall_s": round(wall, 3),
+        "histories_per_s": round(n_run * d.n_beamlets / wall, 1),
+        "nnz": int(d.indptr[-1]),
+        "per_beamlet_sigma_median": round(float(np.nanmedian(sig)), 5),
+        "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
+    }
+    dev = spmm.DeviceDij(d)
+    rng = np.random.default_rng(kw["seed"])
+    weights = rng.random((d.n_beamlets, n_vectors))
+    products = {}
+    for precision, elem in (("f64", 8), ("f32", 4)):
+        dev.apply(weights, precision=precision, fetch=False)  # warm (buffers, code objects)
+        started = time.perf_counter()
+        dev.apply(weights, precision=precision, fetch=False)
+        call_s = time.perf_counter() - started
+        kernel_s = dev.last_kernel_ms / 1e3
+        algorithmic = 8.0 * dev.nnz + elem * dev.n_rows * n_vectors + elem * dev.n_cols * n_vectors
+        products[precision] = {
+            "kernel_ms": round(kernel_s * 1e3, 3),
+            "call_s": round(call_s, 4),
+            "vectors_per_s": round(n_vectors / kernel_s, 1),
+            "roofline": {"bound": "hbm", "achieved": round(algorithmic / kernel_s / 1e9, 1), "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(algorithmic / kernel_s / 1e9 / PEAK_HBM_GBS, 4),
+                         "algorithmic_bytes": int(algorithmic)},
+        }
+    out["product"] = {"weight_vectors": n_vectors, "block_entries": dev.block_entries,
+                      "output": "[voxels][vectors] on the device", **products}
+    dev.close()
+    return out, d
+
+
+def _cpu_dose_for_weights(d, vectors: int = 2) -> dict:
+    """The reference's host product (one weight vector at a time) on the full-plan Dij,
+    timed with the oracle's restatement of DijResult.dose_for_weights."""
+    from oracle import mc
+
+    rng = np.random.default_rng(1)
+    started = time.perf_counter()
+    for _ in range(vectors):
+        mc.dose_for_weights(d.indptr, d.indices, d.dose, rng.random(d.n_beamlets), d.n_voxels)
+    per = (time.perf_counter() - started) / vectors
+    return {"s_per_vector": round(per, 3), "cores": 1, "kind": "port",
+            "sample": f"full-plan Dij, {vectors} weight vectors, one thread (NumPy)"}
+
+
 def _cpu_baseline(sample_histories: int, threads: int) -> dict:
     from oracle import mc
     from pyradmc_amd import workloads
@@ -177,6 +262,17 @@ def _cpu_baseline(sample_histories: int, threads: int) -> dict:
     }
 
 
+def _guarded(name: str, leg, *args):
+    """Run a secondary leg; a Python error is recorded in its place, so that the
+    throughput line is still printed."""
+    print(f"bench: {name} leg", file=sys.stderr, flush=True)
+    try:
+        return leg(*args)
+    except Exception as exc:  # noqa: BLE001 - reported in the JSON line
+        print(f"bench: {name} leg failed: {exc!r}", file=sys.stderr, flush=True)
+        return {"error": repr(exc)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +289,12 @@ def main() -> int:
     ap.add_argument("--denoise-model", default=str(ROOT / "pyradmc_amd" / "models" / "denoiser_ct.npz"),
                     help="trained denoiser weights for the denoised-Dij leg (skipped if absent)")
     ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--no-plan", action="store_true", help="skip the full-plan Dij and product leg")
+    ap.add_argument("--plan-probe", type=int, default=20_000,
+                    help="probe histories per beamlet of the full-plan leg")
+    ap.add_argument("--plan-vectors", type=int, default=1000, help="weight vectors of the planning product")
+    ap.add_argument("--plan-cap", type=int, default=4_000_000,
+                    help="largest histories per beamlet the full-plan leg runs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
                          "kernel that calibrates the HBM byte counters of a PMC profile)")
@@ -264,14 +366,20 @@ def main() -> int:
             counters["electron_kernel_ms"] / max(counters["electron_kernel_ms"] + counters["photon_kernel_ms"], 1e-9), 4),
     }
 
-    dij = None if args.no_dij else _dij_leg(args.dij_median)
-    denoised = _denoise_leg(Path(args.denoise_model), 10_000, 0.02) if denoise_leg else None
+    dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median)
+    denoised = _guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02) if denoise_leg else None
+    plan, plan_dij = None, None
+    if not args.no_plan:
+        got = _guarded("full_plan", _plan_leg, 0.02, args.plan_probe, args.plan_vectors, args.plan_cap)
+        plan, plan_dij = got if isinstance(got, tuple) else (got, None)
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         sample = args.cpu_sample or 2_500_000 * threads  # ~10 s at ~0.25M histories/s/thread
         cpu = _cpu_baseline(sample, threads)
+        if plan_dij is not None:
+            cpu["dose_for_weights"] = _guarded("cpu dose_for_weights", _cpu_dose_for_weights, plan_dij)
 
     line = {
         "metric": "histories/s",
@@ -298,6 +406,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "dij": dij,
         "denoised_dij": denoised,
+        "full_plan": plan,
         "events_per_history": {
             k: round(counters[k] / total, 3)
             for k in ("photon_steps", "electron_steps", "photon_voxels", "electron_voxels",

@@ -82,21 +82,31 @@ __device__ inline long long wave_sum(long long v) {
   return v;
 }
 
-// Queue indices handed to the idle lanes of a wave. The wave claims blocks of
-// kRefillBlock indices from the queue head with one atomic per block and gives them
-// to its idle lanes in lane order, so the single head counter sees one atomic per
-// block instead of one per particle. Called by every lane still in the round loop, at
-// the loop head (converged); the pool state is therefore identical in all of them.
-// Which lane transports which particle does not change any result: each particle
-// carries its own random stream and the tallies are integer sums.
+// Queue indices handed to the idle lanes of a wave, in blocks of kRefillBlock given
+// to the idle lanes in lane order. A wave's first block is its own by position (wave w
+// of the grid takes [64 w, 64 w + 64)), with no atomic; later blocks come from one
+// atomic per block on the queue head, offset past those static blocks. So the single
+// head counter sees one atomic per block instead of one per particle, and a wave that
+// starts beyond an exhausted queue exits without touching it: a round can launch at
+// full width whatever its queue holds. (Reading the head before claiming, instead,
+// queues that read behind the head's atomics in L2 and costs far more than it saves.)
+// Called by every lane still in the round loop, at the loop head (converged); the
+// pool state is therefore identical in all of them. Which lane transports which
+// particle does not change any result: each particle carries its own random stream
+// and the tallies are integer sums.
 constexpr unsigned int kRefillBlock = 64;
 
 struct RefillPool {
-  unsigned int next = 0, end = 0;
+  unsigned int next, end;
+  unsigned int n_static;  // indices below this are the waves' static first blocks
+  // Block launches of 256 threads: every wave of the grid owns one static block.
+  __device__ RefillPool()
+      : next(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)),
+        end(blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + kRefillBlock),
+        n_static(gridDim.x * blockDim.x) {}
 };
 
-__device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, unsigned int n_in,
-                                            bool idle) {
+__device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, bool idle) {
   const unsigned long long mask = __ballot(idle ? 1 : 0);
   if (mask == 0ull) return 0u;
   const unsigned int want = static_cast<unsigned int>(__popcll(mask));
@@ -112,12 +122,7 @@ __device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head
     const unsigned long long present = __ballot(1);
     const int leader = __ffsll(static_cast<long long>(present)) - 1;
     unsigned int base = 0u;
-    if (lane == leader) {
-      // An exhausted queue is read, not claimed: a round launched wider than its
-      // queue costs no atomics in the waves that find nothing left.
-      base = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (base < n_in) base = atomicAdd(head, kRefillBlock);
-    }
+    if (lane == leader) base = pool.n_static + atomicAdd(head, kRefillBlock);
     base = __shfl(base, leader);
     idx = rank < avail ? pool.next + rank : base + (rank - avail);
     pool.next = base + (want - avail);
@@ -430,7 +435,7 @@ k_photon_round(TransportParams P, Queue qin,
   volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
   RefillPool pool;
   while (true) {
-    const unsigned int i = refill_index(pool, head, n_in, !active);
+    const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
       if (i >= n_in) break;
       uint32_t meta;
@@ -611,7 +616,7 @@ k_electron_round(TransportParams P, Queue qin,
   volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
   RefillPool pool;
   while (true) {
-    const unsigned int i = refill_index(pool, head, n_in, !active);
+    const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
       if (i >= n_in) break;
       uint32_t meta;

@@ -72,6 +72,9 @@ int fail(hsp_matrix* m, int code, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(_e));                          \
   } while (0)
 
+// Largest grid of 256-thread workgroups one launch takes (2^32 - 1 work-items).
+constexpr long long kMaxWorkgroups = 16777215;
+
 unsigned int grid_for(long long n, long long cap = 65536) {
   long long b = (n + 255) / 256;
   return static_cast<unsigned int>(std::max(1ll, std::min(b, cap)));
@@ -264,21 +267,31 @@ __device__ __forceinline__ long long xcd_local_group() {
   return start + idx;
 }
 
-// Many vectors: a wave per row block, lanes over vectors (64 * KL per pass).
+// Many vectors: a wave per row block, lanes over vectors, 64 * KL vectors per pass.
+// The pass is the outer grid index (groups_per_pass workgroups each, a multiple of
+// 8): the whole grid finishes one slice of W (n_cols x 64 KL, 2.3 MB for 1125
+// columns in float64) before the next, so the slice in use stays L2-resident, at the
+// price of reading the row blocks once per pass.
 template <typename T, typename V, int KL>
 __global__ void __launch_bounds__(256) k_spmm_wide(const long long* __restrict__ blk_ptr,
                                                    const int* __restrict__ blk_col,
                                                    const V* __restrict__ blk_val,
                                                    const T* __restrict__ w, int n_w, long long r0,
-                                                   long long r1, T* __restrict__ out) {
+                                                   long long r1, unsigned int groups_per_pass,
+                                                   T* __restrict__ out) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   // The wave index is uniform; saying so keeps the block's columns and values in
   // scalar registers and its empty-entry tests on the scalar unit.
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const long long b = r0 / kBlockRows + xcd_local_group() * 4 + wave;
+  const unsigned int pass = blockIdx.x / groups_per_pass, local = blockIdx.x % groups_per_pass;
+  // Within a pass, each XCD (workgroups go round-robin over the 8) takes one
+  // contiguous range of row blocks.
+  const long long group = static_cast<long long>(local & 7u) * (groups_per_pass >> 3) + (local >> 3);
+  const long long b = r0 / kBlockRows + group * 4 + wave;
   if (b * kBlockRows >= r1) return;
   const long long u0 = blk_ptr[b], u1 = blk_ptr[b + 1];
-  for (int k0 = 0; k0 < n_w; k0 += 64 * KL) {
+  {
+    const int k0 = static_cast<int>(pass) * 64 * KL;
     T acc[kBlockRows][KL];
 #pragma unroll
     for (int i = 0; i < kBlockRows; ++i)
@@ -346,26 +359,41 @@ __global__ void __launch_bounds__(256) k_spmm_narrow(const long long* __restrict
     if (q < n_w) dst[q] = acc[q];
 }
 
+// Launch shape of a product over rows [r0, r1): workgroups per vector pass, passes,
+// and vectors per lane (0 for the narrow kernel, whose passes count is 1).
+struct ProductShape {
+  long long groups_per_pass, passes;
+  int kl;
+};
+
+ProductShape product_shape(int n_w, long long r0, long long r1) {
+  const long long blocks = (r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows;
+  if (n_w <= 8) return {(blocks + 256 / kBlockRows - 1) / (256 / kBlockRows), 1, 0};
+  const int kl = n_w <= 64 ? 1 : n_w <= 128 ? 2 : 4;
+  return {((blocks + 3) / 4 + 7) / 8 * 8, (n_w + 64 * kl - 1) / (64 * kl), kl};
+}
+
 template <typename T, typename V>
 void launch_product(hsp_matrix* m, const T* w, int n_w, long long r0, long long r1, T* out) {
   const V* bv = static_cast<const V*>(m->blk_val);
-  const long long blocks = (r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows;
-  if (n_w <= 8) {
-    const long long groups = (blocks + 256 / kBlockRows - 1) / (256 / kBlockRows);
-    hipLaunchKernelGGL((k_spmm_narrow<T, V>), dim3(static_cast<unsigned int>(groups)), dim3(256), 0, m->stream,
-                       m->blk_ptr, m->blk_col, bv, w, n_w, r0, r1, out);
+  const ProductShape shape = product_shape(n_w, r0, r1);
+  if (shape.kl == 0) {
+    hipLaunchKernelGGL((k_spmm_narrow<T, V>), dim3(static_cast<unsigned int>(shape.groups_per_pass)), dim3(256), 0,
+                       m->stream, m->blk_ptr, m->blk_col, bv, w, n_w, r0, r1, out);
     return;
   }
-  const dim3 grid(static_cast<unsigned int>((blocks + 3) / 4));
-  if (n_w <= 64)
+  const unsigned int gpp = static_cast<unsigned int>(shape.groups_per_pass);
+  const int kl = shape.kl;
+  const dim3 grid(static_cast<unsigned int>(shape.groups_per_pass * shape.passes));
+  if (kl == 1)
     hipLaunchKernelGGL((k_spmm_wide<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, out);
-  else if (n_w <= 128)
+                       r0, r1, gpp, out);
+  else if (kl == 2)
     hipLaunchKernelGGL((k_spmm_wide<T, V, 2>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, out);
+                       r0, r1, gpp, out);
   else
     hipLaunchKernelGGL((k_spmm_wide<T, V, 4>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, out);
+                       r0, r1, gpp, out);
 }
 
 // Exclusive scan of n device counts into out[0..n]; returns the total.
@@ -502,8 +530,9 @@ int check_apply(hsp_matrix* m, int n_w, long long r0, long long r1, int precisio
   if (n_w < 1) return fail(m, HSP_EINVAL, "n_w must be at least 1");
   if (precision != HSP_F64 && precision != HSP_F32) return fail(m, HSP_EINVAL, "unknown precision");
   if (r0 < 0 || r1 > m->n_rows || r0 >= r1) return fail(m, HSP_EINVAL, "row range outside the matrix");
-  if ((r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows > 4ll * 0x7fffffffll)
-    return fail(m, HSP_EINVAL, "row range too large for one launch");
+  const ProductShape shape = product_shape(n_w, r0, r1);
+  if (shape.groups_per_pass * shape.passes > kMaxWorkgroups)
+    return fail(m, HSP_EINVAL, "row range and vector count too large for one launch: split the rows");
   return HSP_OK;
 }
 

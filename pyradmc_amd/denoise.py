@@ -18,7 +18,8 @@ Network: four levels of two 3x3x3 convolutions (widths 16, 32, 64, 128 by defaul
 about 1.3 M parameters), 2x2x2 average pooling down, nearest upsampling up,
 additive skips, leaky ReLU (slope 0.1), and a residual head: prediction = noisy
 input + head output. Loss: squared error weighted by 1 / (sigma_target^2 + floor^2)
-(the target's own Monte Carlo variance, floored), averaged over voxels.
+(the target's own Monte Carlo variance, floored), or by the target dose (the weight
+of the dose-weighted error that judges a column), averaged over voxels.
 """
 
 from __future__ import annotations
@@ -27,6 +28,7 @@ import ctypes
 import math
 import sys
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -498,28 +500,38 @@ def _pack_device(noisy, density):
     return x, x.reshape(-1, CIN_PAD).t().contiguous()
 
 
-def _stack_device(pairs: list[ColumnPair], device: str):
+def _stack_device(pairs: list[ColumnPair], device: str, weighting: str = "sigma"):
+    """Device tensors of the pairs and the per-voxel loss weight: ``"sigma"`` weighs
+    by 1/(sigma_target^2 + floor^2); ``"dose"`` by the target dose plus 1 percent of
+    the column scale (the dose-weighted error that judges a column, with a floor that
+    keeps the low-dose tail in the loss)."""
     torch = _torch()
+    if weighting not in ("sigma", "dose"):
+        raise ValueError(f"weighting must be 'sigma' or 'dose', got {weighting!r}")
 
     def up(name):
         return torch.from_numpy(np.ascontiguousarray(np.stack([getattr(p, name) for p in pairs]),
                                                      dtype=np.float32)).to(device)
 
     noisy, dens, target = up("noisy"), up("density"), up("target")
-    sig = up("target_sigma")
-    weight = 1.0 / (sig * sig + SIGMA_FLOOR * SIGMA_FLOOR)
-    del sig
+    if weighting == "dose":
+        weight = target.clamp(min=0.0) + 0.01
+    else:
+        sig = up("target_sigma")
+        weight = 1.0 / (sig * sig + SIGMA_FLOOR * SIGMA_FLOOR)
+        del sig
     return noisy, dens, target, weight
 
 
 def train(model: Conv3dDenoiser, pairs: list[ColumnPair], epochs: int = 10, batch: int = 8,
-          lr: float = 1e-3, lr_final: float = 3e-5, crop: int = 64, seed: int = 0, log=None) -> list[float]:
+          lr: float = 1e-3, lr_final: float = 3e-5, crop: int = 64, seed: int = 0, log=None,
+          weighting: str = "sigma", on_epoch=None) -> list[float]:
     """Adam with cosine learning-rate decay on random crop^3 sub-windows (random
     x/y flips); the pairs are held on the device for the whole run. Returns the mean
-    loss of each epoch."""
+    loss of each epoch; ``on_epoch(epoch)`` runs after each epoch."""
     torch = _torch()
     rng = np.random.default_rng(seed)
-    noisy, dens, target, weight = _stack_device(pairs, model.device)
+    noisy, dens, target, weight = _stack_device(pairs, model.device, weighting)
     n = len(pairs)
     depth = noisy.shape[3]
     steps_per_epoch = max(n // batch, 1)
@@ -558,6 +570,8 @@ def train(model: Conv3dDenoiser, pairs: list[ColumnPair], epochs: int = 10, batc
         history.append(float(loss_sum.item()) / steps_per_epoch)
         if log:
             log(f"epoch {epoch}: loss {history[-1]:.6g} lr {rate:.3g}")
+        if on_epoch:
+            on_epoch(epoch)
     return history
 
 
@@ -788,6 +802,10 @@ def _main(argv=None) -> int:
     tr.add_argument("--batch", type=int, default=8)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--seed", type=int, default=20260801)
+    tr.add_argument("--weighting", default="sigma",
+                    help="loss weighting: sigma, dose, or both (two models from one set of pairs; "
+                         "the second is saved with '_dose' before the suffix)")
+    tr.add_argument("--eval-every", type=int, default=5, help="held-out evaluation every N epochs")
     args = ap.parse_args(argv)
     started = time.perf_counter()
     say = lambda m: print(m, flush=True)  # noqa: E731
@@ -795,21 +813,34 @@ def _main(argv=None) -> int:
     held = generate_pairs(args.held_out, args.beamlets, args.low, args.high, seed=args.seed + 1_000_003,
                           log=say)
     t_data = time.perf_counter() - started
-    model = Conv3dDenoiser(seed=args.seed)
-    t0 = time.perf_counter()
-    hist = train(model, pairs, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=say)
-    _torch().cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    ev = evaluate(model, held)
-    state = model.state_dict()
-    np.savez_compressed(args.out, **state, widths=np.array(model.widths))
-    print(json.dumps({
-        "pairs": len(pairs), "held_out": len(held), "parameters": model.n_parameters,
-        "data_s": round(t_data, 1), "train_s": round(t_train, 1), "loss": [round(v, 6) for v in hist],
-        "held_out_noisy_rms_median": float(np.median(ev["noisy"])),
-        "held_out_denoised_rms_median": float(np.median(ev["denoised"])),
-        "held_out_denoised_rms_p90": float(np.percentile(ev["denoised"], 90)),
-    }), flush=True)
+    weightings = ("sigma", "dose") if args.weighting == "both" else (args.weighting,)
+    for k, weighting in enumerate(weightings):
+        out = Path(args.out)
+        if k:
+            out = out.with_name(out.stem + "_" + weighting + out.suffix)
+        model = Conv3dDenoiser(seed=args.seed)
+
+        def progress(epoch, model=model):
+            if args.eval_every and (epoch + 1) % args.eval_every == 0:
+                ev = evaluate(model, held)
+                say(f"epoch {epoch}: held-out denoised rms median {np.median(ev['denoised']):.5f} "
+                    f"p90 {np.percentile(ev['denoised'], 90):.5f} (noisy {np.median(ev['noisy']):.5f})")
+
+        t0 = time.perf_counter()
+        hist = train(model, pairs, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=say,
+                     weighting=weighting, on_epoch=progress)
+        _torch().cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        ev = evaluate(model, held)
+        np.savez_compressed(out, **model.state_dict(), widths=np.array(model.widths))
+        print(json.dumps({
+            "out": str(out), "weighting": weighting,
+            "pairs": len(pairs), "held_out": len(held), "parameters": model.n_parameters,
+            "data_s": round(t_data, 1), "train_s": round(t_train, 1), "loss": [round(v, 6) for v in hist],
+            "held_out_noisy_rms_median": float(np.median(ev["noisy"])),
+            "held_out_denoised_rms_median": float(np.median(ev["denoised"])),
+            "held_out_denoised_rms_p90": float(np.percentile(ev["denoised"], 90)),
+        }), flush=True)
     return 0
 
 

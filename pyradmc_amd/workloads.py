@@ -11,6 +11,12 @@
   lung / soft-tissue / bone ellipsoids; 100 spectral beamlets of 5x5 mm on a
   10x10 cm grid from a point source at 100 cm SAD; lognormal per-beamlet counts
   (median 2e5, sigma 0.8).
+* ``full_plan`` - 256x256x128 voxels at 2 mm, the same kind of CT-like phantom; five
+  coplanar beams at gantry 0/72/144/216/288 degrees about the y axis through the
+  isocentre (the grid centre), each 15x15 spectral beamlets of 5x5 mm from 100 cm SAD
+  (1125 columns). Each beam is its own source (the gantry rotation is applied to the
+  focal point, beamlet centres and aperture axes on the host); ``hstack_dij`` joins
+  the five beams' Dijs column-wise, in gantry order.
 
 All heterogeneity is density-only water (the analytic source is water-only).
 """
@@ -22,6 +28,7 @@ import math
 import numpy as np
 
 from pyradmc_amd.data.analytic import AnalyticCrossSections
+from pyradmc_amd.scoring import DijResult
 from pyradmc_amd.geometry import (
     PencilBeamSource,
     SpectralBeamletSource,
@@ -119,3 +126,71 @@ def dose_weighted_rms(dose: np.ndarray, reference: np.ndarray) -> float:
     r = reference[mask]
     d = dose[mask]
     return math.sqrt(float(np.sum(r * (d - r) ** 2) / np.sum(r))) / r_max
+
+
+FULL_PLAN_GANTRY_DEG = (0.0, 72.0, 144.0, 216.0, 288.0)
+
+
+def full_plan_grid(seed: int = 20260711) -> VoxelGrid:
+    return ct_like_grid(seed, shape=(256, 256, 128), spacing=0.2)
+
+
+def full_plan_beams(grid: VoxelGrid, n_side: int = 15, pitch: float = 0.5, sad: float = 100.0,
+                    gantry_deg=FULL_PLAN_GANTRY_DEG) -> list:
+    """One SpectralBeamletSource per gantry angle. Gantry 0 points along +z; a gantry
+    angle t turns the beam about the y axis through the isocentre, to direction
+    (sin t, 0, cos t), with the aperture's u axis (cos t, 0, -sin t) and v axis +y."""
+    iso = np.array([grid.shape[i] * grid.spacing[i] / 2.0 for i in range(3)])
+    half = (n_side - 1) / 2.0
+    beams = []
+    for deg in gantry_deg:
+        t = math.radians(deg)
+        d = np.array([math.sin(t), 0.0, math.cos(t)])
+        u = np.array([math.cos(t), 0.0, -math.sin(t)])
+        v = np.array([0.0, 1.0, 0.0])
+        centers = [tuple(iso + (i - half) * pitch * u + (j - half) * pitch * v)
+                   for i in range(n_side) for j in range(n_side)]
+        beams.append(SpectralBeamletSource(bremsstrahlung_6mv_like(), tuple(iso - sad * d), centers, pitch, pitch,
+                                           u_axis=tuple(u), v_axis=tuple(v)))
+    return beams
+
+
+def full_plan(seed: int = 20260711):
+    grid = full_plan_grid(seed)
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    return grid, xs, full_plan_beams(grid), dict(n_batches=10, seed=seed, truncation=1e-4)
+
+
+def hstack_dij(parts: list) -> DijResult:
+    """Column-wise join of Dijs over one grid (beam after beam); energies add."""
+    if not parts:
+        raise ValueError("nothing to join")
+    first = parts[0]
+    if any(tuple(p.grid_shape) != tuple(first.grid_shape) or p.n_batches != first.n_batches for p in parts):
+        raise ValueError("Dijs to join must share the grid and the batch count")
+    indptr = [np.zeros(1, dtype=np.int64)]
+    offset = 0
+    for p in parts:
+        indptr.append(np.asarray(p.indptr[1:], dtype=np.int64) + offset)
+        offset += int(p.indptr[-1])
+    counts = [np.broadcast_to(np.asarray(p.n_histories_per_beamlet, dtype=np.int64), (p.n_beamlets,))
+              for p in parts]
+    counts = np.concatenate(counts)
+    return DijResult(
+        grid_shape=tuple(first.grid_shape),
+        n_beamlets=sum(p.n_beamlets for p in parts),
+        n_histories_per_beamlet=int(counts[0]) if np.all(counts == counts[0]) else counts,
+        n_batches=first.n_batches,
+        truncation=first.truncation,
+        indptr=np.concatenate(indptr),
+        indices=np.concatenate([p.indices for p in parts]),
+        dose=np.concatenate([p.dose for p in parts]),
+        sigma=np.concatenate([p.sigma for p in parts]),
+        energy_emitted=sum(p.energy_emitted for p in parts),
+        energy_deposited=sum(p.energy_deposited for p in parts),
+        energy_escaped=sum(p.energy_escaped for p in parts),
+        energy_unscored=sum(p.energy_unscored for p in parts),
+        correlated=first.correlated,
+        scoring_mode=first.scoring_mode,
+        provenance=first.provenance,
+    )
