@@ -10,6 +10,12 @@ cross-sections - one complete ``HipEngine.run``: generation, transport, per-batc
 float64 fold and finalize, with the phantom, tables and GS grid already resident in
 HBM. Data are synthetic. Prints one JSON line.
 
+Before the warmup, ``pyradmc_amd.autotune`` picks the chunk size and the rounds' launch
+shape for this workload (untimed; no result depends on them; ``--no-autotune`` keeps
+the defaults). The timed steps run with per-kernel HIP events, so their rounds launch
+kernel by kernel; ``graph_replay`` times the same steps unprofiled, where each group of
+four rounds replays a captured HIP graph.
+
 ``roofline`` is for the dominant kernel (the electron round): algorithmic bytes per
 launch - 48 B per particle popped, 48 B per queue push, 5 B per voxel fetch (float32
 density + uint8 material), 16 B per int64 tally atomic; tables and the GS grid are
@@ -295,6 +301,7 @@ def main() -> int:
     ap.add_argument("--plan-vectors", type=int, default=1000, help="weight vectors of the planning product")
     ap.add_argument("--plan-cap", type=int, default=4_000_000,
                     help="largest histories per beamlet the full-plan leg runs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape and chunk size")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
                          "kernel that calibrates the HBM byte counters of a PMC profile)")
@@ -326,6 +333,11 @@ def main() -> int:
 
     grid, xs, src, kw = workloads.slab_phantom(n_histories=args.histories, n_batches=args.batches)
     engine = HipEngine(grid, xs)
+    tuned = None
+    if not args.no_autotune:
+        # Launch shape and chunk size for this workload (untimed; results do not depend on them).
+        print("bench: autotune", file=sys.stderr, flush=True)
+        tuned = engine.autotune(src, args.histories, args.batches, seed=999, repeats=2, use_cache=False)
     for w in range(args.warmup):
         engine.run(src, args.histories, args.batches, seed=1000 + w)
     engine.set_profiling(True)
@@ -339,6 +351,13 @@ def main() -> int:
     engine.set_profiling(False)
     total = args.histories * args.steps
     value = total / elapsed
+    # The same steps unprofiled: groups of rounds replay the captured HIP graph (the
+    # profiled steps launch kernel by kernel so that each one is timed).
+    graph_counts = 0
+    started = time.perf_counter()
+    for k in range(args.steps):
+        graph_counts += engine.run(src, args.histories, args.batches, seed=kw["seed"] + k).counters["graph_launches"]
+    replay_elapsed = time.perf_counter() - started
 
     launches = max(int(counters["electron_launches"]), 1)
     mean_launch_s = counters["electron_kernel_ms"] / 1e3 / launches
@@ -401,7 +420,12 @@ def main() -> int:
             "n_batches": args.batches,
             "device": engine.device_name,
             "tally": "int64 fixed point (1e-9 MeV), f64 fold",
+            "launch": {**engine.launch, "chunk_size": engine.chunk_size or "auto",
+                       "autotuned": tuned is not None, "profiled_steps": "kernel by kernel"},
         },
+        "graph_replay": {"ms_per_step": round(replay_elapsed / args.steps * 1e3, 3),
+                         "histories_per_s": round(total / replay_elapsed, 1),
+                         "graph_launches": int(graph_counts)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dij": dij,

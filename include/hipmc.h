@@ -94,6 +94,7 @@ typedef struct hmc_ledger {
   int64_t electron_voxels, electron_deposits, electron_pushes, electron_pops, electron_steps;
   int64_t photon_launches, electron_launches;
   double photon_kernel_ms, electron_kernel_ms; /* only with hmc_set_profiling(ctx, 1) */
+  int64_t graph_launches; /* groups of rounds replayed from the captured graph */
 } hmc_ledger;
 
 /* Lifetime ------------------------------------------------------------------- */
@@ -130,6 +131,14 @@ int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
 /* Time every transport launch with HIP events on the context's stream (no
  * reference counterpart: the measurement hook for the per-kernel roofline). */
 int hmc_set_profiling(hmc_ctx *ctx, int enabled);
+/* Launch shape of the transport rounds: lanes per CU of each round's grid (multiple
+ * of 256 in [256, 8192]; default 2048), queue indices a wave claims at a time
+ * (multiple of 64 in [64, 1024]; default 64), and whether each group of four rounds
+ * is replayed from a captured HIP graph (default 1; profiled transports launch one
+ * by one, so that every kernel is timed). None of them changes a result.
+ * No reference counterpart beyond the chunk_size / queue_factor knobs of WarpEngine
+ * (engine.py:369-407): these are the autotuner's knobs. */
+int hmc_set_launch(hmc_ctx *ctx, int round_lanes_per_cu, int refill_block, int use_graphs);
 
 /* Work ----------------------------------------------------------------------- */
 /* Generate and transport every primary of the work items into the dense tallies:

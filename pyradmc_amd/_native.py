@@ -115,7 +115,7 @@ class Ledger(Structure):
             "photon_steps", "electron_voxels", "electron_deposits", "electron_pushes",
             "electron_pops", "electron_steps", "photon_launches", "electron_launches",
         )
-    ] + [("photon_kernel_ms", c_double), ("electron_kernel_ms", c_double)]
+    ] + [("photon_kernel_ms", c_double), ("electron_kernel_ms", c_double), ("graph_launches", c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -149,6 +149,7 @@ def lib():
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
         "hmc_set_profiling": (c_int, [ctx_p, c_int]),
+        "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int]),
         "hmc_transport": (c_int, [ctx_p, c_uint64, POINTER(WorkItem), c_int, c_int64, c_int, POINTER(Ledger)]),
         "hmc_fold": (c_int, [ctx_p, c_int, _f64p, POINTER(c_int64)]),
         "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),

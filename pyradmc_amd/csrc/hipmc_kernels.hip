@@ -82,10 +82,11 @@ __device__ inline long long wave_sum(long long v) {
   return v;
 }
 
-// Queue indices handed to the idle lanes of a wave, in blocks of kRefillBlock given
-// to the idle lanes in lane order. A wave's first block is its own by position (wave w
-// of the grid takes [64 w, 64 w + 64)), with no atomic; later blocks come from one
-// atomic per block on the queue head, offset past those static blocks. So the single
+// Queue indices handed to the idle lanes of a wave, in blocks of R (the refill block,
+// a multiple of 64 set per transport call) given to the idle lanes in lane order. A
+// wave's first block is its own by position (wave w of the grid takes [R w, R w + R)),
+// with no atomic; later blocks come from one atomic per block on the queue head,
+// offset past those static blocks. So the single
 // head counter sees one atomic per block instead of one per particle, and a wave that
 // starts beyond an exhausted queue exits without touching it: a round can launch at
 // full width whatever its queue holds. (Reading the head before claiming, instead,
@@ -94,16 +95,16 @@ __device__ inline long long wave_sum(long long v) {
 // pool state is therefore identical in all of them. Which lane transports which
 // particle does not change any result: each particle carries its own random stream
 // and the tallies are integer sums.
-constexpr unsigned int kRefillBlock = 64;
-
 struct RefillPool {
   unsigned int next, end;
+  unsigned int block;     // R
   unsigned int n_static;  // indices below this are the waves' static first blocks
-  // Block launches of 256 threads: every wave of the grid owns one static block.
-  __device__ RefillPool()
-      : next(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)),
-        end(blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + kRefillBlock),
-        n_static(gridDim.x * blockDim.x) {}
+  // Every wave of the grid owns one static block.
+  __device__ explicit RefillPool(unsigned int r)
+      : next((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * r),
+        end((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * r + r),
+        block(r),
+        n_static(gridDim.x * (blockDim.x >> 6) * r) {}
 };
 
 __device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head, bool idle) {
@@ -122,11 +123,11 @@ __device__ inline unsigned int refill_index(RefillPool& pool, unsigned int* head
     const unsigned long long present = __ballot(1);
     const int leader = __ffsll(static_cast<long long>(present)) - 1;
     unsigned int base = 0u;
-    if (lane == leader) base = pool.n_static + atomicAdd(head, kRefillBlock);
+    if (lane == leader) base = pool.n_static + atomicAdd(head, pool.block);
     base = __shfl(base, leader);
     idx = rank < avail ? pool.next + rank : base + (rank - avail);
     pool.next = base + (want - avail);
-    pool.end = base + kRefillBlock;
+    pool.end = base + pool.block;
   }
   return idx;
 }
@@ -163,6 +164,7 @@ struct TransportParams {
   float pcut, ecut, step_fraction, resolution;
   float log_pcut;  // logf(pcut), for the bremsstrahlung spectrum
   int transport_electrons, msc_gs, dose_to_water, max_steps;
+  unsigned int refill_block;  // queue indices a wave claims at a time (multiple of 64)
 };
 
 // Per-lane books kept in registers and flushed once per lane at the end of a round;
@@ -433,7 +435,7 @@ k_photon_round(TransportParams P, Queue qin,
     qp.pushed = &ledger_red[7 * 4 + wave_in_block];
   }
   volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
-  RefillPool pool;
+  RefillPool pool(P.refill_block);
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
@@ -614,7 +616,7 @@ k_electron_round(TransportParams P, Queue qin,
     qe.pushed = &ledger_red[7 * 4 + wave_in_block];
   }
   volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
-  RefillPool pool;
+  RefillPool pool(P.refill_block);
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
     if (!active) {
@@ -1299,6 +1301,14 @@ struct hmc_ctx {
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;  // per-launch timing events (profiling)
 
+  // launch shape of the transport rounds (hmc_set_launch)
+  int round_lanes_per_cu = 2048;
+  int refill_block = 64;
+  bool use_graphs = true;
+  // the captured group of rounds and what it was captured with
+  hipGraphExec_t round_graph = nullptr;
+  std::vector<unsigned char> round_graph_key;
+
   // truncation
   unsigned long long* d_colmax = nullptr;
   long long* d_chunk_counts = nullptr;
@@ -1317,12 +1327,16 @@ int fail(hmc_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+// A failed call is reported once: the runtime's last-error slot is cleared so a later,
+// unrelated hipGetLastError() does not return it again.
 #define HMC_HIP(ctx, call)                                                              \
   do {                                                                                  \
     hipError_t _e = (call);                                                             \
-    if (_e != hipSuccess)                                                               \
+    if (_e != hipSuccess) {                                                             \
+      (void)hipGetLastError();                                                          \
       return fail(ctx, _e == hipErrorOutOfMemory ? HMC_ENOMEM : HMC_EHIP,               \
                   std::string(#call) + ": " + hipGetErrorString(_e));                   \
+    }                                                                                   \
   } while (0)
 
 // Device buffer of at least `count` elements: an existing allocation that is large
@@ -1414,6 +1428,7 @@ void hmc_destroy(hmc_ctx* ctx) {
   for (void* p : ctx->table_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->round_graph) (void)hipGraphExecDestroy(ctx->round_graph);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1719,6 +1734,101 @@ int hmc_set_profiling(hmc_ctx* ctx, int enabled) {
   return HMC_OK;
 }
 
+int hmc_set_launch(hmc_ctx* ctx, int round_lanes_per_cu, int refill_block, int use_graphs) {
+  if (!ctx) return HMC_EINVAL;
+  if (round_lanes_per_cu < 256 || round_lanes_per_cu > 8192 || round_lanes_per_cu % 256 != 0)
+    return fail(ctx, HMC_EINVAL, "round lanes per CU must be a multiple of 256 in [256, 8192]");
+  if (refill_block < 64 || refill_block > 1024 || refill_block % 64 != 0)
+    return fail(ctx, HMC_EINVAL, "refill block must be a multiple of 64 in [64, 1024]");
+  ctx->round_lanes_per_cu = round_lanes_per_cu;
+  ctx->refill_block = refill_block;
+  ctx->use_graphs = use_graphs != 0;
+  return HMC_OK;
+}
+
+// Enqueue one group of kRoundsPerSync photon / electron rounds on the context's stream,
+// starting from queue roles (0, 1 | 2, 3): photons and electrons in queues 0 and 1,
+// outputs to 2 and 3, roles swapped after each round. The group has an even number of
+// rounds, so it ends with the starting roles and every group is the same sequence of
+// launches. With events (profiling), each round's two kernels are bracketed.
+constexpr int kRoundsPerSync = 4;
+
+static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned int round_blocks,
+                               unsigned int cap, hipEvent_t* events) {
+  unsigned int* counts = ctx->d_counts;
+  unsigned int* heads = ctx->d_counts + 4;
+  int pa = 0, ea = 1, pb = 2, eb = 3;
+  for (int r = 0; r < kRoundsPerSync; ++r) {
+    Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
+    Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
+    hipEvent_t* ev = events ? events + 4 * r : nullptr;
+    // Photon round: consume pa; electrons append to ea, photons to pb.
+    if (ev) HMC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+    hipLaunchKernelGGL(k_photon_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p, heads + pa,
+                       qa_e, qb_p);
+    HMC_HIP(ctx, hipGetLastError());
+    if (ev) HMC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    // Electron round: consume ea (including the photon round's pushes).
+    if (ev) HMC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+    hipLaunchKernelGGL(k_electron_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e, heads + ea,
+                       qb_p, qb_e);
+    HMC_HIP(ctx, hipGetLastError());
+    if (ev) HMC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+    hipLaunchKernelGGL(k_reset_consumed, dim3(1), dim3(64), 0, ctx->stream, counts, heads, pa, ea, cap,
+                       ctx->d_ledger);
+    HMC_HIP(ctx, hipGetLastError());
+    std::swap(pa, pb);
+    std::swap(ea, eb);
+  }
+  return HMC_OK;
+}
+
+// Launch one group of rounds: as a replay of the captured graph when graphs are on
+// (captured again whenever anything the launches take by value has changed), else
+// launch by launch. Profiled groups always launch one by one: events recorded inside
+// a captured graph cannot be timed (hipEventElapsedTime rejects them).
+static int launch_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned int round_blocks,
+                              unsigned int cap, hipEvent_t* events, bool* replayed) {
+  *replayed = false;
+  if (!ctx->use_graphs || events) return enqueue_round_group(ctx, P, round_blocks, cap, events);
+  std::vector<unsigned char> key(sizeof(P) + sizeof(ctx->d_q) + 3 * sizeof(unsigned int) + sizeof(void*) * 2);
+  unsigned char* k = key.data();
+  std::memcpy(k, &P, sizeof(P));
+  k += sizeof(P);
+  std::memcpy(k, ctx->d_q, sizeof(ctx->d_q));
+  k += sizeof(ctx->d_q);
+  const unsigned int ints[3] = {round_blocks, cap, 0u};
+  std::memcpy(k, ints, sizeof(ints));
+  k += sizeof(ints);
+  const void* ptrs[2] = {ctx->d_counts, ctx->d_ledger};
+  std::memcpy(k, ptrs, sizeof(ptrs));
+  if (!ctx->round_graph || key != ctx->round_graph_key) {
+    if (ctx->round_graph) {
+      HMC_HIP(ctx, hipGraphExecDestroy(ctx->round_graph));
+      ctx->round_graph = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HMC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_round_group(ctx, P, round_blocks, cap, events);
+    const hipError_t end = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != HMC_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HMC_HIP(ctx, end);
+    const hipError_t inst = hipGraphInstantiate(&ctx->round_graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+      ctx->round_graph = nullptr;
+      HMC_HIP(ctx, inst);
+    }
+    ctx->round_graph_key = key;
+  }
+  HMC_HIP(ctx, hipGraphLaunch(ctx->round_graph, ctx->stream));
+  *replayed = true;
+  return HMC_OK;
+}
+
 int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n_items,
                   int64_t chunk, int queue_factor, hmc_ledger* ledger) {
   if (!ctx || !items || n_items < 1 || chunk < 1 || queue_factor < 1 || !ledger)
@@ -1769,6 +1879,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   }
 
   TransportParams P;
+  std::memset(&P, 0, sizeof(P));  // padding too: the bytes key the captured graph
   P.g = ctx->g;
   P.s = ctx->s;
   P.t = ctx->t;
@@ -1783,6 +1894,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   P.msc_gs = ctx->phys.msc_gs;
   P.dose_to_water = ctx->phys.dose_to_water;
   P.max_steps = 1 << 20;
+  P.refill_block = static_cast<unsigned int>(ctx->refill_block);
 
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ledger,
                      static_cast<long long>(L_COUNT), 0ull);
@@ -1791,10 +1903,11 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   // Rounds launch at full width: lanes of an exhausted queue exit at once, so the host
   // never needs a queue's size to launch it, and it enqueues kRoundsPerSync photon /
   // electron rounds between count reads.
-  const unsigned int round_blocks =
-      static_cast<unsigned int>(blocks_for(static_cast<long long>(std::max(ctx->cu_count, 1)) * 2048));
-  constexpr int kRoundsPerSync = 4;
-  long long overflow = 0, rounds = 0, launches[2] = {0, 0};
+  const unsigned int round_blocks = static_cast<unsigned int>(
+      blocks_for(static_cast<long long>(std::max(ctx->cu_count, 1)) * ctx->round_lanes_per_cu));
+  if (static_cast<long long>(round_blocks) * 4 * ctx->refill_block > (1ll << 30))
+    return fail(ctx, HMC_EINVAL, "round width times refill block too large");
+  long long overflow = 0, rounds = 0, launches[2] = {0, 0}, graph_launches = 0;
   double kernel_ms[2] = {0.0, 0.0};
   if (ctx->profile && ctx->ev_pool.size() < 4 * kRoundsPerSync) {
     for (size_t k = ctx->ev_pool.size(); k < 4 * kRoundsPerSync; ++k) {
@@ -1804,42 +1917,25 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     }
   }
   unsigned int* counts = ctx->d_counts;
-  unsigned int* heads = ctx->d_counts + 4;
 
   for (long long p0 = 0; p0 < total && !overflow; p0 += chunk_n) {
     const long long n_gen = std::min(chunk_n, total - p0);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, 8, 0u);
-    int pa = 0, ea = 1, pb = 2, eb = 3;  // queue roles
+    const int pa = 0, ea = 1;  // queue roles at the start of every group of rounds
     Queue qpa{ctx->d_q[pa], counts + pa, cap}, qea{ctx->d_q[ea], counts + ea, cap};
     hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
                        ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
                        static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());
     while (true) {
-      for (int r = 0; r < kRoundsPerSync; ++r) {
-        Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
-        Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
-        hipEvent_t* ev = ctx->profile ? &ctx->ev_pool[4 * r] : nullptr;
-        // Photon round: consume pa; electrons append to ea, photons to pb.
-        if (ev) HMC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-        hipLaunchKernelGGL(k_photon_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p,
-                           heads + pa, qa_e, qb_p);
-        HMC_HIP(ctx, hipGetLastError());
-        if (ev) HMC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-        // Electron round: consume ea (including the photon round's pushes).
-        if (ev) HMC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-        hipLaunchKernelGGL(k_electron_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
-                           heads + ea, qb_p, qb_e);
-        HMC_HIP(ctx, hipGetLastError());
-        if (ev) HMC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-        hipLaunchKernelGGL(k_reset_consumed, dim3(1), dim3(64), 0, ctx->stream, counts, heads, pa,
-                           ea, cap, ctx->d_ledger);
-        std::swap(pa, pb);
-        std::swap(ea, eb);
-        ++rounds;
-        ++launches[0];
-        ++launches[1];
-      }
+      bool replayed = false;
+      if ((rc = launch_round_group(ctx, P, round_blocks, cap, ctx->profile ? ctx->ev_pool.data() : nullptr,
+                                   &replayed)))
+        return rc;
+      graph_launches += replayed ? 1 : 0;
+      rounds += kRoundsPerSync;
+      launches[0] += kRoundsPerSync;
+      launches[1] += kRoundsPerSync;
       unsigned int host_counts[4];
       HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts), hipMemcpyDeviceToHost,
                                   ctx->stream));
@@ -1887,6 +1983,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   ledger->electron_launches = launches[1];
   ledger->photon_kernel_ms = kernel_ms[0];
   ledger->electron_kernel_ms = kernel_ms[1];
+  ledger->graph_launches = graph_launches;
   if (overflow)
     return fail(ctx, HMC_EOVERFLOW,
                 "secondary queue overflow (" + std::to_string(overflow) + " > capacity " +

@@ -67,9 +67,11 @@ int fail(hsp_matrix* m, int code, const std::string& msg) {
 #define HSP_HIP(m, call)                                                                       \
   do {                                                                                         \
     hipError_t _e = (call);                                                                    \
-    if (_e != hipSuccess)                                                                      \
+    if (_e != hipSuccess) {                                                                    \
+      (void)hipGetLastError(); /* reported here, not again by a later call */                  \
       return fail(m, _e == hipErrorOutOfMemory ? HSP_ENOMEM : HSP_EHIP,                        \
                   std::string(#call) + ": " + hipGetErrorString(_e));                          \
+    }                                                                                          \
   } while (0)
 
 // Largest grid of 256-thread workgroups one launch takes (2^32 - 1 work-items).

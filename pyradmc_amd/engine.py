@@ -117,6 +117,7 @@ class HipEngine:
     _info: dict | None = field(default=None, init=False, repr=False)
     _scoring_default: object = field(default=None, init=False, repr=False)
     _scoring_last: object = field(default=None, init=False, repr=False)
+    launch: dict = field(default_factory=dict, init=False, repr=False)  # set_launch's knobs
 
     def __post_init__(self) -> None:
         if self.queue_factor < 1:
@@ -131,6 +132,7 @@ class HipEngine:
         self._index = _device_index(self.device)
         self._ctx = nat.Context(self._index)
         self._info = nat.device_info(self._index)
+        self.launch = {"round_lanes_per_cu": 2048, "refill_block": 64, "use_graphs": True}
 
     def close(self) -> None:
         if self._ctx is not None:
@@ -413,6 +415,23 @@ class HipEngine:
         """Time every transport launch with HIP events (per-kernel ``*_kernel_ms`` in
         the result's ``counters``); a measurement hook, off by default."""
         self._ctx.call("hmc_set_profiling", 1 if enabled else 0)
+
+    def set_launch(self, round_lanes_per_cu: int = 2048, refill_block: int = 64,
+                   use_graphs: bool = True) -> None:
+        """Launch shape of the transport rounds: lanes per CU of each round's grid,
+        queue indices a wave claims at a time, and whether groups of rounds replay a
+        captured HIP graph. Results do not depend on them (each particle carries its own
+        random stream; tallies are integers); ``pyradmc_amd.autotune`` picks them."""
+        self._ctx.call("hmc_set_launch", int(round_lanes_per_cu), int(refill_block), 1 if use_graphs else 0)
+        self.launch = {"round_lanes_per_cu": int(round_lanes_per_cu), "refill_block": int(refill_block),
+                       "use_graphs": bool(use_graphs)}
+
+    def autotune(self, source, n_histories: int, n_batches: int, seed: int = 1, **kw):
+        """Tune chunk size and launch shape on a timed ``run`` of this workload and apply
+        the best (``pyradmc_amd.autotune.tune_run``); returns the chosen configuration."""
+        from pyradmc_amd import autotune
+
+        return autotune.tune_run(self, source, n_histories, n_batches, seed=seed, **kw)
 
     # --------------------------------------------------------------------- run
 
