@@ -13,8 +13,8 @@ HBM. Data are synthetic. Prints one JSON line.
 Before the warmup, ``pyradmc_amd.autotune`` picks the chunk size and the rounds' launch
 shape for this workload (untimed; no result depends on them; ``--no-autotune`` keeps
 the defaults). The timed steps run with per-kernel HIP events, so their rounds launch
-kernel by kernel; ``graph_replay`` times the same steps unprofiled, where each group of
-four rounds replays a captured HIP graph.
+kernel by kernel; ``unprofiled`` times the same steps without events, where each group
+of four rounds replays a captured HIP graph when the tuned shape has graphs on.
 
 ``roofline`` is for the dominant kernel (the electron round): algorithmic bytes per
 launch - 48 B per particle popped, 48 B per queue push, 5 B per voxel fetch (float32
@@ -302,6 +302,9 @@ def main() -> int:
     ap.add_argument("--plan-cap", type=int, default=4_000_000,
                     help="largest histories per beamlet the full-plan leg runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape and chunk size")
+    ap.add_argument("--launch", default=None,
+                    help="LANES,REFILL: fix the rounds' lanes per CU and refill block instead of tuning "
+                         "(e.g. the shape a tuned run reported, to profile that run's kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
                          "kernel that calibrates the HBM byte counters of a PMC profile)")
@@ -334,7 +337,10 @@ def main() -> int:
     grid, xs, src, kw = workloads.slab_phantom(n_histories=args.histories, n_batches=args.batches)
     engine = HipEngine(grid, xs)
     tuned = None
-    if not args.no_autotune:
+    if args.launch:
+        lanes, refill = (int(v) for v in args.launch.split(","))
+        engine.set_launch(lanes, refill, True)
+    elif not args.no_autotune:
         # Launch shape and chunk size for this workload (untimed; results do not depend on them).
         print("bench: autotune", file=sys.stderr, flush=True)
         tuned = engine.autotune(src, args.histories, args.batches, seed=999, repeats=2, use_cache=False)
@@ -351,8 +357,8 @@ def main() -> int:
     engine.set_profiling(False)
     total = args.histories * args.steps
     value = total / elapsed
-    # The same steps unprofiled: groups of rounds replay the captured HIP graph (the
-    # profiled steps launch kernel by kernel so that each one is timed).
+    # The same steps unprofiled: with graphs on, groups of rounds replay the captured
+    # HIP graph (the profiled steps launch kernel by kernel so that each one is timed).
     graph_counts = 0
     started = time.perf_counter()
     for k in range(args.steps):
@@ -423,7 +429,7 @@ def main() -> int:
             "launch": {**engine.launch, "chunk_size": engine.chunk_size or "auto",
                        "autotuned": tuned is not None, "profiled_steps": "kernel by kernel"},
         },
-        "graph_replay": {"ms_per_step": round(replay_elapsed / args.steps * 1e3, 3),
+        "unprofiled": {"ms_per_step": round(replay_elapsed / args.steps * 1e3, 3),
                          "histories_per_s": round(total / replay_elapsed, 1),
                          "graph_launches": int(graph_counts)},
         "roofline": roofline,

@@ -141,6 +141,107 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
   }
 }
 
+// The same convolution for the full-resolution layers (w == 64, h % 4 == 0, cin 16 or
+// 32, cout 16 x TM): a workgroup computes 4 consecutive h-rows of one z-slice (256
+// positions; wave r takes row r) and first stages the input those rows read - 3 z x
+// 6 h x 66 w positions with their one-voxel halo, zero outside the volume - in LDS
+// (38 KB for cin 16, 76 KB for cin 32). Every tap's operand then comes from LDS
+// instead of a gathered global load: the kernel reads each input element from HBM
+// once per workgroup instead of once per tap. The k-loop, MFMA operands and output
+// are those of k_conv_fwd, so the two kernels give identical sums.
+template <int TM, int CIN>
+__global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict__ x,
+                                                       const __bf16* __restrict__ wt,
+                                                       const float* __restrict__ bias,
+                                                       __bf16* __restrict__ y_nd,
+                                                       __bf16* __restrict__ y_cm, int n, int d, int h,
+                                                       int cout, float slope) {
+  constexpr int kW = 64, kTW = kW + 2, kTH = 6, kTZ = 3, kC8 = CIN / 8;
+  __shared__ u32x4 tile[kTZ * kTH * kTW * kC8];
+  const int rows = h / 4;
+  const int hb = static_cast<int>(blockIdx.x % rows);
+  const long long t = blockIdx.x / rows;
+  const int z = static_cast<int>(t % d);
+  const int nn = static_cast<int>(t / d);
+  const int h0 = 4 * hb;
+  for (int i = threadIdx.x; i < kTZ * kTH * kTW * kC8; i += 256) {
+    const int c8 = i % kC8;
+    const int pos = i / kC8;
+    const int xx = pos % kTW, yy = (pos / kTW) % kTH, zz = pos / (kTW * kTH);
+    const int gz = z + zz - 1, gy = h0 + yy - 1, gx = xx - 1;
+    u32x4 v = u32x4{0u, 0u, 0u, 0u};
+    if (gz >= 0 && gz < d && gy >= 0 && gy < h && gx >= 0 && gx < kW) {
+      const long long q = ((static_cast<long long>(nn) * d + gz) * h + gy) * kW + gx;
+      v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
+    }
+    tile[i] = v;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int row = threadIdx.x >> 6;
+  const int col = lane & 15;
+  const int kg = lane >> 4;
+  f32x4 acc[TM][4];
+#pragma unroll
+  for (int m = 0; m < TM; ++m)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[m][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  constexpr int K = 27 * CIN;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + 8 * kg;
+    bf16x8 a[TM], b[4];
+    if (k < K) {
+      const int tap = k / CIN;
+      const int c = k - tap * CIN;
+      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // halo-shifted: 0..2
+#pragma unroll
+      for (int m = 0; m < TM; ++m)
+        a[m] = load8(wt + (static_cast<long long>(tap) * cout + 16 * m + col) * CIN + c);
+      const int base = ((dz * kTH + row + dy) * kTW + dx + col) * kC8 + c / 8;
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        const u32x4 v = tile[base + 16 * nb * kC8];
+        std::memcpy(&b[nb], &v, 16);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < TM; ++m) a[m] = zero8();
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) b[nb] = zero8();
+    }
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) acc[m][nb] = mfma16(a[m], b[nb], acc[m][nb]);
+  }
+
+  const long long P = static_cast<long long>(n) * d * h * kW;
+  const long long p0 = ((static_cast<long long>(nn) * d + z) * h + h0 + row) * kW;
+#pragma unroll
+  for (int m = 0; m < TM; ++m) {
+    const int cb = 16 * m + 4 * kg;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const long long p = p0 + 16 * nb + col;
+      __bf16 out[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = acc[m][nb][r] + (bias ? bias[cb + r] : 0.0f);
+        out[r] = static_cast<__bf16>(act(v, slope));
+      }
+      uint2 packed;
+      std::memcpy(&packed, out, 8);
+      *reinterpret_cast<uint2*>(y_nd + p * cout + cb) = packed;
+      if (y_cm) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y_cm[static_cast<long long>(cb + r) * P + p] = out[r];
+      }
+    }
+  }
+}
+
 // 8 channel-major elements x[c][row + x0 + dx .. + 7] of one W-row (w % 8 == 0,
 // x0 % 8 == 0), zero outside [0, w): the tap's x shift as a funnel shift of aligned
 // 16-byte loads.
@@ -514,12 +615,26 @@ int hdn_conv3d_fwd(const void* x, const void* wt, const float* bias, void* y_nd,
   if (!x || !wt || !y_nd || !shape_ok(n, d, h, w)) return HDN_EINVAL;
   const long long P = static_cast<long long>(n) * d * h * w;
   if (P % 64 != 0 || cin % 16 != 0 || cin < 16 || cout % 16 != 0 || cout < 16) return HDN_EINVAL;
-  const int tm = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
-  const dim3 grid(static_cast<unsigned int>((P + 255) / 256), static_cast<unsigned int>(cout / (16 * tm)));
   const __bf16* xb = static_cast<const __bf16*>(x);
   const __bf16* wb = static_cast<const __bf16*>(wt);
   __bf16* yb = static_cast<__bf16*>(y_nd);
   __bf16* cb = static_cast<__bf16*>(y_cm);
+  if (w == 64 && h % 4 == 0 && (cin == 16 || cin == 32) && (cout == 16 || (cout == 32 && cin == 16))) {
+    // Full-resolution layers: input staged in LDS (k_conv_fwd_tile).
+    const dim3 tiles(static_cast<unsigned int>(static_cast<long long>(n) * d * (h / 4)));
+    if (cin == 16 && cout == 32)
+      hipLaunchKernelGGL((k_conv_fwd_tile<2, 16>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+                         d, h, cout, slope);
+    else if (cin == 16)
+      hipLaunchKernelGGL((k_conv_fwd_tile<1, 16>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+                         d, h, cout, slope);
+    else
+      hipLaunchKernelGGL((k_conv_fwd_tile<1, 32>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+                         d, h, cout, slope);
+    return launch_status();
+  }
+  const int tm = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+  const dim3 grid(static_cast<unsigned int>((P + 255) / 256), static_cast<unsigned int>(cout / (16 * tm)));
   if (tm == 4)
     hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
                        n, d, h, w, cin, cout, slope);

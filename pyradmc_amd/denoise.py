@@ -600,7 +600,7 @@ def dose_weighted_rms(dose: np.ndarray, reference: np.ndarray) -> float:
     return math.sqrt(float(np.sum(r * (d - r) ** 2) / np.sum(r))) / r_max
 
 
-def evaluate(model: Conv3dDenoiser, pairs: list[ColumnPair], batch: int = 4) -> dict:
+def evaluate(model: Conv3dDenoiser, pairs: list[ColumnPair], batch: int = 32) -> dict:
     """Dose-weighted RMS error against the full-statistics target, per column, of the
     low-history input and of its denoised prediction."""
     noisy_err, den_err = [], []
@@ -647,7 +647,7 @@ class DenoisedDijBuilder:
         self.fallback = int(fallback_histories)
         self.n_batches = int(n_batches)
 
-    def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 8,
+    def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 64,
               group_size: int | None = None):
         """(DijResult, DenoiseReport) for the source's beamlets. Columns go through the
         engine in groups; each group's low-history means stay on the device, its
