@@ -303,7 +303,8 @@ def main() -> int:
                     help="largest histories per beamlet the full-plan leg runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape and chunk size")
     ap.add_argument("--launch", default=None,
-                    help="LANES,REFILL: fix the rounds' lanes per CU and refill block instead of tuning "
+                    help="LANES,REFILL[,PUSH]: fix the rounds' lanes per CU, refill block and push chunk "
+                         "instead of tuning "
                          "(e.g. the shape a tuned run reported, to profile that run's kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run three device-to-device copies of this many MiB (a known-traffic "
@@ -338,8 +339,8 @@ def main() -> int:
     engine = HipEngine(grid, xs)
     tuned = None
     if args.launch:
-        lanes, refill = (int(v) for v in args.launch.split(","))
-        engine.set_launch(lanes, refill, True)
+        shape = [int(v) for v in args.launch.split(",")]
+        engine.set_launch(shape[0], shape[1], shape[2] if len(shape) > 2 else 64, True)
     elif not args.no_autotune:
         # Launch shape and chunk size for this workload (untimed; results do not depend on them).
         print("bench: autotune", file=sys.stderr, flush=True)

@@ -132,13 +132,15 @@ int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
  * reference counterpart: the measurement hook for the per-kernel roofline). */
 int hmc_set_profiling(hmc_ctx *ctx, int enabled);
 /* Launch shape of the transport rounds: lanes per CU of each round's grid (multiple
- * of 256 in [256, 8192]; default 2048), queue indices a wave claims at a time
- * (multiple of 64 in [64, 1024]; default 64), and whether each group of four rounds
- * is replayed from a captured HIP graph (default 1; profiled transports launch one
- * by one, so that every kernel is timed). None of them changes a result.
- * No reference counterpart beyond the chunk_size / queue_factor knobs of WarpEngine
- * (engine.py:369-407): these are the autotuner's knobs. */
-int hmc_set_launch(hmc_ctx *ctx, int round_lanes_per_cu, int refill_block, int use_graphs);
+ * of 256 in [256, 8192]; default 2048), queue indices a wave claims at a time and
+ * output slots it reserves at a time (each a multiple of 64 in [64, 1024]; default
+ * 64), and whether each group of four rounds is replayed from a captured HIP graph
+ * (default 1; profiled transports launch one by one, so that every kernel is
+ * timed). None of them changes a result. No reference counterpart beyond the
+ * chunk_size / queue_factor knobs of WarpEngine (engine.py:369-407): these are the
+ * autotuner's knobs. */
+int hmc_set_launch(hmc_ctx *ctx, int round_lanes_per_cu, int refill_block, int push_chunk,
+                   int use_graphs);
 
 /* Work ----------------------------------------------------------------------- */
 /* Generate and transport every primary of the work items into the dense tallies:

@@ -149,7 +149,7 @@ def lib():
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
         "hmc_set_profiling": (c_int, [ctx_p, c_int]),
-        "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int]),
+        "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int, c_int]),
         "hmc_transport": (c_int, [ctx_p, c_uint64, POINTER(WorkItem), c_int, c_int64, c_int, POINTER(Ledger)]),
         "hmc_fold": (c_int, [ctx_p, c_int, _f64p, POINTER(c_int64)]),
         "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),

@@ -9,6 +9,7 @@ the tallies are integer sums, so dose and Dij are bit-identical under any of the
 * ``round_lanes_per_cu`` - lanes per CU in each round's grid (persistent lanes; the
   resident share runs first, the rest as earlier lanes retire);
 * ``refill_block`` - queue indices a wave claims with one atomic;
+* ``push_chunk`` - output queue slots a wave reserves with one atomic;
 * ``use_graphs`` - replay each group of four rounds from a captured HIP graph.
 
 ``tune`` runs a coordinate descent over candidate values, timing a caller-supplied
@@ -20,6 +21,7 @@ workload pick it up with ``load``.
 from __future__ import annotations
 
 import json
+import math
 import os
 import statistics
 import time
@@ -29,6 +31,7 @@ from pathlib import Path
 DEFAULT_CANDIDATES = {
     "round_lanes_per_cu": (512, 768, 1024, 1536, 2048, 3072),
     "refill_block": (64, 128, 256, 512, 1024),
+    "push_chunk": (64, 128, 256),
     "chunk_size": (None, 4 << 20, 2 << 20),
     "use_graphs": (True, False),
 }
@@ -39,6 +42,7 @@ class LaunchConfig:
     chunk_size: int | None = None  # None: the engine sizes chunks from free memory
     round_lanes_per_cu: int = 2048
     refill_block: int = 64
+    push_chunk: int = 64
     use_graphs: bool = True
 
 
@@ -49,7 +53,7 @@ def cache_path() -> Path:
 
 def apply(engine, config: LaunchConfig) -> None:
     engine.chunk_size = config.chunk_size
-    engine.set_launch(config.round_lanes_per_cu, config.refill_block, config.use_graphs)
+    engine.set_launch(config.round_lanes_per_cu, config.refill_block, config.push_chunk, config.use_graphs)
 
 
 def workload_key(engine, source, n_histories: int, n_batches: int, transport_electrons: bool = True) -> str:
@@ -106,14 +110,19 @@ def tune(engine, work, candidates: dict | None = None, repeats: int = 3, start: 
     def measure(config: LaunchConfig) -> float:
         key = tuple(sorted(asdict(config).items()))
         if key not in measured:
-            apply(engine, config)
-            work()  # warm: allocations, code objects, graph capture
-            times = []
-            for _ in range(repeats):
-                t0 = timer()
-                work()
-                times.append(timer() - t0)
-            measured[key] = statistics.median(times)
+            try:
+                apply(engine, config)
+                work()  # warm: allocations, code objects, graph capture
+                times = []
+                for _ in range(repeats):
+                    t0 = timer()
+                    work()
+                    times.append(timer() - t0)
+                measured[key] = statistics.median(times)
+            except RuntimeError as exc:  # e.g. queue overflow at this shape: not a candidate
+                measured[key] = math.inf
+                if log:
+                    log(f"autotune: {asdict(config)} failed: {exc}")
             trials.append((asdict(config), measured[key]))
             if log:
                 log(f"autotune: {asdict(config)} -> {measured[key] * 1e3:.3f} ms")

@@ -59,6 +59,7 @@ struct Queue {
   unsigned int capacity;
   volatile unsigned int* chunk = nullptr;
   volatile long long* pushed = nullptr;  // this wave's push count (LDS), see chunk_reserve
+  unsigned int push_chunk = 64;          // slots a wave reserves per atomic (multiple of 64)
 };
 
 // Kind tag of a padding record: a slot a producer wave reserved but did not fill.
@@ -216,12 +217,11 @@ struct LaneTally {
 };
 
 // Queue slots for the lanes of a wave that push at one call site. A wave reserves
-// kPushChunk slots at a time with one atomic on the shared count and hands them out in
-// lane order, finishing its current chunk before starting the next, so the only
+// q.push_chunk slots at a time with one atomic on the shared count and hands them out
+// in lane order, finishing its current chunk before starting the next, so the only
 // unfilled slots are each wave's tail at the end of the round (seal_chunk pads them).
 // The chunk state lives in LDS, so the lanes executing a divergent call site always
 // see the wave's current state; only the first active lane reads and writes it.
-constexpr unsigned int kPushChunk = 64;
 
 __device__ inline unsigned int chunk_reserve(const Queue& q) {
   const unsigned long long mask = __ballot(1);
@@ -238,9 +238,9 @@ __device__ inline unsigned int chunk_reserve(const Queue& q) {
     if (cnt <= avail) {
       q.chunk[0] = next + cnt;
     } else {
-      fresh = atomicAdd(q.count, kPushChunk);  // cnt - avail <= 64 = kPushChunk
+      fresh = atomicAdd(q.count, q.push_chunk);  // cnt - avail <= 64 <= q.push_chunk
       q.chunk[0] = fresh + (cnt - avail);
-      q.chunk[1] = fresh + kPushChunk;
+      q.chunk[1] = fresh + q.push_chunk;
     }
   }
   next = __shfl(next, leader);
@@ -1304,6 +1304,7 @@ struct hmc_ctx {
   // launch shape of the transport rounds (hmc_set_launch)
   int round_lanes_per_cu = 2048;
   int refill_block = 64;
+  int push_chunk = 64;
   bool use_graphs = true;
   // the captured group of rounds and what it was captured with
   hipGraphExec_t round_graph = nullptr;
@@ -1734,14 +1735,17 @@ int hmc_set_profiling(hmc_ctx* ctx, int enabled) {
   return HMC_OK;
 }
 
-int hmc_set_launch(hmc_ctx* ctx, int round_lanes_per_cu, int refill_block, int use_graphs) {
+int hmc_set_launch(hmc_ctx* ctx, int round_lanes_per_cu, int refill_block, int push_chunk, int use_graphs) {
   if (!ctx) return HMC_EINVAL;
   if (round_lanes_per_cu < 256 || round_lanes_per_cu > 8192 || round_lanes_per_cu % 256 != 0)
     return fail(ctx, HMC_EINVAL, "round lanes per CU must be a multiple of 256 in [256, 8192]");
   if (refill_block < 64 || refill_block > 1024 || refill_block % 64 != 0)
     return fail(ctx, HMC_EINVAL, "refill block must be a multiple of 64 in [64, 1024]");
+  if (push_chunk < 64 || push_chunk > 1024 || push_chunk % 64 != 0)
+    return fail(ctx, HMC_EINVAL, "push chunk must be a multiple of 64 in [64, 1024]");
   ctx->round_lanes_per_cu = round_lanes_per_cu;
   ctx->refill_block = refill_block;
+  ctx->push_chunk = push_chunk;
   ctx->use_graphs = use_graphs != 0;
   return HMC_OK;
 }
@@ -1761,6 +1765,8 @@ static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned 
   for (int r = 0; r < kRoundsPerSync; ++r) {
     Queue qa_p{ctx->d_q[pa], counts + pa, cap}, qa_e{ctx->d_q[ea], counts + ea, cap};
     Queue qb_p{ctx->d_q[pb], counts + pb, cap}, qb_e{ctx->d_q[eb], counts + eb, cap};
+    const unsigned int pc = static_cast<unsigned int>(ctx->push_chunk);
+    qa_p.push_chunk = qa_e.push_chunk = qb_p.push_chunk = qb_e.push_chunk = pc;
     hipEvent_t* ev = events ? events + 4 * r : nullptr;
     // Photon round: consume pa; electrons append to ea, photons to pb.
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
@@ -1797,7 +1803,7 @@ static int launch_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned i
   k += sizeof(P);
   std::memcpy(k, ctx->d_q, sizeof(ctx->d_q));
   k += sizeof(ctx->d_q);
-  const unsigned int ints[3] = {round_blocks, cap, 0u};
+  const unsigned int ints[3] = {round_blocks, cap, static_cast<unsigned int>(ctx->push_chunk)};
   std::memcpy(k, ints, sizeof(ints));
   k += sizeof(ints);
   const void* ptrs[2] = {ctx->d_counts, ctx->d_ledger};

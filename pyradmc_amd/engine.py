@@ -132,7 +132,7 @@ class HipEngine:
         self._index = _device_index(self.device)
         self._ctx = nat.Context(self._index)
         self._info = nat.device_info(self._index)
-        self.launch = {"round_lanes_per_cu": 2048, "refill_block": 64, "use_graphs": True}
+        self.launch = {"round_lanes_per_cu": 2048, "refill_block": 64, "push_chunk": 64, "use_graphs": True}
 
     def close(self) -> None:
         if self._ctx is not None:
@@ -416,15 +416,17 @@ class HipEngine:
         the result's ``counters``); a measurement hook, off by default."""
         self._ctx.call("hmc_set_profiling", 1 if enabled else 0)
 
-    def set_launch(self, round_lanes_per_cu: int = 2048, refill_block: int = 64,
+    def set_launch(self, round_lanes_per_cu: int = 2048, refill_block: int = 64, push_chunk: int = 64,
                    use_graphs: bool = True) -> None:
         """Launch shape of the transport rounds: lanes per CU of each round's grid,
-        queue indices a wave claims at a time, and whether groups of rounds replay a
-        captured HIP graph. Results do not depend on them (each particle carries its own
-        random stream; tallies are integers); ``pyradmc_amd.autotune`` picks them."""
-        self._ctx.call("hmc_set_launch", int(round_lanes_per_cu), int(refill_block), 1 if use_graphs else 0)
+        queue indices a wave claims at a time, output slots it reserves at a time, and
+        whether groups of rounds replay a captured HIP graph. Results do not depend on
+        them (each particle carries its own random stream; tallies are integers);
+        ``pyradmc_amd.autotune`` picks them."""
+        self._ctx.call("hmc_set_launch", int(round_lanes_per_cu), int(refill_block), int(push_chunk),
+                       1 if use_graphs else 0)
         self.launch = {"round_lanes_per_cu": int(round_lanes_per_cu), "refill_block": int(refill_block),
-                       "use_graphs": bool(use_graphs)}
+                       "push_chunk": int(push_chunk), "use_graphs": bool(use_graphs)}
 
     def autotune(self, source, n_histories: int, n_batches: int, seed: int = 1, **kw):
         """Tune chunk size and launch shape on a timed ``run`` of this workload and apply

@@ -16,8 +16,8 @@ class _FakeEngine:
         self._info = {"cu_count": 256}
         self.grid = SimpleNamespace(shape=(8, 8, 8))
 
-    def set_launch(self, lanes, refill, graphs):
-        self.launch = {"round_lanes_per_cu": lanes, "refill_block": refill, "use_graphs": graphs}
+    def set_launch(self, lanes, refill, push, graphs):
+        self.launch = {"round_lanes_per_cu": lanes, "refill_block": refill, "push_chunk": push, "use_graphs": graphs}
 
 
 def _cost(engine) -> float:
@@ -41,7 +41,7 @@ def test_coordinate_descent_finds_the_separable_minimum():
     config, seconds, trials = autotune.tune(eng, work, candidates=candidates, repeats=2, timer=lambda: clock[0])
     assert (config.round_lanes_per_cu, config.refill_block, config.chunk_size, config.use_graphs) == (3072, 128, 2 << 20, True)
     assert seconds == 1.0
-    assert eng.launch == {"round_lanes_per_cu": 3072, "refill_block": 128, "use_graphs": True}
+    assert eng.launch == {"round_lanes_per_cu": 3072, "refill_block": 128, "push_chunk": 64, "use_graphs": True}
     assert eng.chunk_size == 2 << 20
     configs = [tuple(sorted(c.items())) for c, _ in trials]
     assert len(configs) == len(set(configs))  # each configuration timed once
@@ -51,14 +51,15 @@ def test_coordinate_descent_finds_the_separable_minimum():
 def test_cache_round_trip_applies_the_stored_configuration(tmp_path):
     eng = _FakeEngine()
     path = tmp_path / "autotune.json"
-    cfg = autotune.LaunchConfig(chunk_size=1 << 20, round_lanes_per_cu=1536, refill_block=256, use_graphs=False)
+    cfg = autotune.LaunchConfig(chunk_size=1 << 20, round_lanes_per_cu=1536, refill_block=256, push_chunk=128,
+                                use_graphs=False)
     autotune.store("k1", cfg, 0.5, path)
     autotune.store("k2", autotune.LaunchConfig(), 0.7, path)
     assert autotune.load(eng, "missing", path) is None
     got = autotune.load(eng, "k1", path)
     assert got == cfg
     assert eng.chunk_size == 1 << 20
-    assert eng.launch == {"round_lanes_per_cu": 1536, "refill_block": 256, "use_graphs": False}
+    assert eng.launch == {"round_lanes_per_cu": 1536, "refill_block": 256, "push_chunk": 128, "use_graphs": False}
 
 
 def test_workload_key_separates_what_the_shape_depends_on():
@@ -89,3 +90,19 @@ def test_second_pass_follows_interacting_knobs():
     assert (one.round_lanes_per_cu, one.refill_block) == (2048, 256)
     two, t, _ = autotune.tune(eng, work, candidates=candidates, repeats=1, timer=lambda: clock[0], passes=2)
     assert (two.round_lanes_per_cu, two.refill_block) == (1024, 256) and t == 1.0
+
+
+def test_failing_candidate_is_skipped():
+    """A shape whose run raises (a queue overflow, say) is not a candidate."""
+    eng = _FakeEngine()
+    clock = [0.0]
+
+    def work():
+        if eng.launch["push_chunk"] == 256:
+            raise RuntimeError("secondary queue overflow")
+        clock[0] += 2.0 if eng.launch["push_chunk"] == 64 else 1.0
+
+    config, t, trials = autotune.tune(eng, work, candidates={"push_chunk": (64, 128, 256)}, repeats=1,
+                                      timer=lambda: clock[0])
+    assert config.push_chunk == 128 and t == 1.0
+    assert eng.launch["push_chunk"] == 128

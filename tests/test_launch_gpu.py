@@ -40,7 +40,8 @@ def test_launch_shape_and_graphs_change_no_bit(slab):
     eager = _engine(grid, xs, use_graphs=False).run(src, 300_000, 4, seed=11)
     assert eager.counters["graph_launches"] == 0
     for launch in ({}, {"round_lanes_per_cu": 1024}, {"round_lanes_per_cu": 4096, "refill_block": 256},
-                   {"refill_block": 128, "use_graphs": False}):
+                   {"refill_block": 128, "use_graphs": False}, {"round_lanes_per_cu": 1024, "push_chunk": 256},
+                   {"refill_block": 512, "push_chunk": 128, "use_graphs": False}):
         got = _engine(grid, xs, **launch).run(src, 300_000, 4, seed=11)
         _same(eager, got)
         assert (got.counters["graph_launches"] > 0) == launch.get("use_graphs", True), launch
@@ -78,7 +79,7 @@ def test_launch_arguments_rejected(slab):
     grid, xs, _ = slab
     eng = _engine(grid, xs)
     for bad in ({"round_lanes_per_cu": 100}, {"round_lanes_per_cu": 16384}, {"refill_block": 96},
-                {"refill_block": 2048}):
+                {"refill_block": 2048}, {"push_chunk": 32}, {"push_chunk": 100}):
         with pytest.raises(ValueError):
             eng.set_launch(**bad)
 
