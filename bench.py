@@ -53,6 +53,16 @@ def _algorithmic_bytes(c: dict, prefix: str) -> float:
             + 5.0 * c[f"{prefix}_voxels"] + 16.0 * c[f"{prefix}_deposits"])
 
 
+EVENT_KEYS = ("photon_steps", "electron_steps", "photon_voxels", "electron_voxels", "photon_deposits",
+              "electron_deposits", "photon_pushes", "electron_pushes")
+
+
+def _events(counters: dict, histories: float) -> dict:
+    """Per-history event counts from a run's ledger counters: the denominators of the
+    algorithmic bytes (5 B per voxel fetch, 16 B per tally atomic, 48 B per push and pop)."""
+    return {k: round(counters.get(k, 0) / histories, 3) for k in EVENT_KEYS}
+
+
 def _per_beamlet_sigma(d) -> np.ndarray:
     """Median over each column's voxels with D >= 0.5 max of sigma/D."""
     out = np.zeros(d.n_beamlets)
@@ -95,6 +105,7 @@ def _dij_leg(median: float) -> dict:
         # Sigma scales as N^-1/2 at fixed layout: projected wall time for every beamlet's
         # median sigma at 2 percent, from the worst (90th percentile) column.
         "wall_s_at_2pct_projected": round(wall * (float(np.nanpercentile(sig, 90)) / 0.02) ** 2, 4),
+        "events_per_history": _events(d.counters or {}, float(counts.sum())),
     }
 
 
@@ -178,12 +189,18 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[di
     engine = HipEngine(grid, xs)
     engine.run_dij(beams[0], 10 * nb, nb, seed=1, truncation=trunc)  # warm
 
+    events: dict = {}
+
     def plan(n: int, seed: int):
         started = time.perf_counter()
         parts = [engine.run_dij(b, n, nb, seed=seed + i, truncation=trunc, value_dtype=np.float32)
                  for i, b in enumerate(beams)]
-        d = workloads.hstack_dij(parts)
-        return d, time.perf_counter() - started
+        elapsed = time.perf_counter() - started
+        events.clear()
+        for part in parts:
+            for k, v in (part.counters or {}).items():
+                events[k] = events.get(k, 0) + v
+        return workloads.hstack_dij(parts), elapsed
 
     probe, probe_wall = plan(probe_n, kw["seed"])
     p90 = float(np.nanpercentile(_per_beamlet_sigma(probe), 90))
@@ -208,6 +225,7 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[di
         "nnz": int(d.indptr[-1]),
         "per_beamlet_sigma_median": round(float(np.nanmedian(sig)), 5),
         "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
+        "events_per_history": _events(events, float(n_run) * d.n_beamlets),
     }
     dev = spmm.DeviceDij(d)
     rng = np.random.default_rng(kw["seed"])
@@ -438,11 +456,7 @@ def main() -> int:
         "dij": dij,
         "denoised_dij": denoised,
         "full_plan": plan,
-        "events_per_history": {
-            k: round(counters[k] / total, 3)
-            for k in ("photon_steps", "electron_steps", "photon_voxels", "electron_voxels",
-                      "photon_deposits", "electron_deposits", "photon_pushes", "electron_pushes")
-        },
+        "events_per_history": _events(counters, total),
     }
     engine.close()
     print(json.dumps(line), flush=True)
