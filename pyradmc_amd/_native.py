@@ -4,6 +4,11 @@ Loads the in-tree library and fails loudly when it is missing: the engine has no
 CPU fallback. Status codes map onto the exception types the reference engine raises
 (ValueError for bad arguments, RuntimeError for in-kernel failures and call order,
 MemoryError for device allocation).
+
+The PyTorch wheel carries its own HIP runtime. A process that loads this library first
+and torch afterwards ends up with two runtimes, and torch then sees no device. So when
+torch is installed, it is imported before the library is loaded: the library then binds
+to torch's runtime and the two share devices, streams and memory.
 """
 
 from __future__ import annotations
@@ -129,6 +134,10 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401  (one HIP runtime per process; see the module docstring)
+    except ImportError:
+        pass
     if not LIB_PATH.is_file():
         raise RuntimeError(
             f"native library {LIB_PATH} is missing: build it with "

@@ -57,10 +57,30 @@ struct Queue {
   Particle* data;
   unsigned int* count;
   unsigned int capacity;
-  volatile unsigned int* chunk = nullptr;
-  volatile long long* pushed = nullptr;  // this wave's push count (LDS), see chunk_reserve
-  unsigned int push_chunk = 64;          // slots a wave reserves per atomic (multiple of 64)
+  unsigned int push_chunk = 64;  // slots a wave reserves per atomic (multiple of 64)
+  int lds = -1;                  // which of the wave's two output-chunk states (LDS) is this queue's
 };
+
+// Per-wave state of a round kernel in LDS, at file scope so that the device functions
+// address it directly (LDS instructions; a pointer to it kept in a kernel argument
+// would become a generic address): the two output queues' current chunks
+// [wave][queue][next, end], and the block's ledger reduction slots [book][wave]
+// (books 6 and 7: the wave's pops and pushes).
+__shared__ unsigned int g_chunk[4][2][2];
+__shared__ long long g_books[8 * 4];
+
+// Accesses to that state: relaxed atomics at wave scope, so that every access is a real
+// LDS access (lanes of divergent branches see one another's updates) and the compiler
+// still knows the address space.
+__device__ inline unsigned int lds_load(unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ inline void lds_store(unsigned int* p, unsigned int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ inline void lds_add(long long* p, long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 
 // Kind tag of a padding record: a slot a producer wave reserved but did not fill.
 constexpr uint32_t kEmptyKind = 3u;
@@ -232,15 +252,17 @@ __device__ inline unsigned int chunk_reserve(const Queue& q) {
       static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
   unsigned int next = 0u, avail = 0u, fresh = 0u;
   if (lane == leader) {
-    q.pushed[0] += cnt;
-    next = q.chunk[0];
-    avail = q.chunk[1] - next;
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    unsigned int* chunk = g_chunk[wave][q.lds];
+    lds_add(&g_books[7 * 4 + wave], cnt);
+    next = lds_load(chunk);
+    avail = lds_load(chunk + 1) - next;
     if (cnt <= avail) {
-      q.chunk[0] = next + cnt;
+      lds_store(chunk, next + cnt);
     } else {
       fresh = atomicAdd(q.count, q.push_chunk);  // cnt - avail <= 64 <= q.push_chunk
-      q.chunk[0] = fresh + (cnt - avail);
-      q.chunk[1] = fresh + q.push_chunk;
+      lds_store(chunk, fresh + (cnt - avail));
+      lds_store(chunk + 1, fresh + q.push_chunk);
     }
   }
   next = __shfl(next, leader);
@@ -252,18 +274,20 @@ __device__ inline unsigned int chunk_reserve(const Queue& q) {
 // Pad the wave's unfilled reserved slots with empty records. Every lane of the wave
 // calls this after its round loop.
 __device__ inline void seal_chunk(const Queue& q) {
-  const unsigned int next = q.chunk[0], end = q.chunk[1];
+  unsigned int* chunk = g_chunk[threadIdx.x >> 6][q.lds];
+  const unsigned int next = lds_load(chunk), end = lds_load(chunk + 1);
   for (unsigned int sl = next + __lane_id(); sl < end; sl += 64u)
     if (sl < q.capacity) q.data[sl].meta = kEmptyKind;
 }
 
 // Add the lanes (of those executing the call) that popped a real particle to the
 // wave's pop count in LDS.
-__device__ inline void count_pops(volatile long long* wave_pops, bool real) {
+__device__ inline void count_pops(bool real) {
   const unsigned long long mask = __ballot(real ? 1 : 0);
   const unsigned long long active = __ballot(1);
-  if (static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(active)) - 1)
-    wave_pops[0] += __popcll(mask);
+  if (static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(active)) - 1) {
+    lds_add(&g_books[6 * 4 + (threadIdx.x >> 6)], __popcll(mask));
+  }
 }
 
 // Append one particle (chunk_reserve over the lanes executing the call).
@@ -423,18 +447,11 @@ k_photon_round(TransportParams P, Queue qin,
 
   // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
   // ledger reduction slots.
-  __shared__ unsigned int chunk_state[4][2][2];
-  __shared__ long long ledger_red[8 * 4];  // [book][wave]; books 6, 7 = pops, pushes
   const int wave_in_block = static_cast<int>(threadIdx.x >> 6);
-  {
-    if (__lane_id() < 4) chunk_state[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
-    if (__lane_id() < 2) ledger_red[(6 + __lane_id()) * 4 + wave_in_block] = 0;
-    qe.chunk = chunk_state[wave_in_block][0];
-    qp.chunk = chunk_state[wave_in_block][1];
-    qe.pushed = &ledger_red[7 * 4 + wave_in_block];
-    qp.pushed = &ledger_red[7 * 4 + wave_in_block];
-  }
-  volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
+  if (__lane_id() < 4) g_chunk[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
+  if (__lane_id() < 2) g_books[(6 + __lane_id()) * 4 + wave_in_block] = 0;
+  qe.lds = 0;
+  qp.lds = 1;
   RefillPool pool(P.refill_block);
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
@@ -443,7 +460,7 @@ k_photon_round(TransportParams P, Queue qin,
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
       const bool real = (meta & 3u) != kEmptyKind;  // else padding left by a producer wave
-      count_pops(wave_pops, real);
+      count_pops(real);
       if (!real) continue;
       column = meta >> 3;
       base = static_cast<long long>(column) * P.s.n_voxels;
@@ -579,7 +596,7 @@ k_photon_round(TransportParams P, Queue qin,
   seal_chunk(qe);
   seal_chunk(qp);
   tal.flush_pending(P.edep);
-  tal.flush(P.ledger, L_P_VOX, ledger_red);
+  tal.flush(P.ledger, L_P_VOX, g_books);
 }
 
 // --------------------------------------------------------------- electron round
@@ -604,18 +621,11 @@ k_electron_round(TransportParams P, Queue qin,
 
   // Per-wave chunk state of the two output queues (chunk_reserve), and the block's
   // ledger reduction slots.
-  __shared__ unsigned int chunk_state[4][2][2];
-  __shared__ long long ledger_red[8 * 4];  // [book][wave]; books 6, 7 = pops, pushes
   const int wave_in_block = static_cast<int>(threadIdx.x >> 6);
-  {
-    if (__lane_id() < 4) chunk_state[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
-    if (__lane_id() < 2) ledger_red[(6 + __lane_id()) * 4 + wave_in_block] = 0;
-    qp.chunk = chunk_state[wave_in_block][0];
-    qe.chunk = chunk_state[wave_in_block][1];
-    qp.pushed = &ledger_red[7 * 4 + wave_in_block];
-    qe.pushed = &ledger_red[7 * 4 + wave_in_block];
-  }
-  volatile long long* wave_pops = &ledger_red[6 * 4 + wave_in_block];
+  if (__lane_id() < 4) g_chunk[wave_in_block][__lane_id() >> 1][__lane_id() & 1] = 0u;
+  if (__lane_id() < 2) g_books[(6 + __lane_id()) * 4 + wave_in_block] = 0;
+  qp.lds = 0;
+  qe.lds = 1;
   RefillPool pool(P.refill_block);
   while (true) {
     const unsigned int i = refill_index(pool, head, !active);
@@ -624,7 +634,7 @@ k_electron_round(TransportParams P, Queue qin,
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
       const bool real = (meta & 3u) != kEmptyKind;  // else padding left by a producer wave
-      count_pops(wave_pops, real);
+      count_pops(real);
       if (!real) continue;
       column = meta >> 3;
       positron = (meta & 3u) == POSITRON;
@@ -797,7 +807,7 @@ k_electron_round(TransportParams P, Queue qin,
   seal_chunk(qp);
   seal_chunk(qe);
   tal.flush_pending(P.edep);
-  tal.flush(P.ledger, L_E_VOX, ledger_red);
+  tal.flush(P.ledger, L_E_VOX, g_books);
 }
 
 // -------------------------------------------------------------------- generate
