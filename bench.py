@@ -150,9 +150,9 @@ def _denoise_leg(model_path: Path, low: int, target: float) -> dict:
     plain_err = _window_errors(plain, ref, src, grid)
     builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, tolerance=target,
                                          fallback_histories=n_plain, n_batches=nb)
-    builder.build(src, seed=11)  # warm (kernels, allocator)
+    builder.build(src, seed=11, value_dtype=np.float32)  # warm (kernels, allocator)
     started = time.perf_counter()
-    den, report = builder.build(src, seed=12, truncation=1e-4)
+    den, report = builder.build(src, seed=12, truncation=1e-4, value_dtype=np.float32)
     wall_den = time.perf_counter() - started
     den_err = _window_errors(den, ref, src, grid)
     engine.close()

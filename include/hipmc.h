@@ -161,6 +161,10 @@ int hmc_finalize(hmc_ctx *ctx, int n_batches, double *mean, double *sigma);
  * host prefix (engine.py:1496-1557). counts[n_columns] out; entries fetched next. */
 int hmc_truncate(hmc_ctx *ctx, double truncation, int64_t *counts, int64_t *total);
 int hmc_fetch_csc(hmc_ctx *ctx, int64_t *indices, double *dose, double *sigma);
+/* The same entries with float32 dose and sigma (rounded to nearest on the device), for a
+ * float32 CSC at half the bytes over PCIe; the reference's DijAssembler.finalize
+ * (pyradmc/scoring/dij.py:397-434) converts its float64 columns on the host. */
+int hmc_fetch_csc_f32(hmc_ctx *ctx, int64_t *indices, float *dose, float *sigma);
 
 /* Beam-aligned windows of the finalized columns, for the learned denoiser (no
  * counterpart in the reference; the denoised Dij builder in pyradmc_amd/denoise.py).

@@ -164,6 +164,7 @@ def lib():
         "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),
         "hmc_truncate": (c_int, [ctx_p, c_double, POINTER(c_int64), POINTER(c_int64)]),
         "hmc_fetch_csc": (c_int, [ctx_p, POINTER(c_int64), _f64p, _f64p]),
+        "hmc_fetch_csc_f32": (c_int, [ctx_p, POINTER(c_int64), _f32p, _f32p]),
         "hmc_reset_maps": (c_int, [ctx_p]),
         "hmc_set_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_get_quanta": (c_int, [ctx_p, POINTER(c_int64)]),

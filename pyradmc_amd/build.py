@@ -36,6 +36,13 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
+# Transport kernels only: float32 division and square root within HIP's 2.5 / 1 ulp
+# bounds (reciprocal-based) instead of correctly rounded sequences. Sampling and
+# geometry stay well inside their statistical tolerance, results stay identical from
+# run to run, and the round kernels run 3-4 % faster. The float64 fold, finalize and
+# truncation arithmetic, and the other libraries' kernels, are not affected.
+TRANSPORT_FLAGS = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+
 
 def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
@@ -64,7 +71,7 @@ def build(verbose: bool = False) -> Path:
     spmm_o = build_dir / "spmm_kernels.o"
     gs_o = build_dir / "gs_build.o"
     _run(
-        [hipcc, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c",
+        [hipcc, *HIP_FLAGS, *TRANSPORT_FLAGS, "-I", str(ROOT / "include"), "-c",
          str(CSRC / "hipmc_kernels.hip"), "-o", str(kernels_o)],
         verbose,
     )

@@ -1027,6 +1027,17 @@ __global__ void k_reset_consumed(unsigned int* counts, unsigned int* heads, int 
   }
 }
 
+// Float32 copies of the compacted dose and sigma, rounded to nearest (as a float64 array's
+// conversion to float32 on the host), so that a float32 CSC crosses PCIe at half the bytes.
+__global__ void k_narrow_f32(const double* __restrict__ dose, const double* __restrict__ sigma, long long n,
+                             float* __restrict__ out) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    out[i] = static_cast<float>(dose[i]);
+    out[n + i] = static_cast<float>(sigma[i]);
+  }
+}
+
 // Beam-aligned windows of finalized columns, for the denoiser: out[c][z][x][y] =
 // mean[c][((x0_c + x) * ny + (y0_c + y)) * nz + z] as float (x < wx, y < wy, all z).
 __global__ void k_gather_windows(const double* __restrict__ mean, const int* __restrict__ origins,
@@ -1327,6 +1338,7 @@ struct hmc_ctx {
   long long* d_out_idx = nullptr;
   double* d_out_dose = nullptr;
   double* d_out_sigma = nullptr;
+  float* d_out32 = nullptr;  // float32 copies of the compacted dose and sigma (hmc_fetch_csc_f32)
   long long trunc_total = 0;
   size_t out_cap = 0;
 };
@@ -1434,7 +1446,8 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_up,           ctx->d_q[0],        ctx->d_q[1],       ctx->d_q[2],
                   ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
                   ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
-                  ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins};
+                  ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins,
+                  ctx->d_out32};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
@@ -2163,6 +2176,25 @@ int hmc_scatter_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, in
   HMC_HIP(ctx, hipGetLastError());
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->finalized = true;  // s1 / s2 now hold mean / sigma for hmc_truncate
+  return HMC_OK;
+}
+
+int hmc_fetch_csc_f32(hmc_ctx* ctx, int64_t* indices, float* dose, float* sigma) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  const long long n = ctx->trunc_total;
+  if (n == 0) return HMC_OK;
+  if (!indices || !dose || !sigma) return fail(ctx, HMC_EINVAL, "null output");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_out32, static_cast<size_t>(2 * n)))) return rc;
+  hipLaunchKernelGGL(k_narrow_f32, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, ctx->stream, ctx->d_out_dose,
+                     ctx->d_out_sigma, n, ctx->d_out32);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipMemcpyAsync(indices, ctx->d_out_idx, sizeof(long long) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HMC_HIP(ctx, hipMemcpyAsync(dose, ctx->d_out32, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipMemcpyAsync(sigma, ctx->d_out32 + n, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HMC_OK;
 }
 

@@ -648,18 +648,22 @@ class DenoisedDijBuilder:
         self.n_batches = int(n_batches)
 
     def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 64,
-              group_size: int | None = None):
+              group_size: int | None = None, value_dtype=np.float64):
         """(DijResult, DenoiseReport) for the source's beamlets. Columns go through the
         engine in groups; each group's low-history means stay on the device, its
         windows are gathered straight into the network's input buffers, and the
         denoised means and sigmas are written back for the engine's own truncation
-        and CSC compaction."""
+        and CSC compaction. ``value_dtype`` is the CSC dose and sigma type, float64 or
+        float32, as in ``HipEngine.run_dij``."""
         import time
 
         from pyradmc_amd import SpectralBeamletSource
         from pyradmc_amd.scoring import DijResult
 
         torch = _torch()
+        value_dtype = np.dtype(value_dtype)
+        if value_dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ValueError(f"value_dtype must be float64 or float32, got {value_dtype}")
         t0 = time.perf_counter()
         eng, model = self.engine, self.model
         grid = eng.grid
@@ -725,11 +729,13 @@ class DenoisedDijBuilder:
                           ctypes.byref(total))
             mtot = int(total.value)
             idx = np.empty(mtot, dtype=np.int64)
-            dose = np.empty(mtot, dtype=np.float64)
-            sig = np.empty(mtot, dtype=np.float64)
+            dose = np.empty(mtot, dtype=value_dtype)
+            sig = np.empty(mtot, dtype=value_dtype)
             if mtot:
-                eng._ctx.call("hmc_fetch_csc", _native.ptr(idx, _native.c_int64), _native.ptr(dose, ctypes.c_double),
-                              _native.ptr(sig, ctypes.c_double))
+                narrow = value_dtype == np.dtype(np.float32)
+                vt = ctypes.c_float if narrow else ctypes.c_double
+                eng._ctx.call("hmc_fetch_csc_f32" if narrow else "hmc_fetch_csc", _native.ptr(idx, _native.c_int64),
+                              _native.ptr(dose, vt), _native.ptr(sig, vt))
             col_counts.append(kept)
             col_idx.append(idx)
             col_dose.append(dose)
@@ -748,7 +754,8 @@ class DenoisedDijBuilder:
             redo = np.flatnonzero(fallback).tolist()
             sub = SpectralBeamletSource(source.spectrum, source.focal_point, [source.centers[j] for j in redo],
                                         source.width_u, source.width_v, source.u_axis, source.v_axis)
-            mc = eng.run_dij(sub, self.fallback, self.n_batches, seed=seed + 2, truncation=truncation)
+            mc = eng.run_dij(sub, self.fallback, self.n_batches, seed=seed + 2, truncation=truncation,
+                             value_dtype=value_dtype)
             parts_i, parts_d, parts_s, new_counts = [], [], [], kept_all.copy()
             slot = {j: k for k, j in enumerate(redo)}
             for j in range(n_cols):

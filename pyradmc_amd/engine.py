@@ -88,6 +88,13 @@ _BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source"}
 _UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
 
 
+def _joined(parts: list, dtype) -> np.ndarray:
+    """One array from the per-group pieces, without a copy when there is one group."""
+    if not parts:
+        return np.zeros(0, dtype)
+    return parts[0] if len(parts) == 1 else np.concatenate(parts)
+
+
 def _in_kernel(source) -> bool:
     """Exact-type routing (pyradmc/backends/warp/engine.py:523-531): only the built-in
     source types themselves get the in-kernel generator; any other source, a subclass
@@ -604,6 +611,7 @@ class HipEngine:
         value_dtype = np.dtype(value_dtype)
         if value_dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
             raise ValueError(f"value_dtype must be float64 or float32, got {value_dtype}")
+        narrow = value_dtype == np.dtype(np.float32)
         n_beamlets = int(source.n_beamlets)
         counts = np.asarray(n_histories_per_beamlet, dtype=np.int64)
         uneven = counts.ndim == 1
@@ -664,10 +672,18 @@ class HipEngine:
             self._ctx.call("hmc_truncate", float(truncation), nat.ptr(kept, nat.c_int64), nat.ctypes.byref(total))
             m = int(total.value)
             idx = np.empty(m, dtype=np.int64)
-            dose = np.empty(m, dtype=np.float64)
-            sig = np.empty(m, dtype=np.float64)
-            if m:
-                self._ctx.call("hmc_fetch_csc", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.c_double), nat.ptr(sig, nat.c_double))
+            if narrow:  # converted on the device: half the value bytes over PCIe
+                dose = np.empty(m, dtype=np.float32)
+                sig = np.empty(m, dtype=np.float32)
+                if m:
+                    self._ctx.call("hmc_fetch_csc_f32", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.ctypes.c_float),
+                                   nat.ptr(sig, nat.ctypes.c_float))
+            else:
+                dose = np.empty(m, dtype=np.float64)
+                sig = np.empty(m, dtype=np.float64)
+                if m:
+                    self._ctx.call("hmc_fetch_csc", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.c_double),
+                                   nat.ptr(sig, nat.c_double))
             col_counts.append(kept)
             col_idx.append(idx)
             col_dose.append(dose.astype(value_dtype, copy=False))
@@ -690,9 +706,9 @@ class HipEngine:
             n_batches=n_batches,
             truncation=float(truncation),
             indptr=indptr,
-            indices=np.concatenate(col_idx) if col_idx else np.zeros(0, np.int64),
-            dose=np.concatenate(col_dose) if col_dose else np.zeros(0, value_dtype),
-            sigma=np.concatenate(col_sigma) if col_sigma else np.zeros(0, value_dtype),
+            indices=_joined(col_idx, np.int64),
+            dose=_joined(col_dose, value_dtype),
+            sigma=_joined(col_sigma, value_dtype),
             energy_emitted=energy_emitted,
             energy_deposited=energy_deposited,
             energy_escaped=float(books.get("escaped", 0)) * ENERGY_QUANTUM_MEV,

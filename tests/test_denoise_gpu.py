@@ -323,3 +323,25 @@ def test_denoised_builder_falls_back_to_plain_monte_carlo(dn):
     np.testing.assert_array_equal(dij.sigma, plain.sigma)
     assert abs(dij.energy_emitted - dij.energy_deposited - dij.energy_escaped - dij.energy_unscored) \
         < 1e-6 * dij.energy_emitted
+
+
+def test_denoised_builder_float32_values_are_the_float64_values_rounded(dn):
+    # No fallbacks (an unbounded error budget): the same seed gives the same denoised
+    # columns, and a float32 CSC holds the float64 one rounded to nearest.
+    from pyradmc_amd import HipEngine
+
+    grid, xs, src = _small_ct(dn)
+    eng = HipEngine(grid, xs)
+    builder = dn.DenoisedDijBuilder(eng, dn.Conv3dDenoiser(seed=5), low_histories=4_000, tolerance=1e9,
+                                    fallback_histories=40_000, n_batches=4)
+    wide, report = builder.build(src, seed=31, truncation=1e-4)
+    narrow, _ = builder.build(src, seed=31, truncation=1e-4, value_dtype=np.float32)
+    with pytest.raises(ValueError):
+        builder.build(src, seed=31, value_dtype=np.float16)
+    eng.close()
+    assert not report.fallback.any() and wide.indptr[-1] > 0
+    assert wide.dose.dtype == np.float64 and narrow.dose.dtype == np.float32
+    np.testing.assert_array_equal(narrow.indptr, wide.indptr)
+    np.testing.assert_array_equal(narrow.indices, wide.indices)
+    np.testing.assert_array_equal(narrow.dose, wide.dose.astype(np.float32))
+    np.testing.assert_array_equal(narrow.sigma, wide.sigma.astype(np.float32))

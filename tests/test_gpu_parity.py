@@ -368,8 +368,18 @@ def test_ct_dij_three_schedules_bit_identical():
         runs.append(eng.run_dij(src, counts, kw["n_batches"], seed=kw["seed"], truncation=kw["truncation"],
                                 beamlet_group_size=group, value_dtype=np.float32))
         eng.close()
+    # float32 values are narrowed on the device: the same numbers as the float64
+    # CSC converted on the host (round to nearest).
+    eng = HipEngine(grid, xs, chunk_size=1 << 20)
+    wide = eng.run_dij(src, counts, kw["n_batches"], seed=kw["seed"], truncation=kw["truncation"],
+                       beamlet_group_size=17)
+    eng.close()
     a = runs[0]
     assert a.dose.dtype == np.float32 and a.indptr[-1] > 0
+    assert wide.dose.dtype == np.float64
+    np.testing.assert_array_equal(a.indices, wide.indices)
+    np.testing.assert_array_equal(a.dose, wide.dose.astype(np.float32))
+    np.testing.assert_array_equal(a.sigma, wide.sigma.astype(np.float32))
     for b in runs[1:]:
         np.testing.assert_array_equal(a.indptr, b.indptr)
         np.testing.assert_array_equal(a.indices, b.indices)
