@@ -75,15 +75,27 @@ def _per_beamlet_sigma(d) -> np.ndarray:
     return out
 
 
-def _dij_leg(median: float) -> dict:
+def _shaped_engine(grid, xs, launch: dict | None):
+    """An engine with the slab-phantom step's launch shape (the chunk size stays
+    automatic: it follows each workload's size)."""
+    from pyradmc_amd import HipEngine
+
+    engine = HipEngine(grid, xs)
+    if launch:
+        engine.set_launch(launch["round_lanes_per_cu"], launch["refill_block"], launch["push_chunk"],
+                          launch["use_graphs"])
+    return engine
+
+
+def _dij_leg(median: float, launch: dict | None = None) -> dict:
     """The batched uneven Dij: 128x128x96 CT-like phantom at 2.5 mm, 100 spectral
     beamlets of 5x5 mm from 100 cm SAD, lognormal per-beamlet counts, truncation 1e-4,
     float32 CSC values - one complete run_dij, timed wall to wall (CSC on the host)."""
-    from pyradmc_amd import HipEngine, workloads
+    from pyradmc_amd import workloads
 
     grid, xs, src, kw = workloads.ct_like_dij(median=median)
     counts = kw["n_histories_per_beamlet"]
-    engine = HipEngine(grid, xs)
+    engine = _shaped_engine(grid, xs, launch)
     small = np.maximum(kw["n_batches"], counts // 100 // kw["n_batches"] * kw["n_batches"])
     engine.run_dij(src, small, kw["n_batches"], seed=1, truncation=kw["truncation"])  # warm
     started = time.perf_counter()
@@ -106,6 +118,7 @@ def _dij_leg(median: float) -> dict:
         # median sigma at 2 percent, from the worst (90th percentile) column.
         "wall_s_at_2pct_projected": round(wall * (float(np.nanpercentile(sig, 90)) / 0.02) ** 2, 4),
         "events_per_history": _events(d.counters or {}, float(counts.sum())),
+        "seconds": _phase_seconds(d.counters or {}),
     }
 
 
@@ -123,19 +136,19 @@ def _window_errors(d, ref, source, grid) -> np.ndarray:
     return out
 
 
-def _denoise_leg(model_path: Path, low: int, target: float) -> dict:
+def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None = None) -> dict:
     """The 100-beamlet CT-like Dij at a dose-weighted RMS error target against a
     100x-history Monte Carlo reference: plain Monte Carlo (histories per beamlet set
     from a measured error at 1e5 and sigma ~ N^-1/2, then run and timed) versus
     low-history Monte Carlo plus the trained denoiser with plain Monte Carlo fallback.
     Held-out phantom (a seed never used in training)."""
-    from pyradmc_amd import HipEngine, denoise, workloads
+    from pyradmc_amd import denoise, workloads
     from pyradmc_amd.data.analytic import AnalyticCrossSections
 
     grid = workloads.ct_like_grid(seed=20261111)
     xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
     src = workloads.ct_like_beamlets()
-    engine = HipEngine(grid, xs)
+    engine = _shaped_engine(grid, xs, launch)
     model = denoise.load(model_path)
     nb = 10
     engine.run_dij(src, 10 * nb, nb, seed=1, truncation=1e-4)  # warm
@@ -174,7 +187,16 @@ def _denoise_leg(model_path: Path, low: int, target: float) -> dict:
     }
 
 
-def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[dict, object]:
+def _phase_seconds(counters: dict) -> dict:
+    """run_dij's host wall split: transport (with fold and finalize) and CSC assembly
+    (truncation, device-to-host copy, join), summed over calls; and the column groups."""
+    return {"transport": round(float(counters.get("transport_s", 0.0)), 4),
+            "assemble": round(float(counters.get("assemble_s", 0.0)), 4),
+            "groups": int(counters.get("groups", 0))}
+
+
+def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int,
+              launch: dict | None = None) -> tuple[dict, object]:
     """The full plan: 256x256x128 CT-like phantom at 2 mm, five gantry beams of 15x15
     spectral beamlets (1125 columns). A probe Dij at ``probe_n`` histories per beamlet
     gives each beamlet's sigma; sigma ~ N^-1/2 then sets the histories that bring the
@@ -182,11 +204,11 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[di
     wall (five run_dij calls, float32 CSC on the host). Then the planning product for
     ``n_vectors`` seeded U[0,1) weight vectors on the device, float64 (bit-exact with
     dose_for_weights) and float32, timed with HIP events around the kernel."""
-    from pyradmc_amd import HipEngine, spmm, workloads
+    from pyradmc_amd import spmm, workloads
 
     grid, xs, beams, kw = workloads.full_plan()
     nb, trunc = kw["n_batches"], kw["truncation"]
-    engine = HipEngine(grid, xs)
+    engine = _shaped_engine(grid, xs, launch)
     engine.run_dij(beams[0], 10 * nb, nb, seed=1, truncation=trunc)  # warm
 
     events: dict = {}
@@ -226,6 +248,7 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int) -> tuple[di
         "per_beamlet_sigma_median": round(float(np.nanmedian(sig)), 5),
         "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
         "events_per_history": _events(events, float(n_run) * d.n_beamlets),
+        "seconds": _phase_seconds(events),
     }
     dev = spmm.DeviceDij(d)
     rng = np.random.default_rng(kw["seed"])
@@ -410,11 +433,13 @@ def main() -> int:
             counters["electron_kernel_ms"] / max(counters["electron_kernel_ms"] + counters["photon_kernel_ms"], 1e-9), 4),
     }
 
-    dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median)
-    denoised = _guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02) if denoise_leg else None
+    shape = dict(engine.launch)  # the legs below run at the step's launch shape
+    dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median, shape)
+    denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02, shape)
+                if denoise_leg else None)
     plan, plan_dij = None, None
     if not args.no_plan:
-        got = _guarded("full_plan", _plan_leg, 0.02, args.plan_probe, args.plan_vectors, args.plan_cap)
+        got = _guarded("full_plan", _plan_leg, 0.02, args.plan_probe, args.plan_vectors, args.plan_cap, shape)
         plan, plan_dij = got if isinstance(got, tuple) else (got, None)
 
     cpu = None

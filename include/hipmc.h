@@ -166,6 +166,15 @@ int hmc_fetch_csc(hmc_ctx *ctx, int64_t *indices, double *dose, double *sigma);
  * (pyradmc/scoring/dij.py:397-434) converts its float64 columns on the host. */
 int hmc_fetch_csc_f32(hmc_ctx *ctx, int64_t *indices, float *dose, float *sigma);
 
+/* Page-locked host memory for the fetched CSC arrays (no counterpart in the reference,
+ * whose DijAssembler.finalize, pyradmc/scoring/dij.py:397-434, returns ordinary host
+ * arrays). A device-to-host copy into it runs at the link's rate; into freshly allocated
+ * pageable memory it is bound by first-touch page faults at a fraction of that.
+ * hmc_host_alloc returns HMC_ENOMEM when the allocation fails; hmc_host_free(NULL) is a
+ * no-op. */
+int hmc_host_alloc(size_t bytes, void **out);
+void hmc_host_free(void *ptr);
+
 /* Beam-aligned windows of the finalized columns, for the learned denoiser (no
  * counterpart in the reference; the denoised Dij builder in pyradmc_amd/denoise.py).
  * origins: host int32 [n_cols][2] (x0, y0), n_cols equal to the scoring column count;

@@ -165,6 +165,8 @@ def lib():
         "hmc_truncate": (c_int, [ctx_p, c_double, POINTER(c_int64), POINTER(c_int64)]),
         "hmc_fetch_csc": (c_int, [ctx_p, POINTER(c_int64), _f64p, _f64p]),
         "hmc_fetch_csc_f32": (c_int, [ctx_p, POINTER(c_int64), _f32p, _f32p]),
+        "hmc_host_alloc": (c_int, [ctypes.c_size_t, POINTER(c_void_p)]),
+        "hmc_host_free": (None, [c_void_p]),
         "hmc_reset_maps": (c_int, [ctx_p]),
         "hmc_set_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_get_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
@@ -242,6 +244,33 @@ class Context:
     def call(self, name: str, *args) -> None:
         rc = getattr(self._lib, name)(self.handle, *args)
         check(rc, self.handle, name)
+
+
+class _PinnedBlock:
+    """One page-locked host allocation, exposed to NumPy; freed when the last array
+    viewing it goes away."""
+
+    def __init__(self, n: int, dtype) -> None:
+        dt = np.dtype(dtype)
+        self._ptr = None
+        ptr = c_void_p()
+        rc = lib().hmc_host_alloc(max(1, n * dt.itemsize), ctypes.byref(ptr))
+        if rc != 0 or not ptr.value:
+            raise MemoryError(f"page-locked host allocation of {n * dt.itemsize} bytes failed")
+        self._ptr = ptr.value
+        self.__array_interface__ = {"shape": (n,), "typestr": dt.str, "data": (self._ptr, False), "version": 3}
+
+    def __del__(self) -> None:
+        if self._ptr and _lib is not None:
+            _lib.hmc_host_free(self._ptr)
+            self._ptr = None
+
+
+def pinned_empty(n: int, dtype) -> np.ndarray:
+    """An uninitialised 1-D array of ``n`` elements in page-locked host memory (the
+    destination of a fetched CSC: device-to-host copies into it run at the link's rate).
+    It behaves as any NumPy array; the memory is released with the last view of it."""
+    return np.asarray(_PinnedBlock(int(n), dtype))
 
 
 def device_info(device: int = 0) -> dict:

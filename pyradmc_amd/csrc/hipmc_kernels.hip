@@ -2179,6 +2179,22 @@ int hmc_scatter_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, in
   return HMC_OK;
 }
 
+int hmc_host_alloc(size_t bytes, void** out) {
+  if (!out) return HMC_EINVAL;
+  *out = nullptr;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes == 0 ? 1 : bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return HMC_ENOMEM;
+  }
+  *out = p;
+  return HMC_OK;
+}
+
+void hmc_host_free(void* ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+}
+
 int hmc_fetch_csc_f32(hmc_ctx* ctx, int64_t* indices, float* dose, float* sigma) {
   if (!ctx) return fail(ctx, HMC_EINVAL, "null");
   const long long n = ctx->trunc_total;

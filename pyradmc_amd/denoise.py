@@ -658,6 +658,7 @@ class DenoisedDijBuilder:
         import time
 
         from pyradmc_amd import SpectralBeamletSource
+        from pyradmc_amd.engine import _joined, host_array
         from pyradmc_amd.scoring import DijResult
 
         torch = _torch()
@@ -728,9 +729,9 @@ class DenoisedDijBuilder:
             eng._ctx.call("hmc_truncate", float(truncation), _native.ptr(kept, _native.c_int64),
                           ctypes.byref(total))
             mtot = int(total.value)
-            idx = np.empty(mtot, dtype=np.int64)
-            dose = np.empty(mtot, dtype=value_dtype)
-            sig = np.empty(mtot, dtype=value_dtype)
+            idx = host_array(mtot, np.int64)
+            dose = host_array(mtot, value_dtype)
+            sig = host_array(mtot, value_dtype)
             if mtot:
                 narrow = value_dtype == np.dtype(np.float32)
                 vt = ctypes.c_float if narrow else ctypes.c_double
@@ -747,7 +748,7 @@ class DenoisedDijBuilder:
         kept_all = np.concatenate(col_counts)
         indptr = np.zeros(n_cols + 1, dtype=np.int64)
         np.cumsum(kept_all, out=indptr[1:])
-        indices, dose, sigma = np.concatenate(col_idx), np.concatenate(col_dose), np.concatenate(col_sigma)
+        indices, dose, sigma = _joined(col_idx, np.int64), _joined(col_dose, value_dtype), _joined(col_sigma, value_dtype)
         fallback = disagreement > self.tolerance
         te = time.perf_counter()
         if fallback.any():

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import math
 import threading
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -88,11 +89,26 @@ _BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source"}
 _UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
 
 
+# Fetched CSC arrays at least this large go to page-locked host memory: copies into it
+# run at the link's rate, into fresh pageable memory at the first-touch fault rate.
+_PINNED_MIN_BYTES = 1 << 22
+
+
+def host_array(n: int, dtype) -> np.ndarray:
+    """An uninitialised 1-D host array for fetched entries (page-locked when large)."""
+    dt = np.dtype(dtype)
+    if n * dt.itemsize >= _PINNED_MIN_BYTES:
+        return nat.pinned_empty(n, dt)
+    return np.empty(n, dtype=dt)
+
+
 def _joined(parts: list, dtype) -> np.ndarray:
     """One array from the per-group pieces, without a copy when there is one group."""
     if not parts:
         return np.zeros(0, dtype)
-    return parts[0] if len(parts) == 1 else np.concatenate(parts)
+    if len(parts) == 1:
+        return parts[0]
+    return np.concatenate(parts, out=host_array(sum(len(p) for p in parts), parts[0].dtype))
 
 
 def _in_kernel(source) -> bool:
@@ -661,26 +677,29 @@ class HipEngine:
         deposited_quanta = 0
         col_counts, col_idx, col_dose, col_sigma = [], [], [], []
         analytic, mono_energy = False, 0.0
+        wall = {"groups": 0, "transport_s": 0.0, "assemble_s": 0.0}
         for g0 in range(0, n_beamlets, beamlet_group_size):
             group = range(g0, min(g0 + beamlet_group_size, n_beamlets))
             group_hist = int(counts[g0:group.stop].sum())
+            t0 = time.perf_counter()
             analytic, mono_energy, q = self._dij_group(source, group, counts, n_batches, seed,
                                                       correlated, scoring, books)
+            t1 = time.perf_counter()
             deposited_quanta += q
             kept = np.zeros(len(group), dtype=np.int64)
             total = nat.c_int64()
             self._ctx.call("hmc_truncate", float(truncation), nat.ptr(kept, nat.c_int64), nat.ctypes.byref(total))
             m = int(total.value)
-            idx = np.empty(m, dtype=np.int64)
+            idx = host_array(m, np.int64)
             if narrow:  # converted on the device: half the value bytes over PCIe
-                dose = np.empty(m, dtype=np.float32)
-                sig = np.empty(m, dtype=np.float32)
+                dose = host_array(m, np.float32)
+                sig = host_array(m, np.float32)
                 if m:
                     self._ctx.call("hmc_fetch_csc_f32", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.ctypes.c_float),
                                    nat.ptr(sig, nat.ctypes.c_float))
             else:
-                dose = np.empty(m, dtype=np.float64)
-                sig = np.empty(m, dtype=np.float64)
+                dose = host_array(m, np.float64)
+                sig = host_array(m, np.float64)
                 if m:
                     self._ctx.call("hmc_fetch_csc", nat.ptr(idx, nat.c_int64), nat.ptr(dose, nat.c_double),
                                    nat.ptr(sig, nat.c_double))
@@ -688,10 +707,18 @@ class HipEngine:
             col_idx.append(idx)
             col_dose.append(dose.astype(value_dtype, copy=False))
             col_sigma.append(sig.astype(value_dtype, copy=False))
+            wall["groups"] += 1
+            wall["transport_s"] += t1 - t0
+            wall["assemble_s"] += time.perf_counter() - t1
             emitter.tick(group_hist)
 
+        t2 = time.perf_counter()
         indptr = np.zeros(n_beamlets + 1, dtype=np.int64)
         np.cumsum(np.concatenate(col_counts), out=indptr[1:])
+        indices, dose_all, sigma_all = _joined(col_idx, np.int64), _joined(col_dose, value_dtype), \
+            _joined(col_sigma, value_dtype)
+        wall["assemble_s"] += time.perf_counter() - t2
+        books.update(wall)
         if analytic:
             energy_emitted = float(counts.sum()) * mono_energy
         else:
@@ -706,9 +733,9 @@ class HipEngine:
             n_batches=n_batches,
             truncation=float(truncation),
             indptr=indptr,
-            indices=_joined(col_idx, np.int64),
-            dose=_joined(col_dose, value_dtype),
-            sigma=_joined(col_sigma, value_dtype),
+            indices=indices,
+            dose=dose_all,
+            sigma=sigma_all,
             energy_emitted=energy_emitted,
             energy_deposited=energy_deposited,
             energy_escaped=float(books.get("escaped", 0)) * ENERGY_QUANTUM_MEV,

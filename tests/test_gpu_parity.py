@@ -660,3 +660,20 @@ def test_argument_validation():
         eng.run(NoSampler(), 10, 2, seed=1)
     eng.run(src, 10, 2, seed=1)  # the engine stays usable after rejected calls
     eng.close()
+
+
+def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
+    # Page-locked destinations of fetched CSC entries: ordinary arrays whose memory
+    # lives as long as any view of it.
+    a = hip.pinned_empty(3 << 20, np.float32)
+    assert a.dtype == np.float32 and a.shape == (3 << 20,) and a.flags.writeable
+    a[:] = np.arange(a.size, dtype=np.float32)
+    view = a[5:10]
+    del a
+    np.testing.assert_array_equal(view, np.arange(5, 10, dtype=np.float32))
+    assert hip.pinned_empty(0, np.int64).shape == (0,)
+    from pyradmc_amd.engine import host_array
+
+    assert host_array(10, np.float64).base is None  # small: ordinary memory
+    big = host_array(1 << 20, np.int64)
+    assert isinstance(big.base, hip._PinnedBlock)
