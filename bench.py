@@ -46,6 +46,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0  # dense bf16 MFMA, MI355X
 
 
 def _algorithmic_bytes(c: dict, prefix: str) -> float:
@@ -136,6 +137,34 @@ def _window_errors(d, ref, source, grid) -> np.ndarray:
     return out
 
 
+def _denoiser_roofline(model, shape: tuple[int, int, int, int], repeats: int = 5) -> dict:
+    """The denoiser's forward pass on a batch of windows (the builder's inference
+    batch) against the bf16 MFMA peak: algorithmic FLOPs over the mean time between HIP
+    events on the stream its kernels run on."""
+    import torch
+
+    from pyradmc_amd import denoise
+
+    rng = np.random.default_rng(5)
+    noisy = rng.random(shape, dtype=np.float32)
+    dens = rng.uniform(0.1, 1.0, shape).astype(np.float32)
+    x, _ = denoise.pack_input(noisy, dens)
+    model.predict_tensor(x)  # warm
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        model.predict_tensor(x)
+    stop.record()
+    stop.synchronize()
+    seconds = start.elapsed_time(stop) / 1e3 / repeats
+    flops = denoise.forward_flops(shape, model.widths)
+    achieved = flops / seconds / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_BF16_TFLOPS, 5), "batch": list(shape), "flops_per_batch": flops,
+            "ms_per_batch": round(seconds * 1e3, 3)}
+
+
 def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None = None) -> dict:
     """The 100-beamlet CT-like Dij at a dose-weighted RMS error target against a
     100x-history Monte Carlo reference: plain Monte Carlo (histories per beamlet set
@@ -163,6 +192,7 @@ def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None 
     plain_err = _window_errors(plain, ref, src, grid)
     builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, tolerance=target,
                                          fallback_histories=n_plain, n_batches=nb)
+    mfma = _denoiser_roofline(model, (32, grid.shape[2], denoise.WINDOW_XY, denoise.WINDOW_XY))
     builder.build(src, seed=11, value_dtype=np.float32)  # warm (kernels, allocator)
     started = time.perf_counter()
     den, report = builder.build(src, seed=12, truncation=1e-4, value_dtype=np.float32)
@@ -183,6 +213,7 @@ def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None 
         "denoised_fallback_columns": int(report.fallback.sum()),
         "denoised_error_median": round(float(np.median(den_err)), 5),
         "denoised_error_p90": round(float(np.percentile(den_err, 90)), 5),
+        "inference_roofline": mfma,
         "speedup": round(wall_plain / wall_den, 3) if wall_den > 0 else None,
     }
 

@@ -120,6 +120,20 @@ def layer_specs(cin0: int, widths: tuple[int, int, int, int]):
     ]
 
 
+def forward_flops(shape: tuple[int, int, int, int], widths: tuple[int, int, int, int] = (16, 32, 64, 128)) -> int:
+    """Algorithmic FLOPs of one forward pass over an [N][D][H][W] batch: 2 * 27 * C_in *
+    C_out per output voxel of every convolution, with the real input (2) and output (1)
+    channel counts rather than their MFMA padding."""
+    n, d, h, w = shape
+    level = {"0": 0, "1": 1, "2": 2, "3": 3}
+    total = 0
+    for name, cin, cout in layer_specs(2, widths):
+        lvl = 0 if name == "head" else level[name[1]]
+        voxels = n * (d >> lvl) * (h >> lvl) * (w >> lvl)
+        total += 2 * 27 * cin * (1 if name == "head" else cout) * voxels
+    return total
+
+
 @dataclass
 class _Layer:
     name: str

@@ -159,3 +159,18 @@ def test_dij_result_exports():
     with pytest.warns(UserWarning):
         d.variance_csc()
     assert math.isclose(d.column_dense(0).ravel()[3], 2.0)
+
+
+def test_denoiser_forward_flops_hand_count():
+    # Unit widths on an 8^3 input, counted by hand: level 0 (512 voxels) holds e0a
+    # (2 real input channels), e0b, d0a, d0b and the head (1 real output channel);
+    # levels 1-3 (64, 8, 1 voxels) hold four, four and two convolutions.
+    from pyradmc_amd import denoise
+
+    tap = 2 * 27
+    expected = tap * 2 * 512 + 4 * tap * 512 + 4 * tap * 64 + 4 * tap * 8 + 2 * tap * 1
+    assert expected == 181_548
+    assert denoise.forward_flops((1, 8, 8, 8), (1, 1, 1, 1)) == expected
+    base = denoise.forward_flops((1, 16, 16, 64))
+    assert denoise.forward_flops((3, 16, 16, 64)) == 3 * base
+    assert denoise.forward_flops((1, 32, 32, 128)) == 8 * base
