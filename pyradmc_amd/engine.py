@@ -89,13 +89,14 @@ _BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source"}
 _UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
 
 
-# Fetched CSC arrays at least this large go to page-locked host memory: copies into it
-# run at the link's rate, into fresh pageable memory at the first-touch fault rate.
-_PINNED_MIN_BYTES = 1 << 22
+# Host results at least this large (fetched CSC arrays, a run's dose and sigma) go to
+# page-locked memory: copies into it run at the link's rate, into fresh pageable memory
+# through staging buffers at the first-touch page-fault rate.
+_PINNED_MIN_BYTES = 1 << 20
 
 
 def host_array(n: int, dtype) -> np.ndarray:
-    """An uninitialised 1-D host array for fetched entries (page-locked when large)."""
+    """An uninitialised 1-D host array for device results (page-locked when large)."""
     dt = np.dtype(dtype)
     if n * dt.itemsize >= _PINNED_MIN_BYTES:
         return nat.pinned_empty(n, dt)
@@ -525,8 +526,8 @@ class HipEngine:
             for _ in block:
                 emitter.tick(per_batch)
         n_vox = scoring.n_voxels
-        mean = np.empty(n_vox, dtype=np.float64)
-        sigma = np.empty(n_vox, dtype=np.float64)
+        mean = host_array(n_vox, np.float64)
+        sigma = host_array(n_vox, np.float64)
         self._ctx.call("hmc_finalize", int(n_batches), nat.ptr(mean, nat.c_double), nat.ptr(sigma, nat.c_double))
 
         if analytic:

@@ -675,5 +675,5 @@ def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
     from pyradmc_amd.engine import host_array
 
     assert host_array(10, np.float64).base is None  # small: ordinary memory
-    big = host_array(1 << 20, np.int64)
+    big = host_array(1 << 17, np.int64)
     assert isinstance(big.base, hip._PinnedBlock)
