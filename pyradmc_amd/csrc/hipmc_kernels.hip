@@ -1214,6 +1214,9 @@ __global__ void __launch_bounds__(kTruncBlock) k_count_kept(const double* mean, 
   if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
 }
 
+// One block per chunk, in kTruncPerThread passes of kTruncBlock consecutive voxels (one
+// voxel per thread, coalesced): a ballot prefix within each wave and the wave totals
+// in LDS give every kept entry its slot, in ascending voxel order.
 __global__ void __launch_bounds__(kTruncBlock) k_compact(const double* mean, const double* sigma,
                                                          long long n_vox, int chunks_per_col,
                                                          const unsigned long long* col_max,
@@ -1221,41 +1224,43 @@ __global__ void __launch_bounds__(kTruncBlock) k_compact(const double* mean, con
                                                          const long long* offsets,
                                                          long long* out_idx, double* out_dose,
                                                          double* out_sigma) {
+  constexpr int kWaves = kTruncBlock / 64;
   const long long col = blockIdx.x / chunks_per_col;
   const long long chunk = blockIdx.x - col * chunks_per_col;
-  const long long v0 = chunk * kTruncChunk + static_cast<long long>(threadIdx.x) * kTruncPerThread;
+  const long long v0 = chunk * kTruncChunk;
   double cm;
   const unsigned long long bits = col_max[col];
   memcpy(&cm, &bits, sizeof(cm));
   const double thr = truncation * cm;
-  // Each thread owns kTruncPerThread consecutive voxels; an exclusive block scan of
-  // the per-thread counts gives ordered write cursors.
-  int c = 0;
-  for (int k = 0; k < kTruncPerThread; ++k) {
-    const long long v = v0 + k;
-    if (v < n_vox && keep_entry(mean[col * n_vox + v], thr)) ++c;
-  }
-  __shared__ int scan[kTruncBlock];
-  scan[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = 1; s < kTruncBlock; s <<= 1) {
-    const int add = threadIdx.x >= s ? scan[threadIdx.x - s] : 0;
-    __syncthreads();
-    scan[threadIdx.x] += add;
-    __syncthreads();
-  }
-  long long cursor = offsets[blockIdx.x] + scan[threadIdx.x] - c;
-  for (int k = 0; k < kTruncPerThread; ++k) {
-    const long long v = v0 + k;
+  const int lane = static_cast<int>(__lane_id());
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  __shared__ int wave_kept[2][kWaves];
+  long long cursor = offsets[blockIdx.x];
+  for (int pass = 0; pass < kTruncPerThread; ++pass) {
+    const long long v = v0 + static_cast<long long>(pass) * kTruncBlock + threadIdx.x;
+    double m = 0.0;
+    bool keep = false;
     if (v < n_vox) {
-      const double m = mean[col * n_vox + v];
-      if (keep_entry(m, thr)) {
-        out_idx[cursor] = v;
-        out_dose[cursor] = m;
-        out_sigma[cursor] = sigma[col * n_vox + v];
-        ++cursor;
-      }
+      m = mean[col * n_vox + v];
+      keep = keep_entry(m, thr);
     }
+    const unsigned long long mask = __ballot(keep ? 1 : 0);
+    int* kept = wave_kept[pass & 1];  // alternating rows: one barrier per pass
+    if (lane == 0) kept[wave] = __popcll(mask);
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      before += w < wave ? kept[w] : 0;
+      total += kept[w];
+    }
+    if (keep) {
+      const long long at = cursor + before + __popcll(mask & ((1ull << lane) - 1ull));
+      out_idx[at] = v;
+      out_dose[at] = m;
+      out_sigma[at] = sigma[col * n_vox + v];
+    }
+    cursor += total;
   }
 }
 
