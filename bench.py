@@ -97,6 +97,9 @@ def _dij_leg(median: float, launch: dict | None = None) -> dict:
     grid, xs, src, kw = workloads.ct_like_dij(median=median)
     counts = kw["n_histories_per_beamlet"]
     engine = _shaped_engine(grid, xs, launch)
+    if launch is None:  # launch shape for this workload (untimed; results do not depend on it)
+        engine.autotune_dij(src, counts, kw["n_batches"], seed=999, repeats=2, use_cache=False,
+                            truncation=kw["truncation"], value_dtype=np.float32)
     small = np.maximum(kw["n_batches"], counts // 100 // kw["n_batches"] * kw["n_batches"])
     engine.run_dij(src, small, kw["n_batches"], seed=1, truncation=kw["truncation"])  # warm
     started = time.perf_counter()
@@ -120,6 +123,7 @@ def _dij_leg(median: float, launch: dict | None = None) -> dict:
         "wall_s_at_2pct_projected": round(wall * (float(np.nanpercentile(sig, 90)) / 0.02) ** 2, 4),
         "events_per_history": _events(d.counters or {}, float(counts.sum())),
         "seconds": _phase_seconds(d.counters or {}),
+        "launch": dict(engine.launch, chunk_size=engine.chunk_size or "auto"),
     }
 
 
@@ -240,6 +244,9 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int,
     grid, xs, beams, kw = workloads.full_plan()
     nb, trunc = kw["n_batches"], kw["truncation"]
     engine = _shaped_engine(grid, xs, launch)
+    if launch is None:  # tuned on one beam at 1e5 histories per beamlet, one sweep (untimed)
+        engine.autotune_dij(beams[0], 100_000, nb, seed=999, repeats=2, passes=1, use_cache=False,
+                            truncation=trunc, value_dtype=np.float32)
     engine.run_dij(beams[0], 10 * nb, nb, seed=1, truncation=trunc)  # warm
 
     events: dict = {}
@@ -280,6 +287,7 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int,
         "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
         "events_per_history": _events(events, float(n_run) * d.n_beamlets),
         "seconds": _phase_seconds(events),
+        "launch": dict(engine.launch, chunk_size=engine.chunk_size or "auto"),
     }
     dev = spmm.DeviceDij(d)
     rng = np.random.default_rng(kw["seed"])
@@ -464,9 +472,12 @@ def main() -> int:
             counters["electron_kernel_ms"] / max(counters["electron_kernel_ms"] + counters["photon_kernel_ms"], 1e-9), 4),
     }
 
-    shape = dict(engine.launch)  # the legs below run at the step's launch shape
+    # The Dij legs tune their own launch shape (on their own workloads, untimed) unless
+    # a shape is given; the denoised leg runs at the batched Dij's.
+    shape = dict(engine.launch) if (args.launch or args.no_autotune) else None
     dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median, shape)
-    denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02, shape)
+    dij_shape = shape if shape is not None else (dij or {}).get("launch")
+    denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02, dij_shape)
                 if denoise_leg else None)
     plan, plan_dij = None, None
     if not args.no_plan:

@@ -72,7 +72,7 @@ class OmBooks(Structure):
 
 def build(verbose: bool = False) -> Path:
     """Compile ``mc_oracle.c`` with gcc into ``oracle/liboracle.so``."""
-    tmp = HERE / "liboracle.so.tmp"
+    tmp = HERE / f"liboracle.so.{os.getpid()}.tmp"  # concurrent builders each rename their own
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-pthread", "-o", str(tmp),
            str(HERE / "mc_oracle.c"), "-lm"]
     if verbose:

@@ -15,7 +15,8 @@ the tallies are integer sums, so dose and Dij are bit-identical under any of the
 ``tune`` runs a coordinate descent over candidate values, timing a caller-supplied
 piece of work (the median of ``repeats`` runs per candidate), applies the best and
 stores it in a JSON cache keyed by device and workload, so later engines on the same
-workload pick it up with ``load``.
+workload pick it up with ``load``. ``tune_run`` times an open-field ``run``,
+``tune_dij`` a whole ``run_dij``.
 """
 
 from __future__ import annotations
@@ -154,6 +155,27 @@ def tune_run(engine, source, n_histories: int, n_batches: int, seed: int = 1, re
             return cached
     config, seconds, _ = tune(engine, lambda: engine.run(source, n_histories, n_batches, seed, **run_kw),
                               candidates=candidates, repeats=repeats, log=log)
+    if use_cache:
+        store(key, config, seconds)
+    return config
+
+
+def tune_dij(engine, source, n_histories_per_beamlet, n_batches: int, seed: int = 1, repeats: int = 3,
+             candidates: dict | None = None, use_cache: bool = True, log=None, timer=time.perf_counter,
+             passes: int = 2, **dij_kw) -> LaunchConfig:
+    """Tune on ``engine.run_dij(source, n_histories_per_beamlet, n_batches, seed, **dij_kw)``
+    (transport, reduction and the CSC on the host, timed whole); cached like ``tune_run``
+    under a key that also names the beamlet count."""
+    counts = n_histories_per_beamlet
+    total = int(sum(counts)) if hasattr(counts, "__len__") else int(counts) * int(source.n_beamlets)
+    key = "dij|" + workload_key(engine, source, total, n_batches, dij_kw.get("transport_electrons", True)) \
+        + f"|c{int(source.n_beamlets)}"
+    if use_cache:
+        cached = load(engine, key)
+        if cached is not None:
+            return cached
+    config, seconds, _ = tune(engine, lambda: engine.run_dij(source, counts, n_batches, seed, **dij_kw),
+                              candidates=candidates, repeats=repeats, log=log, timer=timer, passes=passes)
     if use_cache:
         store(key, config, seconds)
     return config

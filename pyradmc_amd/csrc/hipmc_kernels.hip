@@ -1293,6 +1293,9 @@ struct hmc_ctx {
   double* d_s1 = nullptr;
   double* d_s2 = nullptr;
   double* d_nhist = nullptr;
+  // Leading map slots that may hold quanta: transport raises it, a fold of those slots
+  // (which consumes them) clears it, and a reset zeroes only that many slots.
+  int dirty_slots = 0;
   int* d_origins = nullptr;  // window origins for hmc_gather_windows / hmc_scatter_windows
   bool has_scoring = false;
   bool finalized = false;
@@ -1556,6 +1559,7 @@ int hmc_set_scoring(hmc_ctx* ctx, int nx, int ny, int nz, const double* origin,
   ctx->n_columns = n_columns;
   ctx->n_slots = n_slots;
   ctx->has_scoring = true;
+  ctx->dirty_slots = n_slots;  // fresh or reused buffers: clear every slot once
   return hmc_reset_maps(ctx);
 }
 
@@ -1564,15 +1568,17 @@ int hmc_reset_maps(hmc_ctx* ctx) {
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   const long long total = ctx->s.n_voxels * ctx->n_columns;
   const int blocks = blocks_for(total, 256, 65536);
-  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(total * ctx->n_slots, 256, 65536)), dim3(256), 0,
-                     ctx->stream, reinterpret_cast<unsigned long long*>(ctx->d_edep),
-                     total * ctx->n_slots, 0ull);
+  if (ctx->dirty_slots > 0)
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(total * ctx->dirty_slots, 256, 65536)), dim3(256), 0,
+                       ctx->stream, reinterpret_cast<unsigned long long*>(ctx->d_edep),
+                       total * ctx->dirty_slots, 0ull);
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(ctx->d_s1), total, 0ull);
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(ctx->d_s2), total, 0ull);
   HMC_HIP(ctx, hipGetLastError());
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->dirty_slots = 0;
   ctx->finalized = false;
   return HMC_OK;
 }
@@ -1583,6 +1589,7 @@ int hmc_set_quanta(hmc_ctx* ctx, const int64_t* quanta) {
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   const size_t total = static_cast<size_t>(ctx->s.n_voxels * ctx->n_columns * ctx->n_slots);
   HMC_HIP(ctx, hipMemcpy(ctx->d_edep, quanta, sizeof(long long) * total, hipMemcpyHostToDevice));
+  ctx->dirty_slots = ctx->n_slots;
   return HMC_OK;
 }
 
@@ -1879,10 +1886,12 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
 
   std::vector<DevItem> dev_items(static_cast<size_t>(n_items));
   long long total = 0;
+  int slots_written = 0;
   for (int i = 0; i < n_items; ++i) {
     if (items[i].count < 0 || items[i].column < 0 ||
         items[i].column >= ctx->n_columns * ctx->n_slots)
       return fail(ctx, HMC_EINVAL, "work item count/column out of range");
+    slots_written = std::max(slots_written, static_cast<int>(items[i].column / ctx->n_columns) + 1);
     const bool per_beamlet = ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN;
     if (items[i].beamlet < 0 || (per_beamlet && items[i].beamlet >= ctx->src_beamlets))
       return fail(ctx, HMC_EINVAL, "work item beamlet out of range");
@@ -1894,6 +1903,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   }
   if (ctx->src.kind == HMC_SRC_UPLOADED && total > ctx->src.up_n)
     return fail(ctx, HMC_EINVAL, "more histories than uploaded primaries");
+  ctx->dirty_slots = std::max(ctx->dirty_slots, slots_written);
   int rc;
   if (static_cast<size_t>(n_items) > ctx->items_cap) {
     if ((rc = dev_alloc(ctx, &ctx->d_items, static_cast<size_t>(n_items)))) return rc;
@@ -2050,6 +2060,7 @@ int hmc_fold(hmc_ctx* ctx, int n_slots_used, const double* nhist, int64_t* total
   unsigned long long q = 0;
   HMC_HIP(ctx, hipMemcpyAsync(&q, folded, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (n_slots_used >= ctx->dirty_slots) ctx->dirty_slots = 0;  // the fold zeroed what it read
   if (total_quanta) *total_quanta = static_cast<int64_t>(q);
   return HMC_OK;
 }

@@ -459,6 +459,13 @@ class HipEngine:
 
         return autotune.tune_run(self, source, n_histories, n_batches, seed=seed, **kw)
 
+    def autotune_dij(self, source, n_histories_per_beamlet, n_batches: int, seed: int = 1, **kw):
+        """Tune on a timed ``run_dij`` of this workload (``pyradmc_amd.autotune.tune_dij``)
+        and apply the best; returns the chosen configuration."""
+        from pyradmc_amd import autotune
+
+        return autotune.tune_dij(self, source, n_histories_per_beamlet, n_batches, seed=seed, **kw)
+
     # --------------------------------------------------------------------- run
 
     def run(

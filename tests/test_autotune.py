@@ -106,3 +106,30 @@ def test_failing_candidate_is_skipped():
                                       timer=lambda: clock[0])
     assert config.push_chunk == 128 and t == 1.0
     assert eng.launch["push_chunk"] == 128
+
+
+def test_dij_tuning_times_run_dij_and_caches_under_its_own_key(tmp_path, monkeypatch):
+    monkeypatch.setenv("PYRADMC_AMD_CACHE", str(tmp_path / "autotune.json"))
+    eng = _FakeEngine()
+    clock = [0.0]
+    seen = []
+
+    def run_dij(source, counts, n_batches, seed, **kw):
+        seen.append((counts, n_batches, seed, kw))
+        clock[0] += 1.0 + abs(eng.launch["round_lanes_per_cu"] - 1024) / 1024
+
+    eng.run_dij = run_dij
+    src = SimpleNamespace(max_energy=6.0, n_beamlets=225)
+    candidates = {"round_lanes_per_cu": (512, 1024, 2048)}
+    config = autotune.tune_dij(eng, src, 20_000, 10, seed=3, repeats=1, candidates=candidates,
+                               timer=lambda: clock[0], truncation=1e-4)
+    assert config.round_lanes_per_cu == 1024
+    assert seen and all(s == (20_000, 10, 3, {"truncation": 1e-4}) for s in seen)
+    cache = (tmp_path / "autotune.json").read_text()
+    assert '"dij|' in cache and "|c225" in cache
+    calls = len(seen)
+    eng.launch = {}
+    again = autotune.tune_dij(eng, src, 20_000, 10, seed=3, candidates=candidates, timer=lambda: clock[0],
+                              truncation=1e-4)
+    assert again == config and len(seen) == calls  # from the cache, nothing timed
+    assert eng.launch["round_lanes_per_cu"] == 1024
