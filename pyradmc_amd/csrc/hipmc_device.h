@@ -194,14 +194,19 @@ __device__ inline float boundary_distance(const DevGrid& g, float x, float y, fl
   return fmaxf(r, 0.0f);
 }
 
-// Flat scoring index (ix*ny + iy)*nz + iz, or -1 outside the scoring grid
-// (pyradmc/backends/warp/kernels.py:492-532, rectilinear branch).
-__device__ inline long long scoring_index(const DevScoring& s, float x, float y, float z) {
-  if (!inside(x, y, z, s.x_lo, s.y_lo, s.z_lo, s.x_hi, s.y_hi, s.z_hi)) return -1;
+// Flat scoring index (ix*ny + iy)*nz + iz of a position inside the scoring grid.
+__device__ inline long long grid_index(const DevScoring& s, float x, float y, float z) {
   const int ix = axis_index(x, s.x_lo, s.inv_sx, s.nx);
   const int iy = axis_index(y, s.y_lo, s.inv_sy, s.ny);
   const int iz = axis_index(z, s.z_lo, s.inv_sz, s.nz);
   return (static_cast<long long>(ix) * s.ny + iy) * s.nz + iz;
+}
+
+// Flat scoring index, or -1 outside the scoring grid
+// (pyradmc/backends/warp/kernels.py:492-532, rectilinear branch).
+__device__ inline long long scoring_index(const DevScoring& s, float x, float y, float z) {
+  if (!inside(x, y, z, s.x_lo, s.y_lo, s.z_lo, s.x_hi, s.y_hi, s.z_hi)) return -1;
+  return grid_index(s, x, y, z);
 }
 
 // ------------------------------------------------------------------------- physics
@@ -377,9 +382,17 @@ __device__ inline float spr_factor(const DevTables& t, int dose_to_water, float 
   return s_water / s_medium;
 }
 
+// An integer-valued double of magnitude below 2^51 as int64, exactly: adding 1.5 * 2^52
+// places the integer in the low mantissa bits, which the bit pattern then holds (a few
+// instructions where a general float64 -> int64 conversion takes a dozen).
+__device__ inline long long whole_to_ll(double whole) {
+  return __double_as_longlong(whole + 6755399441055744.0) - 0x4338000000000000LL;
+}
+
+// Energy to fixed-point quanta, floor(E / quantum + 1/2): round to nearest for either
+// sign (pyradmc/backends/warp/kernels.py:584-597); |E| stays far below 2^51 quanta.
 __device__ inline long long quanta_round(double energy) {
-  // Round-to-nearest for either sign (pyradmc/backends/warp/kernels.py:584-597).
-  return static_cast<long long>(floor(energy * kInvQuantum + 0.5));
+  return whole_to_ll(floor(energy * kInvQuantum + 0.5));
 }
 
 }  // namespace hmc

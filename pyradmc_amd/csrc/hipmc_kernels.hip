@@ -312,25 +312,40 @@ __device__ inline void push(const Queue& q, uint32_t kind, uint32_t column, floa
   }
 }
 
+// The round kernels come in two builds. kFast is the common configuration: dose to
+// medium (no stopping-power weight), no deposit spreading, and a scoring grid that is
+// the transport grid, so a position inside the transport grid needs no second
+// containment test. The general build covers everything else; both give the same
+// bits where both apply.
+
+// Dose-to-water weight of a deposit (1 under dose to medium).
+template <bool kFast>
+__device__ inline float spr(const TransportParams& P, float e, int mat) {
+  if constexpr (kFast) return 1.0f;
+  else return spr_factor(P.t, P.dose_to_water, e, mat, P.ecut);
+}
+
 // Fixed-point deposit at a position inside the transport grid, routed into the
-// scoring grid or the unscored book (pyradmc/backends/warp/kernels.py:535-584).
+// scoring grid or the unscored book (pyradmc/backends/warp/kernels.py:535-584). As there,
+// quanta are x + 1/2 truncated toward zero (an amount a rounding step left slightly
+// negative rounds toward zero, unlike an escape's floor).
+template <bool kFast>
 __device__ inline void deposit(const TransportParams& P, LaneTally& tal, long long base, float x,
                                float y, float z, float amount, float factor) {
-  const long long idx = scoring_index(P.s, x, y, z);
+  const long long idx = kFast ? grid_index(P.s, x, y, z) : scoring_index(P.s, x, y, z);
   if (idx >= 0) {
-    const long long q = static_cast<long long>(static_cast<double>(amount) *
-                                                   static_cast<double>(factor) * kInvQuantum +
-                                               0.5);
+    const long long q = whole_to_ll(trunc(static_cast<double>(amount) * static_cast<double>(factor) *
+                                          kInvQuantum + 0.5));
     const long long cell = base + idx;
     if (cell != tal.pend_cell) {
       tal.flush_pending(P.edep);
       tal.pend_cell = cell;
     }
     tal.pend_q += q;
-    if (P.dose_to_water)
-      tal.deposited += static_cast<long long>(static_cast<double>(amount) * kInvQuantum + 0.5);
+    if (!kFast && P.dose_to_water)
+      tal.deposited += whole_to_ll(trunc(static_cast<double>(amount) * kInvQuantum + 0.5));
   } else {
-    tal.unscored += static_cast<long long>(static_cast<double>(amount) * kInvQuantum + 0.5);
+    tal.unscored += whole_to_ll(trunc(static_cast<double>(amount) * kInvQuantum + 0.5));
   }
 }
 
@@ -339,18 +354,25 @@ __device__ inline void escape(LaneTally& tal, float amount) {
 }
 
 // Deposit if inside the transport grid, else escaped (kernels.py:600-633).
+template <bool kFast>
 __device__ inline void deposit_or_escape(const TransportParams& P, LaneTally& tal, long long base,
                                          float x, float y, float z, float amount, float factor) {
   if (amount <= 0.0f) return;
-  if (inside_grid(P.g, x, y, z)) deposit(P, tal, base, x, y, z, amount, factor);
+  if (inside_grid(P.g, x, y, z)) deposit<kFast>(P, tal, base, x, y, z, amount, factor);
   else escape(tal, amount);
 }
 
 // One half-substep's continuous loss, in equal pieces no longer than the
 // resolution (pyradmc/transport/electron.py:169-191, 361-380).
+template <bool kFast>
 __device__ inline void deposit_spread(const TransportParams& P, LaneTally& tal, long long base,
                                       float x, float y, float z, float ux, float uy, float uz,
                                       float length, float amount, float factor) {
+  if constexpr (kFast) {  // one piece: the midpoint takes the whole amount
+    const float t = 0.5f * length;
+    deposit_or_escape<kFast>(P, tal, base, x + ux * t, y + uy * t, z + uz * t, amount, factor);
+    return;
+  }
   int n = 1;
   if (P.resolution > 0.0f) {
     n = static_cast<int>(ceilf(length / P.resolution));
@@ -360,17 +382,18 @@ __device__ inline void deposit_spread(const TransportParams& P, LaneTally& tal, 
   const float piece = length / static_cast<float>(n);
   for (int k = 0; k < n; ++k) {
     const float t = (static_cast<float>(k) + 0.5f) * piece;
-    deposit_or_escape(P, tal, base, x + ux * t, y + uy * t, z + uz * t, share, factor);
+    deposit_or_escape<kFast>(P, tal, base, x + ux * t, y + uy * t, z + uz * t, share, factor);
   }
 }
 
 // Positron annihilation at rest: two back-to-back 511 keV photons, isotropic
 // (pyradmc/transport/particles.py:75-102).
+template <bool kFast>
 __device__ inline void annihilate(const TransportParams& P, LaneTally& tal, const Queue& qp,
                                   long long base, uint32_t column, float x, float y, float z,
                                   float w, float factor, uint64_t& rng) {
   if (P.pcut >= kElectronMass) {
-    deposit(P, tal, base, x, y, z, w * 2.0f * kElectronMass, factor);
+    deposit<kFast>(P, tal, base, x, y, z, w * 2.0f * kElectronMass, factor);
     return;
   }
   const float c = 1.0f - 2.0f * uniform(rng);
@@ -429,6 +452,7 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 #define HMC_ROUND_WAVES 5
 #endif
 
+template <bool kFast>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
 k_photon_round(TransportParams P, Queue qin,
                                                       unsigned int* head, Queue qe, Queue qp) {
@@ -533,14 +557,14 @@ k_photon_round(TransportParams P, Queue qin,
         const uint64_t child = spawn_stream(rng);
         push(qe, ELECTRON, column, recoil, w, x, y, z, ex, ey, ez, child);
       } else {
-        deposit(P, tal, base, x, y, z, w * recoil,
-                spr_factor(P.t, P.dose_to_water, recoil, mat, P.ecut));
+        deposit<kFast>(P, tal, base, x, y, z, w * recoil,
+                spr<kFast>(P, recoil, mat));
       }
       const float cg = compton_cos(e, ratio);
       rotate(ux, uy, uz, cg, phi);
       e *= ratio;
       if (e <= P.pcut) {
-        deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+        deposit<kFast>(P, tal, base, x, y, z, w * e, spr<kFast>(P, e, mat));
         active = false;
         continue;
       }
@@ -560,12 +584,12 @@ k_photon_round(TransportParams P, Queue qin,
         const uint64_t child = spawn_stream(rng);
         push(qe, ELECTRON, column, e, w, x, y, z, ux, uy, uz, child);
       } else {
-        deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+        deposit<kFast>(P, tal, base, x, y, z, w * e, spr<kFast>(P, e, mat));
       }
       active = false;
     } else {  // pair production
       const float kinetic = e - 2.0f * kElectronMass;
-      const float f_ann = spr_factor(P.t, P.dose_to_water, kElectronMass, mat, P.ecut);
+      const float f_ann = spr<kFast>(P, kElectronMass, mat);
       if (P.transport_electrons) {
         const float fraction = uniform(rng);
         const float share_e = fraction * kinetic;
@@ -574,21 +598,21 @@ k_photon_round(TransportParams P, Queue qin,
           const uint64_t child = spawn_stream(rng);
           push(qe, ELECTRON, column, share_e, w, x, y, z, ux, uy, uz, child);
         } else {
-          deposit(P, tal, base, x, y, z, w * share_e,
-                  spr_factor(P.t, P.dose_to_water, share_e, mat, P.ecut));
+          deposit<kFast>(P, tal, base, x, y, z, w * share_e,
+                  spr<kFast>(P, share_e, mat));
         }
         if (share_p > P.ecut) {
           const uint64_t child = spawn_stream(rng);
           push(qe, POSITRON, column, share_p, w, x, y, z, ux, uy, uz, child);
         } else {
-          deposit(P, tal, base, x, y, z, w * share_p,
-                  spr_factor(P.t, P.dose_to_water, share_p, mat, P.ecut));
-          annihilate(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
+          deposit<kFast>(P, tal, base, x, y, z, w * share_p,
+                  spr<kFast>(P, share_p, mat));
+          annihilate<kFast>(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
         }
       } else {
-        deposit(P, tal, base, x, y, z, w * kinetic,
-                spr_factor(P.t, P.dose_to_water, kinetic, mat, P.ecut));
-        annihilate(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
+        deposit<kFast>(P, tal, base, x, y, z, w * kinetic,
+                spr<kFast>(P, kinetic, mat));
+        annihilate<kFast>(P, tal, qp, base, column, x, y, z, w, f_ann, rng);
       }
       active = false;
     }
@@ -605,6 +629,7 @@ k_photon_round(TransportParams P, Queue qin,
 // CSDA loss through the restricted range and its inverse, Bernoulli brems, random
 // hinge with GS (or Gaussian) deflection, two half-step deposits, Moller delta
 // (pyradmc/transport/electron.py:292-481, kernels.py:1532-1885).
+template <bool kFast>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
 k_electron_round(TransportParams P, Queue qin,
                                                         unsigned int* head, Queue qp, Queue qe) {
@@ -665,10 +690,10 @@ k_electron_round(TransportParams P, Queue qin,
 
     if (e <= P.ecut) {
       // Terminal: local deposit, annihilation for a positron.
-      deposit(P, tal, base, x, y, z, w * e, spr_factor(P.t, P.dose_to_water, e, mat, P.ecut));
+      deposit<kFast>(P, tal, base, x, y, z, w * e, spr<kFast>(P, e, mat));
       if (positron)
-        annihilate(P, tal, qp, base, column, x, y, z, w,
-                   spr_factor(P.t, P.dose_to_water, kElectronMass, mat, P.ecut), rng);
+        annihilate<kFast>(P, tal, qp, base, column, x, y, z, w,
+                   spr<kFast>(P, kElectronMass, mat), rng);
       active = false;
       continue;
     }
@@ -727,14 +752,14 @@ k_electron_round(TransportParams P, Queue qin,
     }
     const float continuous = de + local_brems;
     const bool emit_brems = uniform(rng) < emit_probability;
-    const float factor = spr_factor(P.t, P.dose_to_water, e, mat, P.ecut);
+    const float factor = spr<kFast>(P, e, mat);
 
     const float s1 = uniform(rng) * s;
     const float s2 = s - s1;
     const float d1 = s > 0.0f ? continuous * (s1 / s) : 0.0f;
     const float d2 = continuous - d1;
 
-    deposit_spread(P, tal, base, x, y, z, ux, uy, uz, s1, w * d1, factor);
+    deposit_spread<kFast>(P, tal, base, x, y, z, ux, uy, uz, s1, w * d1, factor);
     e -= d1;
     x += ux * s1;
     y += uy * s1;
@@ -774,13 +799,13 @@ k_electron_round(TransportParams P, Queue qin,
           push(qp, PHOTON, column, k, photon_w, x, y, z, ux, uy, uz, child);
         }
       } else {
-        deposit_or_escape(P, tal, base, x, y, z, w * k,
-                          spr_factor(P.t, P.dose_to_water, k, mat, P.ecut));
+        deposit_or_escape<kFast>(P, tal, base, x, y, z, w * k,
+                          spr<kFast>(P, k, mat));
       }
       e -= k;
     }
 
-    deposit_spread(P, tal, base, x, y, z, ux, uy, uz, s2, w * d2, factor);
+    deposit_spread<kFast>(P, tal, base, x, y, z, ux, uy, uz, s2, w * d2, factor);
     e -= d2;
     x += ux * s2;
     y += uy * s2;
@@ -1792,8 +1817,20 @@ int hmc_set_launch(hmc_ctx* ctx, int round_lanes_per_cu, int refill_block, int p
 // launches. With events (profiling), each round's two kernels are bracketed.
 constexpr int kRoundsPerSync = 4;
 
+// The rounds' fast build applies: dose to medium, no deposit spreading, and a scoring
+// grid with the transport grid's bounds, voxel sizes and counts.
+static bool fast_rounds(const TransportParams& P) {
+  const DevGrid& g = P.g;
+  const DevScoring& s = P.s;
+  return !P.dose_to_water && P.resolution == 0.0f && g.x_lo == s.x_lo && g.y_lo == s.y_lo &&
+         g.z_lo == s.z_lo && g.x_hi == s.x_hi && g.y_hi == s.y_hi && g.z_hi == s.z_hi &&
+         g.inv_sx == s.inv_sx && g.inv_sy == s.inv_sy && g.inv_sz == s.inv_sz && g.nx == s.nx &&
+         g.ny == s.ny && g.nz == s.nz;
+}
+
 static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned int round_blocks,
                                unsigned int cap, hipEvent_t* events) {
+  const bool fast = fast_rounds(P);
   unsigned int* counts = ctx->d_counts;
   unsigned int* heads = ctx->d_counts + 4;
   int pa = 0, ea = 1, pb = 2, eb = 3;
@@ -1805,14 +1842,22 @@ static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned 
     hipEvent_t* ev = events ? events + 4 * r : nullptr;
     // Photon round: consume pa; electrons append to ea, photons to pb.
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-    hipLaunchKernelGGL(k_photon_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p, heads + pa,
-                       qa_e, qb_p);
+    if (fast)
+      hipLaunchKernelGGL((k_photon_round<true>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p,
+                         heads + pa, qa_e, qb_p);
+    else
+      hipLaunchKernelGGL((k_photon_round<false>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_p,
+                         heads + pa, qa_e, qb_p);
     HMC_HIP(ctx, hipGetLastError());
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     // Electron round: consume ea (including the photon round's pushes).
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-    hipLaunchKernelGGL(k_electron_round, dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e, heads + ea,
-                       qb_p, qb_e);
+    if (fast)
+      hipLaunchKernelGGL((k_electron_round<true>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
+                         heads + ea, qb_p, qb_e);
+    else
+      hipLaunchKernelGGL((k_electron_round<false>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
+                         heads + ea, qb_p, qb_e);
     HMC_HIP(ctx, hipGetLastError());
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     hipLaunchKernelGGL(k_reset_consumed, dim3(1), dim3(64), 0, ctx->stream, counts, heads, pa, ea, cap,

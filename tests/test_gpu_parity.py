@@ -677,3 +677,26 @@ def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
     assert host_array(10, np.float64).base is None  # small: ordinary memory
     big = host_array(1 << 17, np.int64)
     assert isinstance(big.base, hip._PinnedBlock)
+
+
+def test_fast_and_general_round_builds_agree_bit_for_bit():
+    # The round kernels' fast build (dose to medium, no deposit spreading, scoring grid =
+    # transport grid) and their general build must give the same bits where both
+    # apply: in water the dose-to-water weight is exactly 1, and a deposit resolution
+    # longer than any sub-step leaves every half-step in one piece.
+    from pyradmc_amd import ParallelBeamSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = VoxelGrid.uniform_water((16, 16, 24), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    src = ParallelBeamSource(4.0, -0.1, (2.0, 6.0), (2.0, 6.0))
+    eng = _engine(grid, xs)
+    fast = eng.run(src, 400_000, 8, seed=13)
+    to_water = eng.run(src, 400_000, 8, seed=13, scoring_mode="dose_to_water")
+    spread = eng.run(src, 400_000, 8, seed=13, deposit_resolution_cm=1.0e3)
+    eng.close()
+    assert fast.dose.max() > 0.0
+    for other in (to_water, spread):
+        np.testing.assert_array_equal(other.dose, fast.dose)
+        np.testing.assert_array_equal(other.dose_sigma, fast.dose_sigma)
+    assert spread.energy_deposited == fast.energy_deposited
