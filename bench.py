@@ -27,7 +27,13 @@ this workload (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter reads fo
 
 ``cpu_baseline`` times the CPU restatement of the reference algorithm (``oracle/``,
 double precision, one stream per history, pthreads) on this host's cores for a
-bounded sample of the same workload.
+bounded sample of the same workload (and the quick-start pencil, ``pencil``).
+
+Further legs, reported beside the throughput line: ``pencil`` (the reference's
+quick-start pencil in both voxel readings), ``dij`` (the batched uneven 100-beamlet
+Dij), ``denoised_dij`` (denoised against plain Monte Carlo at the same error, with the
+denoiser's MFMA roofline) and ``full_plan`` (five gantry beams, 1125 columns, and the
+1000-vector planning product).
 """
 
 from __future__ import annotations
@@ -86,6 +92,32 @@ def _shaped_engine(grid, xs, launch: dict | None):
         engine.set_launch(launch["round_lanes_per_cu"], launch["refill_block"], launch["push_chunk"],
                           launch["use_graphs"])
     return engine
+
+
+def _pencil_leg(repeats: int = 7) -> dict:
+    """The reference's quick start: a 6 MeV pencil into 16^3 water, 80,000 histories in 10
+    batches, in both readings (1 cm voxels entering at (8, 8, -1) cm; 1 mm voxels at
+    (0.8, 0.8, -0.1) cm). Median wall of whole runs, dose and sigma on the host; the
+    workload is too small to fill the GPU, so this measures launch and host overhead."""
+    import statistics
+
+    from pyradmc_amd import HipEngine, workloads
+
+    out = {}
+    for units in ("cm", "mm"):
+        grid, xs, src, kw = workloads.small_pencil(units)
+        engine = HipEngine(grid, xs)
+        engine.run(src, kw["n_histories"], kw["n_batches"], seed=1)  # warm
+        walls = []
+        for k in range(repeats):
+            started = time.perf_counter()
+            engine.run(src, kw["n_histories"], kw["n_batches"], seed=kw["seed"] + k)
+            walls.append(time.perf_counter() - started)
+        engine.close()
+        wall = statistics.median(walls)
+        out[units] = {"wall_s": round(wall, 5), "histories_per_s": round(kw["n_histories"] / wall, 1)}
+    out["workload"] = "6 MeV pencil, 16^3 water (1 cm and 1 mm voxels), 80,000 histories in 10 batches"
+    return out
 
 
 def _dij_leg(median: float, launch: dict | None = None) -> dict:
@@ -328,6 +360,24 @@ def _cpu_dose_for_weights(d, vectors: int = 2) -> dict:
             "sample": f"full-plan Dij, {vectors} weight vectors, one thread (NumPy)"}
 
 
+def _cpu_pencil(threads: int) -> dict:
+    """The CPU restatement on the quick-start pencil (both readings), for the pencil leg."""
+    from oracle import mc
+    from pyradmc_amd import workloads
+
+    out = {}
+    for units in ("cm", "mm"):
+        grid, _, src, kw = workloads.small_pencil(units)
+        ora = mc.for_grid(grid, src)
+        ora.run(1000, 1, seed=1, threads=threads)  # warm
+        started = time.perf_counter()
+        ora.run(kw["n_histories"], kw["n_batches"], seed=kw["seed"], threads=threads)
+        wall = time.perf_counter() - started
+        out[units] = {"wall_s": round(wall, 4), "histories_per_s": round(kw["n_histories"] / wall, 1)}
+    out.update(cores=threads, kind="port")
+    return out
+
+
 def _cpu_baseline(sample_histories: int, threads: int) -> dict:
     from oracle import mc
     from pyradmc_amd import workloads
@@ -369,6 +419,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="histories for the CPU baseline (default: about 10 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pencil", action="store_true", help="skip the quick-start pencil leg")
     ap.add_argument("--no-dij", action="store_true", help="skip the batched Dij leg")
     ap.add_argument("--dij-median", type=float, default=2e5,
                     help="median per-beamlet history count of the Dij leg")
@@ -475,6 +526,7 @@ def main() -> int:
     # The Dij legs tune their own launch shape (on their own workloads, untimed) unless
     # a shape is given; the denoised leg runs at the batched Dij's.
     shape = dict(engine.launch) if (args.launch or args.no_autotune) else None
+    pencil = None if args.no_pencil else _guarded("pencil", _pencil_leg)
     dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median, shape)
     dij_shape = shape if shape is not None else (dij or {}).get("launch")
     denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02, dij_shape)
@@ -489,6 +541,8 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         sample = args.cpu_sample or 2_500_000 * threads  # ~10 s at ~0.25M histories/s/thread
         cpu = _cpu_baseline(sample, threads)
+        if not args.no_pencil:
+            cpu["pencil"] = _guarded("cpu pencil", _cpu_pencil, threads)
         if plan_dij is not None:
             cpu["dose_for_weights"] = _guarded("cpu dose_for_weights", _cpu_dose_for_weights, plan_dij)
 
@@ -520,6 +574,7 @@ def main() -> int:
                          "graph_launches": int(graph_counts)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pencil": pencil,
         "dij": dij,
         "denoised_dij": denoised,
         "full_plan": plan,
