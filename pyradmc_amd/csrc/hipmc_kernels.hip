@@ -1601,8 +1601,7 @@ int hmc_reset_maps(hmc_ctx* ctx) {
                      reinterpret_cast<unsigned long long*>(ctx->d_s1), total, 0ull);
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(ctx->d_s2), total, 0ull);
-  HMC_HIP(ctx, hipGetLastError());
-  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  HMC_HIP(ctx, hipGetLastError());  // stream-ordered before the next transport: no host wait
   ctx->dirty_slots = 0;
   ctx->finalized = false;
   return HMC_OK;
@@ -2006,6 +2005,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     }
   }
   unsigned int* counts = ctx->d_counts;
+  // The ledger comes back with every count read (one host wait per group of rounds).
+  unsigned long long host_ledger[L_COUNT];
 
   for (long long p0 = 0; p0 < total && !overflow; p0 += chunk_n) {
     const long long n_gen = std::min(chunk_n, total - p0);
@@ -2028,6 +2029,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       unsigned int host_counts[4];
       HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts), hipMemcpyDeviceToHost,
                                   ctx->stream));
+      HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger), hipMemcpyDeviceToHost,
+                                  ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
       if (ctx->profile) {
         for (int r = 0; r < kRoundsPerSync; ++r) {
@@ -2044,10 +2047,11 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       if (rounds > 100000) return fail(ctx, HMC_EHIP, "transport did not drain in 100000 rounds");
     }
   }
-  unsigned long long host_ledger[L_COUNT];
-  HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (rounds == 0) {  // no histories: nothing ran, the ledger holds its cleared state
+    HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
   overflow = std::max<long long>(overflow, static_cast<long long>(host_ledger[L_OVERFLOW]));
   auto L = [&](int slot) { return static_cast<int64_t>(host_ledger[slot]); };
   ledger->escaped = L(L_ESCAPED);

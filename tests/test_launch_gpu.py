@@ -97,3 +97,23 @@ def test_autotune_picks_a_shape_that_changes_no_bit(slab, tmp_path, monkeypatch)
     again = _engine(grid, xs)
     assert again.autotune(src, 200_000, 2, seed=1) == config  # from the cache
     _same(_engine(grid, xs, use_graphs=False).run(src, 200_000, 2, seed=9), eng.run(src, 200_000, 2, seed=9))
+
+
+def test_dij_autotune_picks_a_shape_that_changes_no_bit(tmp_path, monkeypatch):
+    from pyradmc_amd import BeamletGridSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    monkeypatch.setenv("PYRADMC_AMD_CACHE", str(tmp_path / "autotune.json"))
+    grid = VoxelGrid.uniform_water((12, 12, 20), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    src = BeamletGridSource(4.0, -0.1, (1.0, 5.0), (1.0, 5.0), 3, 3)
+    eng = _engine(grid, xs)
+    candidates = {"round_lanes_per_cu": (512, 1024), "push_chunk": (64, 256), "chunk_size": (None, 50_000)}
+    config = eng.autotune_dij(src, 20_000, 4, seed=1, repeats=1, candidates=candidates, truncation=1e-3,
+                              value_dtype=np.float32)
+    assert "dij|" in (tmp_path / "autotune.json").read_text()
+    assert eng.launch["round_lanes_per_cu"] == config.round_lanes_per_cu
+    tuned = eng.run_dij(src, 20_000, 4, seed=5, truncation=1e-3)
+    plain = _engine(grid, xs, use_graphs=False).run_dij(src, 20_000, 4, seed=5, truncation=1e-3)
+    for name in ("indptr", "indices", "dose", "sigma"):
+        np.testing.assert_array_equal(getattr(tuned, name), getattr(plain, name))
