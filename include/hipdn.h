@@ -42,6 +42,9 @@ int hdn_reduce_splits(const float* part, float* out, long long len, int splits, 
 /* Per-channel sums of a channel-major tensor over its P positions: part[splits][C]. */
 int hdn_channel_sum(const void* cm, float* part, long long P, int C, int splits, void* stream);
 
+/* In the element-wise calls below, the channel-major copy (y_cm, dz_cm) may be NULL:
+ * it feeds only the weight gradients, so inference skips it. */
+
 /* dz = dy * act'(y) for the leaky activation (from the stored output y). */
 int hdn_act_bwd(const void* y, const void* dy, void* dz_nd, void* dz_cm, long long P, int C,
                 float slope, void* stream);

@@ -230,26 +230,26 @@ class Conv3dDenoiser:
               lay.cin, lay.cout, SLOPE if activate else 1.0, self._stream())
         return y, y_cm
 
-    def _pool(self, x, dims, C):
+    def _pool(self, x, dims, C, want_cm: bool = True):
         torch = _torch()
         n, d, h, w = dims
         y = torch.empty((n, d // 2, h // 2, w // 2, C), dtype=torch.bfloat16, device=x.device)
-        y_cm = torch.empty((C, y.numel() // C), dtype=torch.bfloat16, device=x.device)
+        y_cm = torch.empty((C, y.numel() // C), dtype=torch.bfloat16, device=x.device) if want_cm else None
         _call("hdn_pool_fwd", _ptr(x), _ptr(y), _ptr(y_cm), n, d // 2, h // 2, w // 2, C, self._stream())
         return y, y_cm
 
-    def _up(self, x, dims_fine, C):
+    def _up(self, x, dims_fine, C, want_cm: bool = True):
         torch = _torch()
         n, d, h, w = dims_fine
         y = torch.empty((n, d, h, w, C), dtype=torch.bfloat16, device=x.device)
-        y_cm = torch.empty((C, y.numel() // C), dtype=torch.bfloat16, device=x.device)
+        y_cm = torch.empty((C, y.numel() // C), dtype=torch.bfloat16, device=x.device) if want_cm else None
         _call("hdn_up_fwd", _ptr(x), _ptr(y), _ptr(y_cm), n, d, h, w, C, self._stream())
         return y, y_cm
 
-    def _add(self, a, b, C):
+    def _add(self, a, b, C, want_cm: bool = True):
         torch = _torch()
         y = torch.empty_like(a)
-        y_cm = torch.empty((C, a.numel() // C), dtype=torch.bfloat16, device=a.device)
+        y_cm = torch.empty((C, a.numel() // C), dtype=torch.bfloat16, device=a.device) if want_cm else None
         _call("hdn_add", _ptr(a), _ptr(b), _ptr(y), _ptr(y_cm), a.numel() // C, C, self._stream())
         return y, y_cm
 
@@ -272,18 +272,19 @@ class Conv3dDenoiser:
 
         h0 = conv("e0a", x, x_cm, 0)
         s0 = conv("e0b", *h0, 0)
-        t = conv("e1a", *self._pool(s0[0], dims[0], c0), 1)
+        # The channel-major copies feed only the weight gradients: inference skips them.
+        t = conv("e1a", *self._pool(s0[0], dims[0], c0, keep), 1)
         s1 = conv("e1b", *t, 1)
-        t = conv("e2a", *self._pool(s1[0], dims[1], c1), 2)
+        t = conv("e2a", *self._pool(s1[0], dims[1], c1, keep), 2)
         s2 = conv("e2b", *t, 2)
-        t = conv("e3a", *self._pool(s2[0], dims[2], c2), 3)
+        t = conv("e3a", *self._pool(s2[0], dims[2], c2, keep), 3)
         t = conv("e3b", *t, 3)
-        t = conv("d2a", *self._up(t[0], dims[2], c3), 2)
-        t = conv("d2b", *self._add(t[0], s2[0], c2), 2)
-        t = conv("d1a", *self._up(t[0], dims[1], c2), 1)
-        t = conv("d1b", *self._add(t[0], s1[0], c1), 1)
-        t = conv("d0a", *self._up(t[0], dims[0], c1), 0)
-        t = conv("d0b", *self._add(t[0], s0[0], c0), 0)
+        t = conv("d2a", *self._up(t[0], dims[2], c3, keep), 2)
+        t = conv("d2b", *self._add(t[0], s2[0], c2, keep), 2)
+        t = conv("d1a", *self._up(t[0], dims[1], c2, keep), 1)
+        t = conv("d1b", *self._add(t[0], s1[0], c1, keep), 1)
+        t = conv("d0a", *self._up(t[0], dims[0], c1, keep), 0)
+        t = conv("d0b", *self._add(t[0], s0[0], c0, keep), 0)
         out, _ = conv("head", *t, 0, activate=False)
         return out, cache
 
