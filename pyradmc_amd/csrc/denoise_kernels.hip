@@ -45,14 +45,28 @@ __device__ inline bf16x8 zero8() {
 
 __device__ inline float act(float v, float slope) { return v >= 0.0f ? v : v * slope; }
 
+// Where a convolution reads its input from (inference fuses the preceding element-wise
+// step into the operand loads; the values are those the separate kernels would store):
+// kSrcPlain x itself; kSrcUp the nearest 2x upsampling of x, which has half the output
+// dims (k_up_fwd); kSrcAdd x + x2 rounded to bf16 per element (k_add).
+constexpr int kSrcPlain = 0, kSrcUp = 1, kSrcAdd = 2;
+
+__device__ inline bf16x8 add8(bf16x8 a, bf16x8 b) {
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) r[e] = static_cast<__bf16>(static_cast<float>(a[e]) + static_cast<float>(b[e]));
+  return r;
+}
+
 // ------------------------------------------------------------------ convolution
 //
 // y[p][co] = act(bias[co] + sum_{tap, ci} x[p + off(tap)][ci] wt[tap][co][ci]) with
 // zero padding. One wave computes 16*TM output channels x 64 positions (4 blocks of
 // 16); a workgroup of 4 waves covers 256 consecutive positions (P % 64 == 0). K = 27*cin runs in
 // chunks of 32; each lane's 8 k values lie inside one tap since cin % 8 == 0.
-template <int TM>
+template <int TM, int SRC>
 __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
+                                                  const __bf16* __restrict__ x2,
                                                   const __bf16* __restrict__ wt,
                                                   const float* __restrict__ bias,
                                                   __bf16* __restrict__ y_nd,
@@ -100,8 +114,15 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
       for (int nb = 0; nb < 4; ++nb) {
         const int zz = pz[nb] + dz, yy = py[nb] + dy, xx = px[nb] + dx;
         if (zz >= 0 && zz < d && yy >= 0 && yy < h && xx >= 0 && xx < w) {
-          const long long q = ((static_cast<long long>(pn[nb]) * d + zz) * h + yy) * w + xx;
-          b[nb] = load8(x + q * cin + c);
+          if constexpr (SRC == kSrcUp) {
+            const long long q =
+                ((static_cast<long long>(pn[nb]) * (d >> 1) + (zz >> 1)) * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1);
+            b[nb] = load8(x + q * cin + c);
+          } else {
+            const long long q = ((static_cast<long long>(pn[nb]) * d + zz) * h + yy) * w + xx;
+            b[nb] = load8(x + q * cin + c);
+            if constexpr (SRC == kSrcAdd) b[nb] = add8(b[nb], load8(x2 + q * cin + c));
+          }
         } else {
           b[nb] = zero8();
         }
@@ -149,8 +170,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
 // instead of a gathered global load: the kernel reads each input element from HBM
 // once per workgroup instead of once per tap. The k-loop, MFMA operands and output
 // are those of k_conv_fwd, so the two kernels give identical sums.
-template <int TM, int CIN>
+template <int TM, int CIN, int SRC>
 __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict__ x,
+                                                       const __bf16* __restrict__ x2,
                                                        const __bf16* __restrict__ wt,
                                                        const float* __restrict__ bias,
                                                        __bf16* __restrict__ y_nd,
@@ -171,8 +193,21 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
     const int gz = z + zz - 1, gy = h0 + yy - 1, gx = xx - 1;
     u32x4 v = u32x4{0u, 0u, 0u, 0u};
     if (gz >= 0 && gz < d && gy >= 0 && gy < h && gx >= 0 && gx < kW) {
-      const long long q = ((static_cast<long long>(nn) * d + gz) * h + gy) * kW + gx;
-      v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
+      if constexpr (SRC == kSrcUp) {
+        const long long q =
+            ((static_cast<long long>(nn) * (d >> 1) + (gz >> 1)) * (h >> 1) + (gy >> 1)) * (kW >> 1) + (gx >> 1);
+        v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
+      } else {
+        const long long q = ((static_cast<long long>(nn) * d + gz) * h + gy) * kW + gx;
+        v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
+        if constexpr (SRC == kSrcAdd) {
+          bf16x8 a8, b8;
+          std::memcpy(&a8, &v, 16);
+          b8 = load8(x2 + q * CIN + 8 * c8);
+          a8 = add8(a8, b8);
+          std::memcpy(&v, &a8, 16);
+        }
+      }
     }
     tile[i] = v;
   }
@@ -610,41 +645,67 @@ bool shape_ok(int n, int d, int h, int w) { return n > 0 && d > 0 && h > 0 && w 
 
 extern "C" {
 
-int hdn_conv3d_fwd(const void* x, const void* wt, const float* bias, void* y_nd, void* y_cm, int n,
-                   int d, int h, int w, int cin, int cout, float slope, void* stream) {
-  if (!x || !wt || !y_nd || !shape_ok(n, d, h, w)) return HDN_EINVAL;
-  const long long P = static_cast<long long>(n) * d * h * w;
-  if (P % 64 != 0 || cin % 16 != 0 || cin < 16 || cout % 16 != 0 || cout < 16) return HDN_EINVAL;
-  const __bf16* xb = static_cast<const __bf16*>(x);
-  const __bf16* wb = static_cast<const __bf16*>(wt);
-  __bf16* yb = static_cast<__bf16*>(y_nd);
-  __bf16* cb = static_cast<__bf16*>(y_cm);
+}  // extern "C"
+
+namespace {
+
+template <int SRC>
+int conv_fwd(const __bf16* xb, const __bf16* x2b, const __bf16* wb, const float* bias, __bf16* yb, __bf16* cb,
+             int n, int d, int h, int w, int cin, int cout, float slope, hipStream_t stream) {
   if (w == 64 && h % 4 == 0 && (cin == 16 || cin == 32) && (cout == 16 || (cout == 32 && cin == 16))) {
     // Full-resolution layers: input staged in LDS (k_conv_fwd_tile).
     const dim3 tiles(static_cast<unsigned int>(static_cast<long long>(n) * d * (h / 4)));
     if (cin == 16 && cout == 32)
-      hipLaunchKernelGGL((k_conv_fwd_tile<2, 16>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+      hipLaunchKernelGGL((k_conv_fwd_tile<2, 16, SRC>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n,
                          d, h, cout, slope);
     else if (cin == 16)
-      hipLaunchKernelGGL((k_conv_fwd_tile<1, 16>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+      hipLaunchKernelGGL((k_conv_fwd_tile<1, 16, SRC>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n,
                          d, h, cout, slope);
     else
-      hipLaunchKernelGGL((k_conv_fwd_tile<1, 32>), tiles, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb, n,
+      hipLaunchKernelGGL((k_conv_fwd_tile<1, 32, SRC>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n,
                          d, h, cout, slope);
     return launch_status();
   }
+  const long long P = static_cast<long long>(n) * d * h * w;
   const int tm = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
   const dim3 grid(static_cast<unsigned int>((P + 255) / 256), static_cast<unsigned int>(cout / (16 * tm)));
   if (tm == 4)
-    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
-                       n, d, h, w, cin, cout, slope);
+    hipLaunchKernelGGL((k_conv_fwd<4, SRC>), grid, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n, d, h, w,
+                       cin, cout, slope);
   else if (tm == 2)
-    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
-                       n, d, h, w, cin, cout, slope);
+    hipLaunchKernelGGL((k_conv_fwd<2, SRC>), grid, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n, d, h, w,
+                       cin, cout, slope);
   else
-    hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, as_stream(stream), xb, wb, bias, yb, cb,
-                       n, d, h, w, cin, cout, slope);
+    hipLaunchKernelGGL((k_conv_fwd<1, SRC>), grid, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n, d, h, w,
+                       cin, cout, slope);
   return launch_status();
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdn_conv3d_fwd_src(const void* x, const void* x2, int src, const void* wt, const float* bias, void* y_nd,
+                       void* y_cm, int n, int d, int h, int w, int cin, int cout, float slope, void* stream) {
+  if (!x || !wt || !y_nd || !shape_ok(n, d, h, w)) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  if (P % 64 != 0 || cin % 16 != 0 || cin < 16 || cout % 16 != 0 || cout < 16) return HDN_EINVAL;
+  if (src == kSrcUp && (d % 2 || h % 2 || w % 2)) return HDN_EINVAL;
+  if (src == kSrcAdd && !x2) return HDN_EINVAL;
+  if (src != kSrcPlain && src != kSrcUp && src != kSrcAdd) return HDN_EINVAL;
+  const __bf16* xb = static_cast<const __bf16*>(x);
+  const __bf16* x2b = static_cast<const __bf16*>(x2);
+  const __bf16* wb = static_cast<const __bf16*>(wt);
+  __bf16* yb = static_cast<__bf16*>(y_nd);
+  __bf16* cb = static_cast<__bf16*>(y_cm);
+  if (src == kSrcUp) return conv_fwd<kSrcUp>(xb, nullptr, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
+  if (src == kSrcAdd) return conv_fwd<kSrcAdd>(xb, x2b, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
+  return conv_fwd<kSrcPlain>(xb, nullptr, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
+}
+
+int hdn_conv3d_fwd(const void* x, const void* wt, const float* bias, void* y_nd, void* y_cm, int n,
+                   int d, int h, int w, int cin, int cout, float slope, void* stream) {
+  return hdn_conv3d_fwd_src(x, nullptr, kSrcPlain, wt, bias, y_nd, y_cm, n, d, h, w, cin, cout, slope, stream);
 }
 
 int hdn_conv3d_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d,

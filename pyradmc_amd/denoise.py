@@ -59,6 +59,8 @@ def _lib():
     i, f = ctypes.c_int, ctypes.c_float
     sig = {
         "hdn_conv3d_fwd": [_c_void, _c_void, _c_void, _c_void, _c_void, i, i, i, i, i, i, f, _c_void],
+        "hdn_conv3d_fwd_src": [_c_void, _c_void, i, _c_void, _c_void, _c_void, _c_void, i, i, i, i, i, i, f,
+                               _c_void],
         "hdn_conv3d_wgrad": [_c_void, _c_void, _c_void, i, i, i, i, i, i, i, _c_void],
         "hdn_reduce_splits": [_c_void, _c_void, _c_ll, i, _c_void],
         "hdn_channel_sum": [_c_void, _c_void, _c_ll, i, i, _c_void],
@@ -221,13 +223,15 @@ class Conv3dDenoiser:
 
     # ---------------------------------------------------------------- forward
 
-    def _conv(self, lay: _Layer, x, dims, activate: bool, want_cm: bool):
+    def _conv(self, lay: _Layer, x, dims, activate: bool, want_cm: bool, x2=None, src: int = 0):
+        """One convolution at output dims ``dims``; ``src`` 1 reads the nearest 2x
+        upsampling of ``x`` (half the dims), 2 reads ``x + x2`` (include/hipdn.h)."""
         torch = _torch()
         n, d, h, w = dims
         y = torch.empty((n, d, h, w, lay.cout), dtype=torch.bfloat16, device=x.device)
         y_cm = torch.empty((lay.cout, n * d * h * w), dtype=torch.bfloat16, device=x.device) if want_cm else None
-        _call("hdn_conv3d_fwd", _ptr(x), _ptr(lay.wf), _ptr(lay.b), _ptr(y), _ptr(y_cm), n, d, h, w,
-              lay.cin, lay.cout, SLOPE if activate else 1.0, self._stream())
+        _call("hdn_conv3d_fwd_src", _ptr(x), _ptr(x2), int(src), _ptr(lay.wf), _ptr(lay.b), _ptr(y), _ptr(y_cm),
+              n, d, h, w, lay.cin, lay.cout, SLOPE if activate else 1.0, self._stream())
         return y, y_cm
 
     def _pool(self, x, dims, C, want_cm: bool = True):
@@ -270,23 +274,43 @@ class Conv3dDenoiser:
                 cache[name] = (inp, inp_cm, y, y_cm, lvl)
             return y, y_cm
 
+        if not keep:
+            return self._infer(x, dims), cache
         h0 = conv("e0a", x, x_cm, 0)
         s0 = conv("e0b", *h0, 0)
-        # The channel-major copies feed only the weight gradients: inference skips them.
-        t = conv("e1a", *self._pool(s0[0], dims[0], c0, keep), 1)
+        t = conv("e1a", *self._pool(s0[0], dims[0], c0), 1)
         s1 = conv("e1b", *t, 1)
-        t = conv("e2a", *self._pool(s1[0], dims[1], c1, keep), 2)
+        t = conv("e2a", *self._pool(s1[0], dims[1], c1), 2)
         s2 = conv("e2b", *t, 2)
-        t = conv("e3a", *self._pool(s2[0], dims[2], c2, keep), 3)
+        t = conv("e3a", *self._pool(s2[0], dims[2], c2), 3)
         t = conv("e3b", *t, 3)
-        t = conv("d2a", *self._up(t[0], dims[2], c3, keep), 2)
-        t = conv("d2b", *self._add(t[0], s2[0], c2, keep), 2)
-        t = conv("d1a", *self._up(t[0], dims[1], c2, keep), 1)
-        t = conv("d1b", *self._add(t[0], s1[0], c1, keep), 1)
-        t = conv("d0a", *self._up(t[0], dims[0], c1, keep), 0)
-        t = conv("d0b", *self._add(t[0], s0[0], c0, keep), 0)
+        t = conv("d2a", *self._up(t[0], dims[2], c3), 2)
+        t = conv("d2b", *self._add(t[0], s2[0], c2), 2)
+        t = conv("d1a", *self._up(t[0], dims[1], c2), 1)
+        t = conv("d1b", *self._add(t[0], s1[0], c1), 1)
+        t = conv("d0a", *self._up(t[0], dims[0], c1), 0)
+        t = conv("d0b", *self._add(t[0], s0[0], c0), 0)
         out, _ = conv("head", *t, 0, activate=False)
         return out, cache
+
+    def _infer(self, x, dims):
+        """The forward pass without anything kept for the backward: no channel-major
+        copies, and each upsampling or skip addition read by the next convolution
+        instead of stored (identical values, so identical outputs)."""
+        L = self.layers
+        c0, c1, c2, _ = self.widths
+
+        def conv(name, inp, lvl, activate=True, x2=None, src=0):
+            return self._conv(L[name], inp, dims[lvl], activate, False, x2=x2, src=src)[0]
+
+        s0 = conv("e0b", conv("e0a", x, 0), 0)
+        s1 = conv("e1b", conv("e1a", self._pool(s0, dims[0], c0, False)[0], 1), 1)
+        s2 = conv("e2b", conv("e2a", self._pool(s1, dims[1], c1, False)[0], 2), 2)
+        t = conv("e3b", conv("e3a", self._pool(s2, dims[2], c2, False)[0], 3), 3)
+        t = conv("d2b", conv("d2a", t, 2, src=1), 2, x2=s2, src=2)
+        t = conv("d1b", conv("d1a", t, 1, src=1), 1, x2=s1, src=2)
+        t = conv("d0b", conv("d0a", t, 0, src=1), 0, x2=s0, src=2)
+        return conv("head", t, 0, activate=False)
 
     # ---------------------------------------------------------------- backward
 

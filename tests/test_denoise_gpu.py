@@ -345,3 +345,23 @@ def test_denoised_builder_float32_values_are_the_float64_values_rounded(dn):
     np.testing.assert_array_equal(narrow.indices, wide.indices)
     np.testing.assert_array_equal(narrow.dose, wide.dose.astype(np.float32))
     np.testing.assert_array_equal(narrow.sigma, wide.sigma.astype(np.float32))
+
+
+@pytest.mark.parametrize("widths", [(16, 32, 64, 128), (16, 16, 32, 32)])
+def test_inference_with_fused_inputs_equals_the_stored_forward(dn, widths):
+    # Inference reads each upsampling and skip addition inside the next convolution
+    # (full-resolution tiled and general kernels alike); the values are those the
+    # separate kernels store, so the head output is the same bit for bit.
+    import torch
+
+    model = dn.Conv3dDenoiser(widths=widths, seed=11)
+    rng = np.random.default_rng(12)
+    shape = (2, 16, 32, 64)
+    x, x_cm = dn.pack_input(rng.random(shape).astype(np.float32),
+                            rng.uniform(0.1, 0.9, shape).astype(np.float32))
+    stored, _ = model.forward(x, x_cm, keep=True)
+    fused, _ = model.forward(x, None, keep=False)
+    torch.cuda.synchronize()
+    assert stored.dtype == fused.dtype and stored.shape == fused.shape
+    assert torch.equal(stored.view(torch.int16), fused.view(torch.int16))
+    assert torch.count_nonzero(stored.float()) > 0
