@@ -162,15 +162,16 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
   }
 }
 
-// The same convolution for the full-resolution layers (w == 64, h % 4 == 0, cin 16 or
-// 32, cout 16 x TM): a workgroup computes 4 consecutive h-rows of one z-slice (256
-// positions; wave r takes row r) and first stages the input those rows read - 3 z x
-// 6 h x 66 w positions with their one-voxel halo, zero outside the volume - in LDS
-// (38 KB for cin 16, 76 KB for cin 32). Every tap's operand then comes from LDS
-// instead of a gathered global load: the kernel reads each input element from HBM
-// once per workgroup instead of once per tap. The k-loop, MFMA operands and output
-// are those of k_conv_fwd, so the two kernels give identical sums.
-template <int TM, int CIN, int SRC>
+// The same convolution for rows of KW = 64 or 32 voxels (w == KW, h % (256 / KW) == 0,
+// cout 16 x TM): a workgroup computes 256 / KW consecutive h-rows of one z-slice (256
+// positions; each wave 64 of them) and first stages the input those rows read - 3 z x
+// (rows + 2) h x (KW + 2) w positions with their one-voxel halo, zero outside the
+// volume - in LDS (KW 64: 38 KB for cin 16, 76 KB for cin 32; KW 32: 33, 65 or 130 KB
+// for cin 16, 32 or 64). Every tap's operand then comes from LDS instead of a gathered
+// global load: the kernel reads each input element from HBM once per workgroup instead
+// of once per tap. The k-loop, MFMA operands and output are those of k_conv_fwd, so
+// the two kernels give identical sums.
+template <int TM, int CIN, int SRC, int KW = 64>
 __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict__ x,
                                                        const __bf16* __restrict__ x2,
                                                        const __bf16* __restrict__ wt,
@@ -178,14 +179,14 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
                                                        __bf16* __restrict__ y_nd,
                                                        __bf16* __restrict__ y_cm, int n, int d, int h,
                                                        int cout, float slope) {
-  constexpr int kW = 64, kTW = kW + 2, kTH = 6, kTZ = 3, kC8 = CIN / 8;
+  constexpr int kW = KW, kRows = 256 / KW, kTW = kW + 2, kTH = kRows + 2, kTZ = 3, kC8 = CIN / 8;
   __shared__ u32x4 tile[kTZ * kTH * kTW * kC8];
-  const int rows = h / 4;
+  const int rows = h / kRows;
   const int hb = static_cast<int>(blockIdx.x % rows);
   const long long t = blockIdx.x / rows;
   const int z = static_cast<int>(t % d);
   const int nn = static_cast<int>(t / d);
-  const int h0 = 4 * hb;
+  const int h0 = kRows * hb;
   for (int i = threadIdx.x; i < kTZ * kTH * kTW * kC8; i += 256) {
     const int c8 = i % kC8;
     const int pos = i / kC8;
@@ -214,9 +215,16 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
-  const int row = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6;
   const int col = lane & 15;
   const int kg = lane >> 4;
+  // Position block nb (16 positions) of this wave: row and first x within the rows.
+  int blk_row[4], blk_x[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    blk_row[nb] = wave * (64 / kW) + (16 * nb) / kW;
+    blk_x[nb] = (16 * nb) % kW;
+  }
   f32x4 acc[TM][4];
 #pragma unroll
   for (int m = 0; m < TM; ++m)
@@ -234,10 +242,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
 #pragma unroll
       for (int m = 0; m < TM; ++m)
         a[m] = load8(wt + (static_cast<long long>(tap) * cout + 16 * m + col) * CIN + c);
-      const int base = ((dz * kTH + row + dy) * kTW + dx + col) * kC8 + c / 8;
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
-        const u32x4 v = tile[base + 16 * nb * kC8];
+        const u32x4 v = tile[((dz * kTH + blk_row[nb] + dy) * kTW + dx + blk_x[nb] + col) * kC8 + c / 8];
         std::memcpy(&b[nb], &v, 16);
       }
     } else {
@@ -253,13 +260,13 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
   }
 
   const long long P = static_cast<long long>(n) * d * h * kW;
-  const long long p0 = ((static_cast<long long>(nn) * d + z) * h + h0 + row) * kW;
+  const long long row0 = (static_cast<long long>(nn) * d + z) * h + h0;
 #pragma unroll
   for (int m = 0; m < TM; ++m) {
     const int cb = 16 * m + 4 * kg;
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
-      const long long p = p0 + 16 * nb + col;
+      const long long p = (row0 + blk_row[nb]) * kW + blk_x[nb] + col;
       __bf16 out[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -664,6 +671,20 @@ int conv_fwd(const __bf16* xb, const __bf16* x2b, const __bf16* wb, const float*
     else
       hipLaunchKernelGGL((k_conv_fwd_tile<1, 32, SRC>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb, n,
                          d, h, cout, slope);
+    return launch_status();
+  }
+  if (w == 32 && h % 8 == 0 && (cin == 16 || cin == 32 || cin == 64) && cout == 32) {
+    // Half-resolution layers: the same staging, 8 rows of 32 per workgroup.
+    const dim3 tiles(static_cast<unsigned int>(static_cast<long long>(n) * d * (h / 8)));
+    if (cin == 16)
+      hipLaunchKernelGGL((k_conv_fwd_tile<2, 16, SRC, 32>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb,
+                         n, d, h, cout, slope);
+    else if (cin == 32)
+      hipLaunchKernelGGL((k_conv_fwd_tile<2, 32, SRC, 32>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb,
+                         n, d, h, cout, slope);
+    else
+      hipLaunchKernelGGL((k_conv_fwd_tile<2, 64, SRC, 32>), tiles, dim3(256), 0, stream, xb, x2b, wb, bias, yb, cb,
+                         n, d, h, cout, slope);
     return launch_status();
   }
   const long long P = static_cast<long long>(n) * d * h * w;

@@ -60,12 +60,15 @@ def _close_bf16(got, ref):
     np.testing.assert_allclose(got, ref, rtol=BF16_RTOL, atol=BF16_RTOL * 0.05 * scale)
 
 
-# The last three shapes (w = 64, h % 4 == 0) run the LDS-tiled kernel of the
-# full-resolution layers, forward and data gradient.
+# Shapes with w = 64 (h % 4 == 0) run the LDS-tiled kernel of the full-resolution
+# layers, forward and data gradient; w = 32 (h % 8 == 0, cout 32) its half-resolution
+# form in forward.
 @pytest.mark.parametrize("shape,cin,cout", [((1, 8, 8, 16), 16, 16), ((1, 4, 8, 8), 32, 48),
                                             ((2, 4, 4, 16), 16, 32), ((1, 4, 4, 8), 32, 64),
                                             ((1, 4, 4, 8), 16, 128), ((1, 3, 4, 64), 16, 16),
-                                            ((2, 2, 8, 64), 16, 32), ((1, 3, 4, 64), 32, 16)])
+                                            ((2, 2, 8, 64), 16, 32), ((1, 3, 4, 64), 32, 16),
+                                            ((1, 3, 8, 32), 16, 32), ((2, 2, 8, 32), 32, 32),
+                                            ((1, 2, 16, 32), 64, 32)])
 def test_conv_forward_data_and_weight_gradients(dn, shape, cin, cout):
     import torch
 
