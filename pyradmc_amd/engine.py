@@ -631,7 +631,13 @@ class HipEngine:
 
         ``value_dtype`` (an extension; float64 as in the reference by default) may be
         ``np.float32`` for the CSC dose and sigma arrays, halving a large Dij's host
-        footprint; indices stay int64 and the device statistics stay float64."""
+        footprint; indices stay int64 and the device statistics stay float64. Arrays of
+        1 MiB or more are page-locked host memory (released with their last view).
+
+        ``counters`` holds the summed transport ledgers (integer event counts and
+        quanta) and, as floats, the host wall split of this call: ``transport_s``
+        (transport, fold and finalize), ``assemble_s`` (truncation, download, join) and
+        ``groups`` (beamlet groups run)."""
         value_dtype = np.dtype(value_dtype)
         if value_dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
             raise ValueError(f"value_dtype must be float64 or float32, got {value_dtype}")
