@@ -475,7 +475,7 @@ def main() -> int:
     elif not args.no_autotune:
         # Launch shape and chunk size for this workload (untimed; results do not depend on them).
         print("bench: autotune", file=sys.stderr, flush=True)
-        tuned = engine.autotune(src, args.histories, args.batches, seed=999, repeats=2, use_cache=False)
+        tuned = engine.autotune(src, args.histories, args.batches, seed=999, repeats=3, use_cache=False)
     for w in range(args.warmup):
         engine.run(src, args.histories, args.batches, seed=1000 + w)
     engine.set_profiling(True)
