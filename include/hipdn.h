@@ -33,10 +33,13 @@ int hdn_conv3d_fwd(const void* x, const void* w, const float* bias, void* y_nd, 
 /* The same convolution reading its input through the preceding element-wise step, for
  * inference (the values equal those of the separate calls, so the outputs are
  * identical): src 0 reads x; src 1 (HDN_SRC_UP) reads hdn_up_fwd(x), x having half
- * the output dims d, h, w (all even); src 2 (HDN_SRC_ADD) reads hdn_add(x, x2). */
+ * the output dims d, h, w (all even); src 2 (HDN_SRC_ADD) reads hdn_add(x, x2); src 3
+ * (HDN_SRC_LOW8) reads x whose channels beyond the first 8 are zero (the padded network
+ * input), skipping them in the reduction. */
 #define HDN_SRC_PLAIN 0
 #define HDN_SRC_UP 1
 #define HDN_SRC_ADD 2
+#define HDN_SRC_LOW8 3
 int hdn_conv3d_fwd_src(const void* x, const void* x2, int src, const void* w, const float* bias, void* y_nd,
                        void* y_cm, int n, int d, int h, int w_, int cin, int cout, float slope, void* stream);
 

@@ -48,8 +48,10 @@ __device__ inline float act(float v, float slope) { return v >= 0.0f ? v : v * s
 // Where a convolution reads its input from (inference fuses the preceding element-wise
 // step into the operand loads; the values are those the separate kernels would store):
 // kSrcPlain x itself; kSrcUp the nearest 2x upsampling of x, which has half the output
-// dims (k_up_fwd); kSrcAdd x + x2 rounded to bf16 per element (k_add).
-constexpr int kSrcPlain = 0, kSrcUp = 1, kSrcAdd = 2;
+// dims (k_up_fwd); kSrcAdd x + x2 rounded to bf16 per element (k_add); kSrcLow8 x whose
+// channels beyond the first 8 are zero (the network's padded input), which the
+// reduction then skips.
+constexpr int kSrcPlain = 0, kSrcUp = 1, kSrcAdd = 2, kSrcLow8 = 3;
 
 __device__ inline bf16x8 add8(bf16x8 a, bf16x8 b) {
   bf16x8 r;
@@ -99,13 +101,14 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const __bf16* __restrict__ x,
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) acc[m][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  const int K = 27 * cin;
+  const int kc = SRC == kSrcLow8 ? 8 : cin;  // channels read per tap
+  const int K = 27 * kc;
   for (int k0 = 0; k0 < K; k0 += 32) {
     const int k = k0 + 8 * kg;
     bf16x8 a[TM], b[4];
     if (k < K) {
-      const int tap = k / cin;
-      const int c = k - tap * cin;
+      const int tap = k / kc;
+      const int c = k - tap * kc;
       const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
 #pragma unroll
       for (int m = 0; m < TM; ++m)
@@ -231,13 +234,14 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) acc[m][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  constexpr int K = 27 * CIN;
+  constexpr int kKC = SRC == kSrcLow8 ? 8 : CIN;  // channels read per tap
+  constexpr int K = 27 * kKC;
   for (int k0 = 0; k0 < K; k0 += 32) {
     const int k = k0 + 8 * kg;
     bf16x8 a[TM], b[4];
     if (k < K) {
-      const int tap = k / CIN;
-      const int c = k - tap * CIN;
+      const int tap = k / kKC;
+      const int c = k - tap * kKC;
       const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // halo-shifted: 0..2
 #pragma unroll
       for (int m = 0; m < TM; ++m)
@@ -713,7 +717,7 @@ int hdn_conv3d_fwd_src(const void* x, const void* x2, int src, const void* wt, c
   if (P % 64 != 0 || cin % 16 != 0 || cin < 16 || cout % 16 != 0 || cout < 16) return HDN_EINVAL;
   if (src == kSrcUp && (d % 2 || h % 2 || w % 2)) return HDN_EINVAL;
   if (src == kSrcAdd && !x2) return HDN_EINVAL;
-  if (src != kSrcPlain && src != kSrcUp && src != kSrcAdd) return HDN_EINVAL;
+  if (src != kSrcPlain && src != kSrcUp && src != kSrcAdd && src != kSrcLow8) return HDN_EINVAL;
   const __bf16* xb = static_cast<const __bf16*>(x);
   const __bf16* x2b = static_cast<const __bf16*>(x2);
   const __bf16* wb = static_cast<const __bf16*>(wt);
@@ -721,6 +725,7 @@ int hdn_conv3d_fwd_src(const void* x, const void* x2, int src, const void* wt, c
   __bf16* cb = static_cast<__bf16*>(y_cm);
   if (src == kSrcUp) return conv_fwd<kSrcUp>(xb, nullptr, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
   if (src == kSrcAdd) return conv_fwd<kSrcAdd>(xb, x2b, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
+  if (src == kSrcLow8) return conv_fwd<kSrcLow8>(xb, nullptr, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
   return conv_fwd<kSrcPlain>(xb, nullptr, wb, bias, yb, cb, n, d, h, w, cin, cout, slope, as_stream(stream));
 }
 

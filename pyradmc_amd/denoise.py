@@ -225,7 +225,8 @@ class Conv3dDenoiser:
 
     def _conv(self, lay: _Layer, x, dims, activate: bool, want_cm: bool, x2=None, src: int = 0):
         """One convolution at output dims ``dims``; ``src`` 1 reads the nearest 2x
-        upsampling of ``x`` (half the dims), 2 reads ``x + x2`` (include/hipdn.h)."""
+        upsampling of ``x`` (half the dims), 2 reads ``x + x2``, 3 reads ``x`` whose
+        channels beyond the first 8 are zero (include/hipdn.h)."""
         torch = _torch()
         n, d, h, w = dims
         y = torch.empty((n, d, h, w, lay.cout), dtype=torch.bfloat16, device=x.device)
@@ -269,7 +270,9 @@ class Conv3dDenoiser:
         cache = {"x": (x, x_cm), "dims": dims}
 
         def conv(name, inp, inp_cm, lvl, activate=True):
-            y, y_cm = self._conv(L[name], inp, dims[lvl], activate, want_cm=keep)
+            # The first layer's input has 2 real channels of 16: its reduction skips the
+            # zero channel group (src 3).
+            y, y_cm = self._conv(L[name], inp, dims[lvl], activate, want_cm=keep, src=3 if name == "e0a" else 0)
             if keep:
                 cache[name] = (inp, inp_cm, y, y_cm, lvl)
             return y, y_cm
@@ -303,7 +306,7 @@ class Conv3dDenoiser:
         def conv(name, inp, lvl, activate=True, x2=None, src=0):
             return self._conv(L[name], inp, dims[lvl], activate, False, x2=x2, src=src)[0]
 
-        s0 = conv("e0b", conv("e0a", x, 0), 0)
+        s0 = conv("e0b", conv("e0a", x, 0, src=3), 0)
         s1 = conv("e1b", conv("e1a", self._pool(s0, dims[0], c0, False)[0], 1), 1)
         s2 = conv("e2b", conv("e2a", self._pool(s1, dims[1], c1, False)[0], 2), 2)
         t = conv("e3b", conv("e3a", self._pool(s2, dims[2], c2, False)[0], 3), 3)

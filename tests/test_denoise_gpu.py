@@ -368,3 +368,21 @@ def test_inference_with_fused_inputs_equals_the_stored_forward(dn, widths):
     assert stored.dtype == fused.dtype and stored.shape == fused.shape
     assert torch.equal(stored.view(torch.int16), fused.view(torch.int16))
     assert torch.count_nonzero(stored.float()) > 0
+
+
+@pytest.mark.parametrize("shape", [(1, 4, 8, 64), (2, 4, 4, 16)])
+def test_first_layer_skips_the_zero_channel_group(dn, shape):
+    # src 3 reads only the first 8 input channels; with the rest zero (the network's
+    # padded input) the outputs equal the full reduction's (tiled and general kernels).
+    import torch
+
+    model = dn.Conv3dDenoiser(seed=4)
+    lay = model.layers["e0a"]
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(shape + (16,)).astype(np.float32)
+    x[..., 8:] = 0.0
+    xd = torch.from_numpy(x).to(torch.bfloat16).to("cuda:0")
+    full, _ = model._conv(lay, xd, shape, True, False, src=0)
+    low, _ = model._conv(lay, xd, shape, True, False, src=3)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(low.float().cpu().numpy(), full.float().cpu().numpy())
