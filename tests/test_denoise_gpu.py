@@ -386,3 +386,19 @@ def test_first_layer_skips_the_zero_channel_group(dn, shape):
     low, _ = model._conv(lay, xd, shape, True, False, src=3)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(low.float().cpu().numpy(), full.float().cpu().numpy())
+
+
+def test_fused_convolution_arguments_rejected(dn):
+    # Checked on the host before any launch: unknown source mode, an upsampled source
+    # with odd output dims, an added source without its second input.
+    import torch
+
+    model = dn.Conv3dDenoiser(seed=4)
+    lay = model.layers["e0b"]
+    x = torch.zeros((1, 4, 4, 16, 16), dtype=torch.bfloat16, device="cuda:0")
+    with pytest.raises(ValueError):
+        model._conv(lay, x, (1, 4, 4, 16), True, False, src=7)
+    with pytest.raises(ValueError):
+        model._conv(lay, x, (1, 3, 4, 16), True, False, src=1)
+    with pytest.raises(ValueError):
+        model._conv(lay, x, (1, 4, 4, 16), True, False, x2=None, src=2)
