@@ -14,6 +14,7 @@ to torch's runtime and the two share devices, streams and memory.
 from __future__ import annotations
 
 import ctypes
+import threading
 from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_uint8, c_uint64, c_void_p
 from pathlib import Path
 
@@ -246,30 +247,94 @@ class Context:
         check(rc, self.handle, name)
 
 
+# Released page-locked blocks kept for reuse. Pinning and unpinning memory costs tens of
+# microseconds per MiB and the unpin waits for the device, so a loop of runs that each
+# return a large result would otherwise pay both per run. At most this many bytes stay
+# cached; ``release_pinned_cache`` returns them to the system.
+PINNED_CACHE_MAX_BYTES = 256 << 20
+_PAGE = 4096
+_pinned_lock = threading.RLock()  # reentrant: a block's __del__ may run inside a take
+_pinned_cache: list[tuple[int, int]] = []  # (bytes, address)
+_pinned_cached_bytes = 0
+
+
+def _pinned_take(nbytes: int) -> tuple[int, int]:
+    """A page-locked block of at least ``nbytes``: a cached one no more than twice that
+    size, else a new allocation. Returns (bytes, address)."""
+    global _pinned_cached_bytes
+    with _pinned_lock:
+        best = -1
+        for k in range(len(_pinned_cache)):
+            size = _pinned_cache[k][0]
+            if nbytes <= size <= 2 * nbytes and (best < 0 or size < _pinned_cache[best][0]):
+                best = k
+        if best >= 0:
+            size, address = _pinned_cache.pop(best)
+            _pinned_cached_bytes -= size
+            return size, address
+    size = -(-max(1, nbytes) // _PAGE) * _PAGE
+    ptr = c_void_p()
+    rc = lib().hmc_host_alloc(size, ctypes.byref(ptr))
+    if rc != 0 or not ptr.value:
+        release_pinned_cache()  # cached blocks may be what the system is short of
+        rc = lib().hmc_host_alloc(size, ctypes.byref(ptr))
+        if rc != 0 or not ptr.value:
+            raise MemoryError(f"page-locked host allocation of {nbytes} bytes failed")
+    return size, ptr.value
+
+
+def _pinned_give(size: int, address: int) -> None:
+    global _pinned_cached_bytes
+    if _lib is None:
+        return
+    with _pinned_lock:
+        if _pinned_cached_bytes + size <= PINNED_CACHE_MAX_BYTES:
+            _pinned_cache.append((size, address))
+            _pinned_cached_bytes += size
+            return
+    _lib.hmc_host_free(address)
+
+
+def release_pinned_cache() -> int:
+    """Free the cached page-locked blocks that no array uses any more; returns the bytes
+    released."""
+    global _pinned_cached_bytes
+    with _pinned_lock:
+        blocks = list(_pinned_cache)
+        _pinned_cache.clear()
+        _pinned_cached_bytes = 0
+    if _lib is not None:
+        for _, address in blocks:
+            _lib.hmc_host_free(address)
+    return sum(size for size, _ in blocks)
+
+
 class _PinnedBlock:
-    """One page-locked host allocation, exposed to NumPy; freed when the last array
-    viewing it goes away."""
+    """One page-locked host block, exposed to NumPy; handed back to the cache (or freed)
+    when the last array viewing it goes away."""
 
     def __init__(self, n: int, dtype) -> None:
         dt = np.dtype(dtype)
         self._ptr = None
-        ptr = c_void_p()
-        rc = lib().hmc_host_alloc(max(1, n * dt.itemsize), ctypes.byref(ptr))
-        if rc != 0 or not ptr.value:
-            raise MemoryError(f"page-locked host allocation of {n * dt.itemsize} bytes failed")
-        self._ptr = ptr.value
+        self._size, address = _pinned_take(n * dt.itemsize)
+        self._ptr = address
         self.__array_interface__ = {"shape": (n,), "typestr": dt.str, "data": (self._ptr, False), "version": 3}
 
     def __del__(self) -> None:
-        if self._ptr and _lib is not None:
-            _lib.hmc_host_free(self._ptr)
-            self._ptr = None
+        if self._ptr:
+            address, self._ptr = self._ptr, None
+            try:
+                _pinned_give(self._size, address)
+            except Exception:  # interpreter shutdown: the process releases its memory
+                pass
 
 
 def pinned_empty(n: int, dtype) -> np.ndarray:
     """An uninitialised 1-D array of ``n`` elements in page-locked host memory (the
     destination of a fetched CSC: device-to-host copies into it run at the link's rate).
-    It behaves as any NumPy array; the memory is released with the last view of it."""
+    It behaves as any NumPy array; when the last view of it goes away its memory goes
+    back to a bounded cache of page-locked blocks (``PINNED_CACHE_MAX_BYTES``) for the
+    next result."""
     return np.asarray(_PinnedBlock(int(n), dtype))
 
 

@@ -176,6 +176,11 @@ enum LedgerSlot {
   L_COUNT
 };
 
+// Word offsets in the context's page-locked readback block.
+constexpr int kStageLedger = 2;                     // after four u32 queue counts
+constexpr int kStageFolded = kStageLedger + L_COUNT;
+constexpr int kStageWords = kStageFolded + 1;
+
 struct TransportParams {
   DevGrid g;
   DevScoring s;
@@ -1341,6 +1346,9 @@ struct hmc_ctx {
   float* d_up = nullptr;
   int src_beamlets = 0;
   bool has_source = false;
+  // What d_edges, d_cdf and d_beamlet hold: a source set again with the same arrays
+  // (every run of a loop over seeds) uploads nothing.
+  std::vector<float> h_edges, h_cdf, h_beamlet;
 
   hmc_physics phys{};
   bool has_physics = false;
@@ -1350,6 +1358,10 @@ struct hmc_ctx {
   size_t q_capacity = 0;
   unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads
   unsigned long long* d_ledger = nullptr;
+  // Page-locked readback of the per-group counts (words 0-1: four u32), the ledger
+  // (words 2..) and the fold's quanta (the last word). Copies into page-locked memory
+  // need no host-side staging, so one wait covers them.
+  unsigned long long* h_stage = nullptr;
   DevItem* d_items = nullptr;
   size_t items_cap = 0;
   bool profile = false;
@@ -1460,7 +1472,12 @@ int hmc_create(int device, hmc_ctx** out) {
     delete ctx;
     return HMC_EHIP;
   }
-  if (dev_alloc(ctx, &ctx->d_counts, 8) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1)) {
+  if (dev_alloc(ctx, &ctx->d_counts, 8) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1) ||
+      hipHostMalloc(reinterpret_cast<void**>(&ctx->h_stage), sizeof(unsigned long long) * kStageWords,
+                    hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    free_ptr(ctx->d_counts);
+    free_ptr(ctx->d_ledger);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HMC_ENOMEM;
@@ -1486,6 +1503,7 @@ void hmc_destroy(hmc_ctx* ctx) {
   for (hipEvent_t e : ctx->ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (ctx->round_graph) (void)hipGraphExecDestroy(ctx->round_graph);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1739,8 +1757,16 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
   if (spectral) {
     if (s->n_bins < 1 || !s->edges || !s->cdf) return fail(ctx, HMC_EINVAL, "spectrum missing");
     std::vector<float> e(s->edges, s->edges + s->n_bins + 1), c(s->cdf, s->cdf + s->n_bins);
-    if ((rc = upload(ctx, &ctx->d_edges, e.data(), e.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_cdf, c.data(), c.size()))) return rc;
+    if (!ctx->d_edges || e != ctx->h_edges) {
+      ctx->h_edges.clear();
+      if ((rc = upload(ctx, &ctx->d_edges, e.data(), e.size()))) return rc;
+      ctx->h_edges = std::move(e);
+    }
+    if (!ctx->d_cdf || c != ctx->h_cdf) {
+      ctx->h_cdf.clear();
+      if ((rc = upload(ctx, &ctx->d_cdf, c.data(), c.size()))) return rc;
+      ctx->h_cdf = std::move(c);
+    }
     d.edges = ctx->d_edges;
     d.cdf = ctx->d_cdf;
   }
@@ -1749,7 +1775,11 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
     if (s->n_beamlets < 1 || !s->beamlet_params)
       return fail(ctx, HMC_EINVAL, "beamlet parameters missing");
     std::vector<float> b(s->beamlet_params, s->beamlet_params + s->n_beamlets * stride);
-    if ((rc = upload(ctx, &ctx->d_beamlet, b.data(), b.size()))) return rc;
+    if (!ctx->d_beamlet || b != ctx->h_beamlet) {
+      ctx->h_beamlet.clear();
+      if ((rc = upload(ctx, &ctx->d_beamlet, b.data(), b.size()))) return rc;
+      ctx->h_beamlet = std::move(b);
+    }
     d.beamlet = ctx->d_beamlet;
     ctx->src_beamlets = s->n_beamlets;
   }
@@ -2006,7 +2036,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   }
   unsigned int* counts = ctx->d_counts;
   // The ledger comes back with every count read (one host wait per group of rounds).
-  unsigned long long host_ledger[L_COUNT];
+  const unsigned long long* host_ledger = ctx->h_stage + kStageLedger;
 
   for (long long p0 = 0; p0 < total && !overflow; p0 += chunk_n) {
     const long long n_gen = std::min(chunk_n, total - p0);
@@ -2026,12 +2056,13 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
       rounds += kRoundsPerSync;
       launches[0] += kRoundsPerSync;
       launches[1] += kRoundsPerSync;
-      unsigned int host_counts[4];
-      HMC_HIP(ctx, hipMemcpyAsync(host_counts, counts, sizeof(host_counts), hipMemcpyDeviceToHost,
+      HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage, counts, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost,
                                   ctx->stream));
-      HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger), hipMemcpyDeviceToHost,
-                                  ctx->stream));
+      HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageLedger, ctx->d_ledger, sizeof(unsigned long long) * L_COUNT,
+                                  hipMemcpyDeviceToHost, ctx->stream));
       HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      unsigned int host_counts[4];
+      std::memcpy(host_counts, ctx->h_stage, sizeof(host_counts));
       if (ctx->profile) {
         for (int r = 0; r < kRoundsPerSync; ++r) {
           for (int k = 0; k < 2; ++k) {
@@ -2048,8 +2079,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     }
   }
   if (rounds == 0) {  // no histories: nothing ran, the ledger holds its cleared state
-    HMC_HIP(ctx, hipMemcpyAsync(host_ledger, ctx->d_ledger, sizeof(host_ledger), hipMemcpyDeviceToHost,
-                                ctx->stream));
+    HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageLedger, ctx->d_ledger, sizeof(unsigned long long) * L_COUNT,
+                                hipMemcpyDeviceToHost, ctx->stream));
     HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   overflow = std::max<long long>(overflow, static_cast<long long>(host_ledger[L_OVERFLOW]));
@@ -2106,9 +2137,10 @@ int hmc_fold(hmc_ctx* ctx, int n_slots_used, const double* nhist, int64_t* total
                      ctx->d_edep, ctx->d_mass, ctx->d_nhist, ctx->s.n_voxels, ctx->n_columns,
                      n_slots_used, ctx->d_s1, ctx->d_s2, folded);
   HMC_HIP(ctx, hipGetLastError());
-  unsigned long long q = 0;
-  HMC_HIP(ctx, hipMemcpyAsync(&q, folded, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
+  HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageFolded, folded, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const unsigned long long q = ctx->h_stage[kStageFolded];
   if (n_slots_used >= ctx->dirty_slots) ctx->dirty_slots = 0;  // the fold zeroed what it read
   if (total_quanta) *total_quanta = static_cast<int64_t>(q);
   return HMC_OK;

@@ -677,6 +677,35 @@ def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
     assert host_array(10, np.float64).base is None  # small: ordinary memory
     big = host_array(1 << 17, np.int64)
     assert isinstance(big.base, hip._PinnedBlock)
+    address = big.__array_interface__["data"][0]
+    del big  # the block goes back to the cache and serves the next result of its size
+    again = host_array(1 << 17, np.int64)
+    assert again.__array_interface__["data"][0] == address
+
+
+def test_source_arrays_set_again_change_nothing_and_new_ones_are_used():
+    # The library skips re-uploading a spectrum equal to the one on the device: one
+    # engine running spectra A, B, A must match fresh engines bit for bit.
+    from pyradmc_amd import SpectralParallelBeamSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+    from pyradmc_amd.geometry import bremsstrahlung_6mv_like
+
+    grid = VoxelGrid.uniform_water((16, 16, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    a = SpectralParallelBeamSource(bremsstrahlung_6mv_like(), -0.1, (2.0, 6.0), (2.0, 6.0))
+    b = SpectralParallelBeamSource(bremsstrahlung_6mv_like(n_bins=40, e_hi=6.0), -0.1, (2.0, 6.0), (2.0, 6.0))
+    eng = _engine(grid, xs)
+    runs = [eng.run(src, 80_000, 4, seed=21).dose for src in (a, b, a)]
+    eng.close()
+    fresh = {}
+    for name, src in (("a", a), ("b", b)):
+        e = _engine(grid, xs)
+        fresh[name] = e.run(src, 80_000, 4, seed=21).dose
+        e.close()
+    assert not np.array_equal(fresh["a"], fresh["b"])
+    np.testing.assert_array_equal(runs[0], fresh["a"])
+    np.testing.assert_array_equal(runs[1], fresh["b"])
+    np.testing.assert_array_equal(runs[2], fresh["a"])
 
 
 def test_fast_and_general_round_builds_agree_bit_for_bit():
