@@ -102,7 +102,8 @@ typedef struct hmc_ledger {
 int hmc_create(int device, hmc_ctx **out);
 void hmc_destroy(hmc_ctx *ctx);
 const char *hmc_last_error(const hmc_ctx *ctx);
-/* Device name, CU count and memory, for auto-sizing (engine.py:175-255). */
+/* Device name, CU count and memory, for auto-sizing (engine.py:175-255). Each output
+   may be NULL; with only free_bytes asked for, the device properties are not read. */
 int hmc_device_info(int device, char *name, int name_len, int *cu_count, int64_t *total_bytes,
                     int64_t *free_bytes);
 

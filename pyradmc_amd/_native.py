@@ -338,6 +338,14 @@ def pinned_empty(n: int, dtype) -> np.ndarray:
     return np.asarray(_PinnedBlock(int(n), dtype))
 
 
+def device_free_bytes(device: int = 0) -> int:
+    """Free device memory now (the cheap part of ``device_info``, for per-run sizing)."""
+    free = c_int64()
+    rc = lib().hmc_device_info(int(device), None, 0, None, None, ctypes.byref(free))
+    check(rc, None, "hmc_device_info")
+    return int(free.value)
+
+
 def device_info(device: int = 0) -> dict:
     h = lib()
     name = ctypes.create_string_buffer(256)

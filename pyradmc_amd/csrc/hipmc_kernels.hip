@@ -1514,16 +1514,18 @@ int hmc_device_info(int device, char* name, int name_len, int* cu_count, int64_t
                     int64_t* free_bytes) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HMC_EINVAL;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HMC_EHIP;
-  if (name && name_len > 0) {
-    // The marketing name can be empty where the ids table is not readable; the
-    // architecture and CU count always identify the part.
-    std::snprintf(name, static_cast<size_t>(name_len), "%s%s%s, %d CUs", prop.name,
-                  prop.name[0] ? " " : "", prop.gcnArchName, prop.multiProcessorCount);
+  if ((name && name_len > 0) || cu_count || total_bytes) {  // the properties query is the slow part
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HMC_EHIP;
+    if (name && name_len > 0) {
+      // The marketing name can be empty where the ids table is not readable; the
+      // architecture and CU count always identify the part.
+      std::snprintf(name, static_cast<size_t>(name_len), "%s%s%s, %d CUs", prop.name,
+                    prop.name[0] ? " " : "", prop.gcnArchName, prop.multiProcessorCount);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (total_bytes) *total_bytes = static_cast<int64_t>(prop.totalGlobalMem);
   }
-  if (cu_count) *cu_count = prop.multiProcessorCount;
-  if (total_bytes) *total_bytes = static_cast<int64_t>(prop.totalGlobalMem);
   if (free_bytes) {
     size_t f = 0, tot = 0;
     (void)hipSetDevice(device);

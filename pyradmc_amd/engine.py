@@ -187,7 +187,7 @@ class HipEngine:
             return int(min(self.chunk_size, max(total, 1)))
         target = min(_CHUNK_HISTORIES_PER_CU * max(self._info["cu_count"], 1), _CHUNK_SIZE_CAP)
         # Queue memory is budgeted against what is free now (other engines may be alive).
-        free = int(nat.device_info(self._index).get("free_bytes", 0))
+        free = nat.device_free_bytes(self._index)
         if free > 0:
             target = min(target, int(free * _CHUNK_MEMORY_FRACTION) // (self.queue_factor * _QUEUE_BYTES_PER_SLOT))
         return int(max(1, min(target, max(total, 1))))

@@ -683,6 +683,14 @@ def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
     assert again.__array_interface__["data"][0] == address
 
 
+def test_free_bytes_query_agrees_with_device_info(hip):
+    # The per-run sizing query skips the device-properties read; same free memory.
+    info = hip.device_info(0)
+    free = hip.device_free_bytes(0)
+    assert 0 < free <= info["total_bytes"]
+    assert abs(free - info["free_bytes"]) < (1 << 30)
+
+
 def test_source_arrays_set_again_change_nothing_and_new_ones_are_used():
     # The library skips re-uploading a spectrum equal to the one on the device: one
     # engine running spectra A, B, A must match fresh engines bit for bit.
