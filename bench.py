@@ -582,6 +582,15 @@ def main() -> int:
     }
     engine.close()
     print(json.dumps(line), flush=True)
+    # Release the large page-locked results and every native handle here, while the
+    # HIP runtime is certainly alive, rather than during interpreter teardown.
+    del plan_dij, engine
+    import gc
+
+    from pyradmc_amd import _native
+
+    gc.collect()
+    _native.shutdown()
     return 0
 
 

@@ -13,8 +13,10 @@ to torch's runtime and the two share devices, streams and memory.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import threading
+import weakref
 from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_uint8, c_uint64, c_void_p
 from pathlib import Path
 
@@ -180,7 +182,40 @@ def lib():
         fn.restype = restype
         fn.argtypes = argtypes
     _lib = h
+    # Registered after torch's import: exit handlers run last-in first-out, so this one
+    # runs before torch's and before the HIP runtime's own teardown.
+    atexit.register(shutdown)
     return h
+
+
+# Native handles still open (contexts, device-resident matrices): anything with a
+# close() that releases device memory, streams or graphs.
+_open_handles: "weakref.WeakSet" = weakref.WeakSet()
+_exiting = False
+
+
+def register_handle(obj) -> None:
+    """Have ``shutdown`` close ``obj`` (it must have ``close()``) if it is still open
+    at process exit."""
+    _open_handles.add(obj)
+
+
+def shutdown() -> None:
+    """Release every native resource in a fixed order before the process exits: open
+    contexts and matrices first (their streams, graphs, device and page-locked
+    buffers), then the cached page-locked result blocks. Blocks that arrays still view
+    are freed as soon as those arrays go, instead of returning to the cache. Runs at
+    exit by itself. Called earlier, the library stays usable (new handles and blocks
+    are made as before), but released blocks are freed from then on, not cached."""
+    global _exiting
+    _exiting = True
+    for obj in list(_open_handles):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _open_handles.clear()
+    release_pinned_cache()
 
 
 def exported_symbols() -> list[str]:
@@ -230,6 +265,7 @@ class Context:
             )
         self.handle = handle
         self.device = device
+        register_handle(self)
 
     def close(self) -> None:
         if self.handle:
@@ -288,7 +324,7 @@ def _pinned_give(size: int, address: int) -> None:
     if _lib is None:
         return
     with _pinned_lock:
-        if _pinned_cached_bytes + size <= PINNED_CACHE_MAX_BYTES:
+        if not _exiting and _pinned_cached_bytes + size <= PINNED_CACHE_MAX_BYTES:
             _pinned_cache.append((size, address))
             _pinned_cached_bytes += size
             return

@@ -141,6 +141,7 @@ class HipEngine:
     _info: dict | None = field(default=None, init=False, repr=False)
     _scoring_default: object = field(default=None, init=False, repr=False)
     _scoring_last: object = field(default=None, init=False, repr=False)
+    _map_bytes: int = field(default=0, init=False, repr=False)  # largest maps + sums set so far
     launch: dict = field(default_factory=dict, init=False, repr=False)  # set_launch's knobs
 
     def __post_init__(self) -> None:
@@ -256,11 +257,19 @@ class HipEngine:
             )
         self._uploaded_tables = key
 
+    def _free_for_maps(self) -> int:
+        """Device memory the tally maps and sums may take: what is free now (other
+        engines, torch and this engine's queues may hold memory taken after
+        construction) plus what this engine's current maps and sums hold, which a new
+        layout reuses or releases."""
+        free = nat.device_free_bytes(self._index)
+        return free + self._map_bytes if free > 0 else 0
+
     def _batch_slots(self, n_batches: int, n_columns: int, n_vox: int) -> int:
         """Batches transported together: each holds its own int64 map of the group's
         columns, folded in batch order afterwards (so any slot count gives the same
-        bits). As many as a quarter of free HBM allows."""
-        free = int(self._info.get("free_bytes", 0))
+        bits). As many as a quarter of the memory available for maps allows."""
+        free = self._free_for_maps()
         per_slot = 8 * n_columns * n_vox
         fit = int(free * _SLOT_MEMORY_FRACTION) // per_slot if free > 0 else 1
         return max(1, min(n_batches, fit))
@@ -287,6 +296,8 @@ class HipEngine:
             nat.ptr(mass, nat.c_double), int(n_columns), int(n_slots),
         )
         self._scoring_last = (scoring, key)
+        # int64 maps of every slot plus the float64 sums s1 and s2
+        self._map_bytes = max(self._map_bytes, 8 * (n_slots + 2) * n_columns * int(scoring.n_voxels))
 
     def _set_physics(self, pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w) -> None:
         p = nat.Physics()
@@ -680,7 +691,7 @@ class HipEngine:
         n_vox = scoring.n_voxels
         if beamlet_group_size is None:
             # float64 sums plus one int64 map per group column; batch slots come after.
-            free = int(self._info.get("free_bytes", 0))
+            free = self._free_for_maps()
             fit = int(free * _GROUP_MEMORY_FRACTION) // (24 * n_vox) if free > 0 else 128
             beamlet_group_size = max(1, min(fit, n_beamlets, 4096))
 

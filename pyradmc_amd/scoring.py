@@ -197,10 +197,9 @@ class DijResult:
     def variance_csc(self, unit: str = "mev_per_g"):
         if self.correlated:
             warnings.warn(
-                "this Dij was sampled with correlated columns: per-entry variances "
-                "are valid within a column, but combining them across columns "
-                "(e.g. a quadrature plan-dose variance) is invalid because the "
-                "columns are statistically dependent",
+                "correlated sampling: the beamlet columns share random streams, so each "
+                "variance describes its own entry only; sums of variances over several "
+                "columns (a plan dose's variance, for one) leave out their covariance",
                 UserWarning,
                 stacklevel=2,
             )

@@ -90,6 +90,7 @@ class DeviceDij:
         if rc != 0:
             _raise(rc, f"hsp_create failed with status {rc} (invalid CSC or device error)")
         self._h = handle
+        _native.register_handle(self)
         self.last_kernel_ms: float | None = None
 
     def _check(self, rc: int, what: str) -> None:

@@ -13,6 +13,11 @@ Usage::
     HOME=<scratch> python tests/golden/make_golden.py tables gs_nodes
     HOME=<scratch> python tests/golden/make_golden.py run_c1_kerma run_c1_cm ...
     HOME=<scratch> python tests/golden/make_golden.py dose_for_weights
+    HOME=<scratch> python tests/golden/make_golden.py hi slab <part> <histories> <shard dir>
+    HOME=<scratch> python tests/golden/make_golden.py merge_hi slab <shard dir> <part> ...
+
+The ``hi`` shards (one process each, so several run side by side) are written to the
+shard directory given on the command line; ``merge_hi`` pools them into one fixture.
 
 HOME is redirected by the caller because the reference persists its GS grid under
 ``~/.cache``.
@@ -322,7 +327,7 @@ def _slab_grid():
                      material=np.zeros(shape, dtype=np.int32))
 
 
-def _run_hi(config: str, part: int, n: int) -> None:
+def _run_hi(config: str, part: int, n: int, shard_dir: Path) -> None:
     """One shard of a high-statistics reference run (distinct seed per shard)."""
     from pyradmc.data.analytic import AnalyticCrossSections
     from pyradmc.geometry.grid import VoxelGrid
@@ -349,7 +354,7 @@ def _run_hi(config: str, part: int, n: int) -> None:
     r = engine.run(source, n, 10, seed=1000 + part, scoring_grid=scoring)
     wall = time.perf_counter() - started
     np.savez_compressed(
-        OUT.parent.parent / "_work" / f"hi_{config}_{part}.npz",
+        shard_dir / f"hi_{config}_{part}.npz",
         dose=r.dose, dose_sigma=r.dose_sigma, energy_emitted=r.energy_emitted,
         energy_deposited=r.energy_deposited, energy_escaped=r.energy_escaped,
         energy_unscored=r.energy_unscored, n=n, wall=wall,
@@ -357,11 +362,10 @@ def _run_hi(config: str, part: int, n: int) -> None:
     print(f"hi {config} part {part}: {n} in {wall:.1f} s ({n / wall:.1f} hist/s)")
 
 
-def _merge_hi(config: str, parts: list[int]) -> None:
+def _merge_hi(config: str, shard_dir: Path, parts: list[int]) -> None:
     """Pool equal-size shards: mean of shard means; standard error from the shard
     standard errors (each from 10 batches), i.e. 10 * shards batches in total."""
-    work = OUT.parent.parent / "_work"
-    runs = [np.load(work / f"hi_{config}_{p}.npz") for p in parts]
+    runs = [np.load(shard_dir / f"hi_{config}_{p}.npz") for p in parts]
     if len({int(r["n"]) for r in runs}) != 1:
         raise ValueError("shards must hold equal history counts")
     k = len(runs)
@@ -390,9 +394,11 @@ TASKS = {
 
 if __name__ == "__main__":
     if sys.argv[1] == "hi":
-        _run_hi(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
+        shards = Path(sys.argv[5])
+        shards.mkdir(parents=True, exist_ok=True)
+        _run_hi(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), shards)
     elif sys.argv[1] == "merge_hi":
-        _merge_hi(sys.argv[2], [int(v) for v in sys.argv[3:]])
+        _merge_hi(sys.argv[2], Path(sys.argv[3]), [int(v) for v in sys.argv[4:]])
     else:
         for task in sys.argv[1:]:
             TASKS[task]()

@@ -83,3 +83,54 @@ def test_release_frees_every_cached_block(fake):
     assert cached > 0
     assert nat.release_pinned_cache() == cached
     assert fake.frees == 4 and not fake.live and nat._pinned_cached_bytes == 0
+
+
+class _Handle:
+    def __init__(self, log):
+        self.log = log
+
+    def close(self):
+        self.log.append(self)
+
+
+def test_shutdown_closes_handles_then_frees_every_block(fake, monkeypatch):
+    monkeypatch.setattr(nat, "_exiting", False)
+    monkeypatch.setattr(nat, "_open_handles", nat.weakref.WeakSet())
+    closed = []
+    handles = [_Handle(closed), _Handle(closed)]
+    for h in handles:
+        nat.register_handle(h)
+    cached = nat.pinned_empty(1 << 16, np.float64)
+    held = nat.pinned_empty(1 << 17, np.float64)  # still viewed when the process exits
+    del cached
+    assert fake.frees == 0 and nat._pinned_cached_bytes > 0
+    nat.shutdown()
+    assert sorted(map(id, closed)) == sorted(map(id, handles))
+    assert fake.frees == 1 and nat._pinned_cached_bytes == 0
+    held[:] = 2.0  # the block a live array views stays valid
+    del held  # ... and is freed, not cached, once the array goes
+    assert fake.frees == 2 and not fake.live and nat._pinned_cached_bytes == 0
+    nat.shutdown()  # a second call finds nothing left
+    assert len(closed) == 2 and fake.frees == 2
+
+
+def test_library_load_registers_shutdown_at_exit():
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    # A child process loads the library, registers a handle and exits without closing
+    # it: the exit handler must close it (the library loads without a device).
+    code = (
+        "from pyradmc_amd import _native as nat\n"
+        "nat.lib()\n"
+        "class H:\n"
+        "    def close(self):\n"
+        "        print('closed at exit', flush=True)\n"
+        "h = H()\n"
+        "nat.register_handle(h)\n"
+    )
+    root = Path(__file__).resolve().parent.parent
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert "closed at exit" in out.stdout
