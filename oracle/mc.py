@@ -59,6 +59,7 @@ class OmConfig(Structure):
         ("n_beamlets", c_int), ("beamlet", _dp),
         ("pcut", c_double), ("ecut", c_double), ("step_fraction", c_double),
         ("resolution", c_double), ("transport_electrons", c_int), ("msc_gs", c_int),
+        ("srest", _dp), ("dose_to_water", c_int),
     ]
 
 
@@ -126,7 +127,12 @@ class Oracle:
     def __init__(self, shape, spacing, origin, density, material, geometry_densities,
                  source: dict, *, pcut=0.05, ecut=0.2, transport_electrons=True, msc_model="gs",
                  step_energy_fraction=None, deposit_resolution_cm=None, scoring=None,
-                 gs=None) -> None:
+                 gs=None, tables=None, scoring_mode="dose_to_medium") -> None:
+        """``tables``: the oracle's own flattened tables (a dict as
+        :meth:`oracle.xs_ref.WaterOracle.tables` returns, e.g. from
+        :mod:`oracle.tab_ref`); analytic water when omitted. ``scoring_mode``
+        ``"dose_to_water"`` weights each deposit by the water/medium restricted
+        stopping-power ratio."""
         self._keep = []
         c = OmConfig()
         c.nx, c.ny, c.nz = (int(v) for v in shape)
@@ -145,22 +151,31 @@ class Oracle:
         self.mass = self._hold(np.ascontiguousarray(scoring["mass"], dtype=np.float64).reshape(-1))
         c.mass = _p(self.mass)
         e_max = float(source.get("max_energy", source.get("energy", 0.0))) * (1.0 + 1.0e-6)
-        water = xs_ref.WaterOracle(geometry_densities)
-        t = water.tables(ecut, pcut, e_max)
+        if tables is None:
+            t = xs_ref.WaterOracle(geometry_densities).tables(ecut, pcut, e_max)
+        else:
+            t = tables
+            if abs(t["e_max"] - e_max) > 1e-12 * e_max or t["ecut"] != ecut or t["pcut"] != pcut:
+                raise ValueError("tables were flattened for other cuts or another e_max")
         self.tables = t
-        c.n, c.n_mat = int(t["n_points"]), 1
+        n_mat = int(np.asarray(t["mu_compton"]).shape[0])
+        if int(np.max(material)) >= n_mat or int(np.min(material)) < 0:
+            raise ValueError("a voxel material has no table row")
+        c.n, c.n_mat = int(t["n_points"]), n_mat
         for name, key in (("majorant", "majorant"), ("mu_c", "mu_compton"), ("mu_p", "mu_photo"),
                           ("mu_pp", "mu_pair"), ("mu_r", "mu_rayleigh"), ("csda", "csda_range"),
                           ("moller", "moller"), ("rrange", "restricted_range"),
                           ("srad", "stopping_radiative"), ("tpow", "scattering_power"),
-                          ("log_eta", "log_eta"), ("inv_range", "energy_of_restricted_range")):
+                          ("log_eta", "log_eta"), ("inv_range", "energy_of_restricted_range"),
+                          ("srest", "stopping_restricted")):
             setattr(c, name, _p(self._hold(np.ascontiguousarray(t[key], dtype=np.float64).reshape(-1))))
         c.p_log_min, c.p_inv = t["photon_log_e_min"], t["photon_inv_dlog"]
         c.e_log_min, c.e_inv = t["electron_log_e_min"], t["electron_inv_dlog"]
         c.r_log_min, c.r_inv = t["range_log_r_min"], t["range_inv_dlog"]
         c.msc_gs = 1 if msc_model == "gs" else 0
         if c.msc_gs:
-            values, ix0, iy0 = gs if gs is not None else gs_grid(t)
+            present = sorted(set(np.unique(np.asarray(material)).astype(int).tolist()))
+            values, ix0, iy0 = gs if gs is not None else gs_grid(t, materials=present)
             values = self._hold(np.ascontiguousarray(values, dtype=np.float64))
             c.gs = _p(values)
             c.gs_ix0, c.gs_iy0 = int(ix0), int(iy0)
@@ -190,6 +205,9 @@ class Oracle:
                                 else (0.20 if msc_model == "gs" else 0.05))
         c.resolution = 0.0 if deposit_resolution_cm is None else float(deposit_resolution_cm)
         c.transport_electrons = 1 if transport_electrons else 0
+        if scoring_mode not in ("dose_to_medium", "dose_to_water"):
+            raise ValueError(f"unknown scoring_mode {scoring_mode!r}")
+        c.dose_to_water = 1 if scoring_mode == "dose_to_water" else 0
         self.cfg = c
         self.n_vox = int(np.prod(scoring["shape"]))
 

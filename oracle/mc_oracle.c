@@ -59,6 +59,9 @@ typedef struct {
   /* physics */
   double pcut, ecut, step_fraction, resolution;
   int transport_electrons, msc_gs;
+  /* dose to water: restricted stopping rows and the switch (scoring/dose_to_water.py) */
+  const double *srest;
+  int dose_to_water;
 } om_config;
 
 typedef struct {
@@ -214,23 +217,39 @@ typedef struct {
   om_books b;
 } tally_t;
 
-static void deposit(const om_config *c, tally_t *t, double x, double y, double z, double amount) {
+/* Dose-to-water weight of a deposit: the restricted collision stopping-power ratio
+ * water / medium at the cutoff-clamped energy, exactly 1 for water or dose to medium
+ * (water_spr, pyradmc/scoring/dose_to_water.py:67-85; table form kernels.py:462-488). */
+static double spr(const om_config *c, double e, int m) {
+  if (!c->dose_to_water || m == 0) return 1.0;
+  double ec = e > c->ecut ? e : c->ecut;
+  return elook(c, c->srest, 0, ec) / elook(c, c->srest, m, ec);
+}
+static int material_at(const om_config *c, const double p[3]) {
+  return c->material[voxel(c, p[0], p[1], p[2])];
+}
+
+/* The tally takes factor * amount; the books take the physical amount. */
+static void deposit(const om_config *c, tally_t *t, double x, double y, double z, double amount,
+                    double factor) {
   double hx = c->sox + c->snx * c->ssx, hy = c->soy + c->sny * c->ssy, hz = c->soz + c->snz * c->ssz;
   if (x < c->sox || x >= hx || y < c->soy || y >= hy || z < c->soz || z >= hz) {
     t->b.unscored += amount;
     return;
   }
   int ix = axis(x, c->sox, c->ssx, c->snx), iy = axis(y, c->soy, c->ssy, c->sny), iz = axis(z, c->soz, c->ssz, c->snz);
-  t->energy[t->base + ((size_t)ix * c->sny + iy) * c->snz + iz] += amount;
+  t->energy[t->base + ((size_t)ix * c->sny + iy) * c->snz + iz] += factor * amount;
   t->b.deposited += amount;
   t->b.deposits++;
 }
-static double deposit_or_escape(const om_config *c, tally_t *t, const double p[3], double amount) {
+static double deposit_or_escape(const om_config *c, tally_t *t, const double p[3], double amount,
+                                double factor) {
   if (amount <= 0.0) return 0.0;
-  if (inside(c, p[0], p[1], p[2])) { deposit(c, t, p[0], p[1], p[2], amount); return 0.0; }
+  if (inside(c, p[0], p[1], p[2])) { deposit(c, t, p[0], p[1], p[2], amount, factor); return 0.0; }
   return amount;
 }
-static double deposit_spread(const om_config *c, tally_t *t, const double p[3], const double u[3], double len, double amount) {
+static double deposit_spread(const om_config *c, tally_t *t, const double p[3], const double u[3], double len,
+                             double amount, double factor) {
   int n = 1;
   if (c->resolution > 0.0) {
     n = (int)ceil(len / c->resolution);
@@ -240,7 +259,7 @@ static double deposit_spread(const om_config *c, tally_t *t, const double p[3], 
   double share = amount / (double)n, piece = len / (double)n, out = 0.0;
   for (int k = 0; k < n; ++k) {
     double s = ((double)k + 0.5) * piece, q[3] = {p[0] + u[0] * s, p[1] + u[1] * s, p[2] + u[2] * s};
-    out += deposit_or_escape(c, t, q, share);
+    out += deposit_or_escape(c, t, q, share, factor);
   }
   return out;
 }
@@ -260,8 +279,9 @@ static void push(stack_t *s, int kind, double e, double w, const double p[3], co
   t->b.pushes++;
 }
 
-static void annihilate(const om_config *c, tally_t *t, stack_t *s, const double p[3], double w, rng_t *r) {
-  if (c->pcut >= ME) { deposit(c, t, p[0], p[1], p[2], w * 2.0 * ME); return; }
+static void annihilate(const om_config *c, tally_t *t, stack_t *s, const double p[3], double w, double factor,
+                       rng_t *r) {
+  if (c->pcut >= ME) { deposit(c, t, p[0], p[1], p[2], w * 2.0 * ME, factor); return; }
   double ct = 1.0 - 2.0 * uniform(r), st = sqrt(fmax(0.0, 1.0 - ct * ct)), phi = TWO_PI * uniform(r);
   double a[3] = {st * cos(phi), st * sin(phi), ct}, b[3] = {-a[0], -a[1], -a[2]};
   push(s, 0, ME, w, p, a, t);
@@ -307,12 +327,12 @@ static double photon(const om_config *c, tally_t *t, stack_t *s, particle_t q, r
         rotate(d, compton_e_cos(e, ratio), phi + PI);
         push(s, 1, recoil, w, p, d, t);
       } else {
-        deposit(c, t, p[0], p[1], p[2], w * recoil);
+        deposit(c, t, p[0], p[1], p[2], w * recoil, spr(c, recoil, m));
       }
       double g[3] = {u[0], u[1], u[2]};
       rotate(g, compton_cos(e, ratio), phi);
       double es = e * ratio;
-      if (es <= c->pcut) { deposit(c, t, p[0], p[1], p[2], w * es); return esc; }
+      if (es <= c->pcut) { deposit(c, t, p[0], p[1], p[2], w * es, spr(c, es, m)); return esc; }
       double cw = w;
       if (es < 0.5 && cw < 4.0) {
         double nw = uniform(r) < 0.5 ? cw / 0.5 : 0.0;
@@ -324,19 +344,19 @@ static double photon(const om_config *c, tally_t *t, stack_t *s, particle_t q, r
       return esc;
     } else if (th < mc + mp) { /* photoelectric */
       if (c->transport_electrons && e > c->ecut) push(s, 1, e, w, p, u, t);
-      else deposit(c, t, p[0], p[1], p[2], w * e);
+      else deposit(c, t, p[0], p[1], p[2], w * e, spr(c, e, m));
       return esc;
     } else if (th < mc + mp + mpp) { /* pair */
       double kin = e - 2.0 * ME;
       if (c->transport_electrons) {
         double f = uniform(r), se = f * kin, sp = (1.0 - f) * kin;
         if (se > c->ecut) push(s, 1, se, w, p, u, t);
-        else deposit(c, t, p[0], p[1], p[2], w * se);
+        else deposit(c, t, p[0], p[1], p[2], w * se, spr(c, se, m));
         if (sp > c->ecut) push(s, 2, sp, w, p, u, t);
-        else { deposit(c, t, p[0], p[1], p[2], w * sp); annihilate(c, t, s, p, w, r); }
+        else { deposit(c, t, p[0], p[1], p[2], w * sp, spr(c, sp, m)); annihilate(c, t, s, p, w, spr(c, ME, m), r); }
       } else {
-        deposit(c, t, p[0], p[1], p[2], w * kin);
-        annihilate(c, t, s, p, w, r);
+        deposit(c, t, p[0], p[1], p[2], w * kin, spr(c, kin, m));
+        annihilate(c, t, s, p, w, spr(c, ME, m), r);
       }
       return esc;
     } else { /* Rayleigh: unreachable with zero coherent data; Thomson angle */
@@ -355,8 +375,9 @@ static double electron(const om_config *c, tally_t *t, stack_t *s, particle_t q,
   if (vacuum_entry(c, p, u) < 0.0) return w * (e + latent);
   for (;;) {
     if (e <= c->ecut) {
-      deposit(c, t, p[0], p[1], p[2], w * e);
-      if (pos) annihilate(c, t, s, p, w, r);
+      int tm = material_at(c, p);
+      deposit(c, t, p[0], p[1], p[2], w * e, spr(c, e, tm));
+      if (pos) annihilate(c, t, s, p, w, spr(c, ME, tm), r);
       return esc;
     }
     t->b.electron_steps++;
@@ -394,7 +415,9 @@ static double electron(const om_config *c, tally_t *t, stack_t *s, particle_t q,
     double cont = de + local;
     int emit = uniform(r) < prob;
     double s1 = uniform(r) * sl, s2 = sl - s1, d1 = sl > 0.0 ? cont * (s1 / sl) : 0.0, d2 = cont - d1;
-    esc += deposit_spread(c, t, p, u, s1, w * d1);
+    /* one weight per substep, at the (E, material) that set its stopping power */
+    double sf = spr(c, e, m);
+    esc += deposit_spread(c, t, p, u, s1, w * d1, sf);
     e -= d1;
     for (int a = 0; a < 3; ++a) p[a] += u[a] * s1;
     if (!inside(c, p[0], p[1], p[2])) return esc + w * (e + latent);
@@ -415,11 +438,11 @@ static double electron(const om_config *c, tally_t *t, stack_t *s, particle_t q,
         if (k < 0.5 && w < 4.0) { pw = uniform(r) < 0.5 ? w / 0.5 : 0.0; esc += (w - pw) * k; }
         if (pw > 0.0) push(s, 0, k, pw, p, u, t);
       } else {
-        esc += deposit_or_escape(c, t, p, w * k);
+        esc += deposit_or_escape(c, t, p, w * k, spr(c, k, m));
       }
       e -= k;
     }
-    esc += deposit_spread(c, t, p, u, s2, w * d2);
+    esc += deposit_spread(c, t, p, u, s2, w * d2, sf);
     e -= d2;
     for (int a = 0; a < 3; ++a) p[a] += u[a] * s2;
     if (!inside(c, p[0], p[1], p[2])) return esc + w * (e + latent);

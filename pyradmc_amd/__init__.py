@@ -16,7 +16,8 @@ from pyradmc_amd.constants import (
     VERSION as __version__,
 )
 from pyradmc_amd.data.analytic import AnalyticCrossSections
-from pyradmc_amd.data.materials import WATER
+from pyradmc_amd.data.materials import MATERIALS, WATER
+from pyradmc_amd.data.tabulated import TabulatedCrossSections, TabulatedData
 from pyradmc_amd.geometry import (
     BeamletGridSource,
     ParallelBeamSource,
@@ -56,6 +57,7 @@ __all__ = [
     "ELECTRON_MASS_MEV",
     "GY_PER_MEV_PER_G",
     "HipEngine",
+    "MATERIALS",
     "PCUT_MEV",
     "ParallelBeamSource",
     "PencilBeamSource",
@@ -66,6 +68,8 @@ __all__ = [
     "SpectralBeamletSource",
     "SpectralParallelBeamSource",
     "Spectrum",
+    "TabulatedCrossSections",
+    "TabulatedData",
     "TransportResult",
     "VoxelGrid",
     "WATER",

@@ -379,11 +379,13 @@ class AnalyticCrossSections:
         return result
 
     def restricted_range_bulk(self, energy: np.ndarray, material: int, delta_cut: float) -> np.ndarray:
-        log_e, cumulative = self._restricted_grid(delta_cut)
+        log_e, cumulative = self.restricted_grid(material, delta_cut)
         e = np.clip(np.asarray(energy, dtype=np.float64), delta_cut, RANGE_GRID_E_MAX)
         return np.interp(np.log(e), log_e, cumulative)
 
-    def _restricted_grid(self, delta_cut: float) -> tuple[np.ndarray, np.ndarray]:
+    def restricted_grid(self, material: int, delta_cut: float) -> tuple[np.ndarray, np.ndarray]:
+        """(log E nodes, restricted range) from the cut to 64 MeV (water)."""
+        self._check_material(material)
         grid = self._restricted_grids.get(delta_cut)
         if grid is None:
             grid = restricted_range_grid(WATER_DATA, delta_cut)
@@ -439,7 +441,7 @@ class AnalyticCrossSections:
     def energy_after_mass_path(
         self, energy: float, material: int, delta_cut: float, mass_path: float
     ) -> float:
-        log_e, cumulative = self._restricted_grid(delta_cut)
+        log_e, cumulative = self.restricted_grid(material, delta_cut)
         remaining = self.restricted_range(energy, material, delta_cut) - mass_path
         if remaining <= 0.0:
             return delta_cut

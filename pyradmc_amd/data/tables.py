@@ -6,9 +6,10 @@ Same grids, node counts and field names as the reference's ``CrossSectionTables`
 ``[ECUT/2, e_max]``), a majorant with ``1 + 1e-6`` headroom, the restricted range on
 the electron grid and its inverse on one shared log-range grid, and the flat
 (Thomson) coherent cumulative. Sources that expose bulk array evaluation (this
-package's :class:`~pyradmc_amd.data.analytic.AnalyticCrossSections`) flatten in one
-vectorized pass; any other object with the scalar query API is flattened point by
-point.
+package's :class:`~pyradmc_amd.data.analytic.AnalyticCrossSections` and
+:class:`~pyradmc_amd.data.tabulated.TabulatedCrossSections`) flatten in one
+vectorized pass per material; any other object with the scalar query API is flattened
+point by point.
 """
 
 from __future__ import annotations
@@ -136,7 +137,7 @@ def build_cross_section_tables(
         if bulk:
             start = float(source.restricted_range_bulk(np.array([e_top]), m, ecut)[0])
             remaining = start - mass_path
-            log_e, cumulative = source._restricted_grid(ecut)
+            log_e, cumulative = source.restricted_grid(m, ecut)
             e_end = np.exp(np.interp(remaining, cumulative, log_e))
             e_end = np.minimum(np.maximum(e_end, ecut), e_top)
             energy_of_rr[m] = np.where(remaining <= 0.0, ecut, e_end)

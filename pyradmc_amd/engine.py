@@ -154,6 +154,15 @@ class HipEngine:
                 "the HIP engine addresses at most 256 materials (uint8 voxel map); this "
                 f"source declares {self.cross_sections.n_materials}"
             )
+        # Every voxel's material must have a table row: the kernels index the rows by the
+        # material map without a bounds check.
+        n_rows = int(self.cross_sections.n_materials)
+        top = int(np.max(np.asarray(self.grid.material))) if np.asarray(self.grid.material).size else 0
+        if top >= n_rows:
+            raise ValueError(
+                f"voxel material index {top} has no row in the cross-section source "
+                f"({n_rows} material{'s' if n_rows != 1 else ''})"
+            )
         self._index = _device_index(self.device)
         self._ctx = nat.Context(self._index)
         self._info = nat.device_info(self._index)

@@ -392,6 +392,53 @@ TASKS = {
 }
 
 
+def _two_material() -> None:
+    """The reference's synthetic two-material slab (water, and 'air' at half the water
+    values) under a 2 MeV parallel beam: KERMA, and electron transport scored as dose to
+    medium and as dose to water. The inputs are tests/problems.py's two_material_*."""
+    from pyradmc.data.tabulated.model import TabulatedData
+    from pyradmc.data.tabulated.source import TabulatedCrossSections
+    from pyradmc.geometry.grid import VoxelGrid
+    from pyradmc.geometry.source import ParallelBeamSource
+
+    sys.path.insert(0, str(OUT.parent.parent))
+    from tests import problems as P
+
+    ReferenceEngine, HostRNG = _engine()
+    d = P.two_material_data()
+    data = TabulatedData(
+        photon_energies=d["photon_energies"], mu_over_rho=dict(d["mu_over_rho"]),
+        electron_energies=d["electron_energies"], restricted_stopping=d["restricted_stopping"],
+        radiative_stopping=d["radiative_stopping"], moller=d["moller"], csda_range=d["csda_range"],
+        scattering_power=d["scattering_power"], delta_cut=d["delta_cut"], materials=d["materials"],
+        provenance=d["provenance"])
+    material = np.zeros(P.TWO_MATERIAL_SHAPE, dtype=np.int32)
+    material[:, :, P.TWO_MATERIAL_SLAB] = 1
+    grid = VoxelGrid(shape=P.TWO_MATERIAL_SHAPE, spacing=P.TWO_MATERIAL_SPACING,
+                     density=np.ones(P.TWO_MATERIAL_SHAPE), material=material)
+    xs = TabulatedCrossSections(data, geometry_densities=grid.max_density_by_material())
+    source = ParallelBeamSource(energy=2.0, z=-1.0, x_range=(0.0, 16.0), y_range=(0.0, 16.0))
+    out = {}
+    for tag, electrons, mode, n, batches, seed in (
+        ("kerma", False, "dose_to_medium", 120_000, 12, 20261201),
+        ("medium", True, "dose_to_medium", 60_000, 12, 20261202),
+        ("water", True, "dose_to_water", 60_000, 12, 20261203),
+    ):
+        engine = ReferenceEngine(grid, xs, HostRNG())
+        started = time.perf_counter()
+        r = engine.run(source, n, batches, seed=seed, transport_electrons=electrons, scoring_mode=mode)
+        wall = time.perf_counter() - started
+        out.update({f"{tag}_dose": r.dose, f"{tag}_sigma": r.dose_sigma,
+                    f"{tag}_books": np.array([r.energy_emitted, r.energy_deposited, r.energy_escaped,
+                                              r.energy_unscored]),
+                    f"{tag}_run": np.array([n, batches, seed]), f"{tag}_seconds": wall})
+        print(f"two_material {tag}: {n} histories in {wall:.1f} s", flush=True)
+    np.savez_compressed(OUT / "ref_two_material_slab.npz", **out)
+
+
+TASKS["two_material"] = _two_material
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "hi":
         shards = Path(sys.argv[5])

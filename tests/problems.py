@@ -35,3 +35,61 @@ def csc_dense(indptr, indices, values, n_voxels: int) -> np.ndarray:
     for c in range(len(indptr) - 1):
         out[c, indices[indptr[c]:indptr[c + 1]]] = values[indptr[c]:indptr[c + 1]]
     return out
+
+
+# ------------------------------------------------------------------ two materials
+#
+# The reference's synthetic two-material instrument (its tests/unit/
+# test_tabulated_source.py:29-78): a water row with known analytic forms on 32-node
+# geometric grids, and an 'air' row at half the water values (twice the CSDA range).
+# A test instrument for the material-row plumbing, not physics.
+
+TWO_MATERIAL_DELTA_CUT = 0.2
+TWO_MATERIAL_SHAPE = (8, 8, 16)
+TWO_MATERIAL_SPACING = (2.0, 2.0, 1.0)
+TWO_MATERIAL_SLAB = slice(5, 9)  # z voxels of material 1
+
+
+def two_material_data(n_p: int = 32, n_e: int = 32) -> dict:
+    """Fields of the compiled container (photon process -> [2][n_p] rows)."""
+    pe = np.geomspace(0.01, 10.0, n_p)
+    ee = np.geomspace(0.01, 10.0, n_e)
+    water_mu = {0: 0.15 * np.ones_like(pe), 1: 1.0e-3 / pe**3,
+                2: np.maximum(pe - 1.022, 0.0) * 1.0e-3, 3: 0.01 / pe}
+    water_e = {"restricted_stopping": 2.0 / ee, "radiative_stopping": 0.01 * ee, "moller": 0.1 / ee,
+               "csda_range": 0.5 * ee, "scattering_power": 1.0 / ee}
+    scale = {"csda_range": 2.0}
+    data = {
+        "photon_energies": pe,
+        "mu_over_rho": {p: np.stack([t, 0.5 * t]) for p, t in water_mu.items()},
+        "electron_energies": ee,
+        "delta_cut": TWO_MATERIAL_DELTA_CUT,
+        "materials": ("water", "air"),
+        "provenance": "synthetic two-material",
+    }
+    for name, t in water_e.items():
+        data[name] = np.stack([t, scale.get(name, 0.5) * t])
+    return data
+
+
+def two_material_slab():
+    """(grid, cross sections) of the slab: 8x8x16 voxels of 2x2x1 cm, unit density
+    everywhere, material 1 at z voxels 5-8 (the reference's
+    tests/integration/test_multimaterial_transport.py:27-48)."""
+    from pyradmc_amd import VoxelGrid
+    from pyradmc_amd.data.tabulated import TabulatedCrossSections, TabulatedData
+
+    material = np.zeros(TWO_MATERIAL_SHAPE, dtype=np.int32)
+    material[:, :, TWO_MATERIAL_SLAB] = 1
+    grid = VoxelGrid(shape=TWO_MATERIAL_SHAPE, spacing=TWO_MATERIAL_SPACING,
+                     density=np.ones(TWO_MATERIAL_SHAPE), material=material)
+    xs = TabulatedCrossSections(TabulatedData(**two_material_data()),
+                                geometry_densities=grid.max_density_by_material())
+    return grid, xs
+
+
+def two_material_beam(energy: float = 2.0):
+    """The slab's 16 x 16 cm parallel beam along +z from z = -1 cm."""
+    from pyradmc_amd import ParallelBeamSource
+
+    return ParallelBeamSource(energy=energy, z=-1.0, x_range=(0.0, 16.0), y_range=(0.0, 16.0))
