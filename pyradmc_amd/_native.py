@@ -286,7 +286,8 @@ class Context:
 # Released page-locked blocks kept for reuse. Pinning and unpinning memory costs tens of
 # microseconds per MiB and the unpin waits for the device, so a loop of runs that each
 # return a large result would otherwise pay both per run. At most this many bytes stay
-# cached; ``release_pinned_cache`` returns them to the system.
+# cached (the oldest blocks go first); ``release_pinned_cache`` returns them to the
+# system.
 PINNED_CACHE_MAX_BYTES = 256 << 20
 _PAGE = 4096
 _pinned_lock = threading.RLock()  # reentrant: a block's __del__ may run inside a take
@@ -300,7 +301,7 @@ def _pinned_take(nbytes: int) -> tuple[int, int]:
     global _pinned_cached_bytes
     with _pinned_lock:
         best = -1
-        for k in range(len(_pinned_cache)):
+        for k in reversed(range(len(_pinned_cache))):  # newest first among equal sizes
             size = _pinned_cache[k][0]
             if nbytes <= size <= 2 * nbytes and (best < 0 or size < _pinned_cache[best][0]):
                 best = k
@@ -320,15 +321,24 @@ def _pinned_take(nbytes: int) -> tuple[int, int]:
 
 
 def _pinned_give(size: int, address: int) -> None:
+    """Cache a released block; when it would pass the byte cap, the oldest cached blocks
+    are freed first (the sizes a loop of runs asks for next are its most recent ones).
+    A block larger than the cap itself, or released while the process exits, is freed."""
     global _pinned_cached_bytes
     if _lib is None:
         return
+    evicted = [address]
     with _pinned_lock:
-        if not _exiting and _pinned_cached_bytes + size <= PINNED_CACHE_MAX_BYTES:
+        if not _exiting and size <= PINNED_CACHE_MAX_BYTES:
+            evicted = []
+            while _pinned_cached_bytes + size > PINNED_CACHE_MAX_BYTES:
+                old_size, old_address = _pinned_cache.pop(0)
+                _pinned_cached_bytes -= old_size
+                evicted.append(old_address)
             _pinned_cache.append((size, address))
             _pinned_cached_bytes += size
-            return
-    _lib.hmc_host_free(address)
+    for a in evicted:
+        _lib.hmc_host_free(a)
 
 
 def release_pinned_cache() -> int:

@@ -665,6 +665,7 @@ def test_argument_validation():
 def test_pinned_host_arrays_behave_as_numpy_arrays(hip):
     # Page-locked destinations of fetched CSC entries: ordinary arrays whose memory
     # lives as long as any view of it.
+    hip.release_pinned_cache()  # start from an empty cache, whatever ran before
     a = hip.pinned_empty(3 << 20, np.float32)
     assert a.dtype == np.float32 and a.shape == (3 << 20,) and a.flags.writeable
     a[:] = np.arange(a.size, dtype=np.float32)

@@ -71,9 +71,16 @@ def test_size_window_and_cap(fake, monkeypatch):
     del small, half
     cached = nat._pinned_cached_bytes
     monkeypatch.setattr(nat, "PINNED_CACHE_MAX_BYTES", cached + (1 << 20))
-    big = nat.pinned_empty(3 << 20, np.uint8)
-    del big  # would pass the cap: freed, not cached
+    huge = nat.pinned_empty((cached >> 20) + (8 << 20), np.uint8)
+    del huge  # larger than the cap itself: freed, not cached
     assert fake.frees == 1 and nat._pinned_cached_bytes == cached
+    big = nat.pinned_empty(2 << 20, np.uint8)  # within the cap, not beside what is cached
+    big_address = _address(big)
+    del big  # would pass the cap: the oldest cached blocks go to make room
+    assert fake.frees >= 2 and nat._pinned_cached_bytes <= nat.PINNED_CACHE_MAX_BYTES
+    assert nat._pinned_cache[-1] == (2 << 20, big_address)
+    again = nat.pinned_empty(2 << 20, np.uint8)
+    assert _address(again) == big_address  # the newest block serves the next run
 
 
 def test_release_frees_every_cached_block(fake):
