@@ -23,6 +23,10 @@ extern "C" {
 #define HDN_OK 0
 #define HDN_EINVAL -1
 #define HDN_EHIP -2
+#define HDN_ESIZE -3 /* the call needs more than the library's fixed device array holds */
+
+/* Floats of the fixed per-GPU device array behind hdn_conv3d_wgrad_noalloc (64 MiB). */
+#define HDN_SCRATCH_FLOATS (16LL * 1024 * 1024)
 
 /* y = act(bias + conv(x, w)) with act(v) = v >= 0 ? v : slope v (slope 1: identity);
  * y_cm may be NULL; bias may be NULL. N*D*H*W % 64 == 0, cin % 16 == 0,
@@ -47,6 +51,19 @@ int hdn_conv3d_fwd_src(const void* x, const void* x2, int src, const void* w, co
  * layer input and output gradient; W % 8 == 0. Sum them with hdn_reduce_splits. */
 int hdn_conv3d_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d,
                      int h, int w_, int cin, int cout, void* stream);
+
+/* No counterpart in the reference (which has no learned component): the weight gradient
+ * for callers that cannot allocate device memory - code under HIP graph capture, training
+ * loops and harnesses that forbid hipMalloc. grad[27][cout][cin] (FP32) receives what
+ * hdn_conv3d_wgrad followed by hdn_reduce_splits would give with the same splits, bit
+ * for bit; the partials go to a fixed device array the library holds on each GPU until
+ * the process ends, so nothing is allocated. Limit: splits * 27 * cout * cin must not
+ * exceed HDN_SCRATCH_FLOATS, else HDN_ESIZE and nothing runs. `stream` must belong to
+ * the current device (else HDN_EINVAL). Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU (other streams, other host threads) race. The
+ * first call on a device looks up the array's address there. */
+int hdn_conv3d_wgrad_noalloc(const void* x_cm, const void* dz_cm, float* grad, int splits, int n,
+                             int d, int h, int w_, int cin, int cout, void* stream);
 
 /* out[i] = sum over s of part[s][i], in split order (deterministic). */
 int hdn_reduce_splits(const float* part, float* out, long long len, int splits, void* stream);

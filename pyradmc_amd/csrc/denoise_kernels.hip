@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/hipdn.h"
 
@@ -706,6 +707,65 @@ int conv_fwd(const __bf16* xb, const __bf16* x2b, const __bf16* wb, const float*
   return launch_status();
 }
 
+// Weight-gradient partials of hdn_conv3d_wgrad_noalloc live here: a library-owned array
+// each GPU holds from the code object's load until the process ends. In host code the
+// name stands for a host-side shadow, not a device address: kernels get the address
+// hipGetSymbolAddress returns for the current device.
+__device__ __align__(256) float scratch[HDN_SCRATCH_FLOATS];
+
+// Validates a weight-gradient call and returns the positions each split owns.
+int wgrad_split(const void* x_cm, const void* dz_cm, int splits, int n, int d, int h, int w, int cin,
+                int cout, long long* per_split_out) {
+  if (!x_cm || !dz_cm || splits < 1 || !shape_ok(n, d, h, w)) return HDN_EINVAL;
+  const long long P = static_cast<long long>(n) * d * h * w;
+  if (w % 8 != 0 || P % 32 != 0 || cin % 16 != 0 || cout % 16 != 0 || cin < 16 || cout < 16)
+    return HDN_EINVAL;
+  long long per_split = (P + splits - 1) / splits;
+  per_split = (per_split + 31) / 32 * 32;
+  if (per_split * (splits - 1) >= P) return HDN_EINVAL;  // every split must own positions
+  *per_split_out = per_split;
+  return HDN_OK;
+}
+
+void launch_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d, int h,
+                  int w, int cin, int cout, long long per_split, hipStream_t stream) {
+  const dim3 grid(static_cast<unsigned int>(cin / 16), static_cast<unsigned int>(cout / 16),
+                  static_cast<unsigned int>(splits));
+  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, stream, static_cast<const __bf16*>(x_cm),
+                     static_cast<const __bf16*>(dz_cm), part, n, d, h, w, cin, cout, per_split);
+}
+
+void launch_reduce(const float* part, float* out, long long len, int splits, hipStream_t stream) {
+  const unsigned int blocks = static_cast<unsigned int>(std::min<long long>(blocks_for(len, 256), 65535));
+  hipLaunchKernelGGL(k_reduce_splits, dim3(blocks), dim3(256), 0, stream, part, out, len, splits);
+}
+
+// Device address of `scratch` on each GPU, looked up once per device with that device
+// current (each GPU holds its own copy of the array at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_address[kMaxDevices] = {};
+
+int scratch_for_current_device(void* stream, float** out) {
+  int device = -1;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return HDN_EHIP;
+  if (stream) {
+    // The kernels run on the stream's device: it must be the current one, whose copy of
+    // the array is the one looked up.
+    int stream_device = -1;
+    if (hipStreamGetDevice(as_stream(stream), &stream_device) != hipSuccess) return HDN_EHIP;
+    if (stream_device != device) return HDN_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_address[device]) {
+    void* address = nullptr;
+    if (hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)) != hipSuccess || !address) return HDN_EHIP;
+    g_scratch_address[device] = static_cast<float*>(address);
+  }
+  *out = g_scratch_address[device];
+  return HDN_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -736,25 +796,33 @@ int hdn_conv3d_fwd(const void* x, const void* wt, const float* bias, void* y_nd,
 
 int hdn_conv3d_wgrad(const void* x_cm, const void* dz_cm, float* part, int splits, int n, int d,
                      int h, int w, int cin, int cout, void* stream) {
-  if (!x_cm || !dz_cm || !part || splits < 1 || !shape_ok(n, d, h, w)) return HDN_EINVAL;
-  const long long P = static_cast<long long>(n) * d * h * w;
-  if (w % 8 != 0 || P % 32 != 0 || cin % 16 != 0 || cout % 16 != 0) return HDN_EINVAL;
-  long long per_split = (P + splits - 1) / splits;
-  per_split = (per_split + 31) / 32 * 32;
-  if (per_split * (splits - 1) >= P) return HDN_EINVAL;  // every split must own positions
-  const dim3 grid(static_cast<unsigned int>(cin / 16), static_cast<unsigned int>(cout / 16),
-                  static_cast<unsigned int>(splits));
-  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, as_stream(stream),
-                     static_cast<const __bf16*>(x_cm), static_cast<const __bf16*>(dz_cm), part, n, d,
-                     h, w, cin, cout, per_split);
+  long long per_split = 0;
+  const int rc = wgrad_split(x_cm, dz_cm, splits, n, d, h, w, cin, cout, &per_split);
+  if (rc != HDN_OK || !part) return rc != HDN_OK ? rc : HDN_EINVAL;
+  launch_wgrad(x_cm, dz_cm, part, splits, n, d, h, w, cin, cout, per_split, as_stream(stream));
   return launch_status();
 }
 
 int hdn_reduce_splits(const float* part, float* out, long long len, int splits, void* stream) {
   if (!part || !out || len < 1 || splits < 1) return HDN_EINVAL;
-  const unsigned int blocks = static_cast<unsigned int>(std::min<long long>(blocks_for(len, 256), 65535));
-  hipLaunchKernelGGL(k_reduce_splits, dim3(blocks), dim3(256), 0, as_stream(stream), part, out, len,
-                     splits);
+  launch_reduce(part, out, len, splits, as_stream(stream));
+  return launch_status();
+}
+
+int hdn_conv3d_wgrad_noalloc(const void* x_cm, const void* dz_cm, float* grad, int splits, int n,
+                             int d, int h, int w, int cin, int cout, void* stream) {
+  long long per_split = 0;
+  const int rc = wgrad_split(x_cm, dz_cm, splits, n, d, h, w, cin, cout, &per_split);
+  if (rc != HDN_OK || !grad) return rc != HDN_OK ? rc : HDN_EINVAL;
+  const long long len = 27LL * cout * cin;
+  if (static_cast<long long>(splits) * len > HDN_SCRATCH_FLOATS) return HDN_ESIZE;
+  float* part = nullptr;
+  const int srv = scratch_for_current_device(stream, &part);
+  if (srv != HDN_OK) return srv;
+  // Every partial is written before it is read, so the array needs no clearing.
+  const hipStream_t s = as_stream(stream);
+  launch_wgrad(x_cm, dz_cm, part, splits, n, d, h, w, cin, cout, per_split, s);
+  launch_reduce(part, grad, len, splits, s);
   return launch_status();
 }
 

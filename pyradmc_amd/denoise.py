@@ -47,6 +47,7 @@ CIN_PAD = 16          # input channels padded to the MFMA k-group multiple
 HEAD_OUT = 16         # head output channels (channel 0 is the prediction)
 WINDOW_XY = 64        # beam-aligned window, voxels
 SIGMA_FLOOR = 0.02    # loss weight floor, in units of the column scale
+SCRATCH_FLOATS = 16 * 1024 * 1024  # HDN_SCRATCH_FLOATS: fixed array of hdn_conv3d_wgrad_noalloc
 
 _c_void = ctypes.c_void_p
 _c_ll = ctypes.c_longlong
@@ -62,6 +63,7 @@ def _lib():
         "hdn_conv3d_fwd_src": [_c_void, _c_void, i, _c_void, _c_void, _c_void, _c_void, i, i, i, i, i, i, f,
                                _c_void],
         "hdn_conv3d_wgrad": [_c_void, _c_void, _c_void, i, i, i, i, i, i, i, _c_void],
+        "hdn_conv3d_wgrad_noalloc": [_c_void, _c_void, _c_void, i, i, i, i, i, i, i, _c_void],
         "hdn_reduce_splits": [_c_void, _c_void, _c_ll, i, _c_void],
         "hdn_channel_sum": [_c_void, _c_void, _c_ll, i, i, _c_void],
         "hdn_act_bwd": [_c_void, _c_void, _c_void, _c_void, _c_ll, i, f, _c_void],
@@ -102,6 +104,9 @@ def _ptr(t) -> int | None:
 
 def _call(name: str, *args) -> None:
     rc = getattr(_lib(), name)(*args)
+    if rc == -3:
+        raise ValueError(f"{name}: the call needs more than the library's fixed device array "
+                         f"of {SCRATCH_FLOATS} floats (status {rc})")
     if rc != 0:
         raise (ValueError if rc == -1 else RuntimeError)(f"{name} failed with status {rc}")
 
