@@ -138,9 +138,25 @@ def _dij_leg(median: float, launch: dict | None = None) -> dict:
     d = engine.run_dij(src, counts, kw["n_batches"], seed=kw["seed"], truncation=kw["truncation"],
                        value_dtype=np.float32)
     wall = time.perf_counter() - started
-    engine.close()
     sig = _per_beamlet_sigma(d)
     med = float(np.nanmedian(sig))
+    # Sigma scales as N^-1/2 at a fixed layout: every count scaled so that the worst
+    # (90th percentile) column reaches 2 percent, and that Dij run and timed.
+    scale = (float(np.nanpercentile(sig, 90)) / 0.02) ** 2
+    counts2 = np.maximum(kw["n_batches"], np.ceil(counts * scale / kw["n_batches"]).astype(np.int64)
+                         * kw["n_batches"])
+    started = time.perf_counter()
+    d2 = engine.run_dij(src, counts2, kw["n_batches"], seed=kw["seed"] + 1, truncation=kw["truncation"],
+                        value_dtype=np.float32)
+    wall2 = time.perf_counter() - started
+    sig2 = _per_beamlet_sigma(d2)
+    at_2pct = {"count_scale": round(scale, 3), "histories": int(counts2.sum()), "wall_s": round(wall2, 4),
+               "histories_per_s": round(float(counts2.sum()) / wall2, 1),
+               "per_beamlet_sigma_median": round(float(np.nanmedian(sig2)), 5),
+               "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig2, 90)), 5)}
+    launch_used = dict(engine.launch, chunk_size=engine.chunk_size or "auto")
+    engine.close()
+    del d2
     return {
         "workload": "CT-like 128x128x96 @ 2.5 mm, 100 beamlets 5x5 mm, SAD 100, lognormal counts "
                     f"(median {median:g}, sigma 0.8), truncation 1e-4, f32 CSC",
@@ -150,12 +166,10 @@ def _dij_leg(median: float, launch: dict | None = None) -> dict:
         "nnz": int(d.indptr[-1]),
         "per_beamlet_sigma_median": round(med, 5),
         "per_beamlet_sigma_p90": round(float(np.nanpercentile(sig, 90)), 5),
-        # Sigma scales as N^-1/2 at fixed layout: projected wall time for every beamlet's
-        # median sigma at 2 percent, from the worst (90th percentile) column.
-        "wall_s_at_2pct_projected": round(wall * (float(np.nanpercentile(sig, 90)) / 0.02) ** 2, 4),
+        "at_2pct": at_2pct,
         "events_per_history": _events(d.counters or {}, float(counts.sum())),
         "seconds": _phase_seconds(d.counters or {}),
-        "launch": dict(engine.launch, chunk_size=engine.chunk_size or "auto"),
+        "launch": launch_used,
     }
 
 
@@ -395,7 +409,36 @@ def _cpu_baseline(sample_histories: int, threads: int) -> dict:
         "kind": "port",
         "sample": f"heterogeneous slab phantom, {sample_histories} histories in 10 batches, "
                   f"{threads} threads, {wall:.1f} s",
+        "host_cpu": _cpu_model(),
+        "reference_engine": _reference_engine_rate(),
     }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _reference_engine_rate() -> dict | None:
+    """The reference's own CPU path on the same workload, as recorded with its slab
+    fixture: pyRadMC's single-threaded NumPy ReferenceEngine, timed when the fixture was
+    written in the build container (it cannot run where the bench runs)."""
+    fixture = ROOT / "tests" / "golden" / "ref_hi_slab.npz"
+    if not fixture.is_file():
+        return None
+    with np.load(fixture) as g:
+        n, cpu_s = int(g["n_histories"]), float(g["cpu_seconds"])
+        shards, batches = int(g["shards"]), int(g["n_batches"])
+    return {"value": round(n / cpu_s, 1), "unit": "histories/s", "cores": 1, "kind": "reference",
+            "sample": f"pyRadMC ReferenceEngine on the heterogeneous slab phantom, {n} histories in "
+                      f"{shards} single-threaded processes x {batches // shards} batches, {cpu_s:.0f} "
+                      "CPU-seconds; measured in the build container (Intel Xeon @ 2.70 GHz, not the "
+                      "GPU host)"}
 
 
 def _guarded(name: str, leg, *args):
@@ -501,11 +544,12 @@ def main() -> int:
     mean_launch_s = counters["electron_kernel_ms"] / 1e3 / launches
     bytes_per_launch = _algorithmic_bytes(counters, "electron") / launches
     achieved = bytes_per_launch / mean_launch_s / 1e9 if mean_launch_s > 0 else 0.0
-    traffic = None
+    traffic, traffic_at = None, None
     tfile = ROOT / "profiles" / "traffic_electron_round.json"
     if tfile.is_file():
         try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+            tinfo = json.loads(tfile.read_text())
+            traffic, traffic_at = tinfo.get("hbm_bytes_per_launch"), tinfo.get("workload")
         except (OSError, ValueError):
             traffic = None
     roofline = {
@@ -515,6 +559,9 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 6),
         "traffic": traffic,
+        # HBM bytes come from a separate rocprofv3 --pmc run (FETCH_SIZE, WRITE_SIZE), not
+        # from this one: the launch shape it was profiled at is named here
+        "traffic_profiled": {"source": "profiles/traffic_electron_round.json", "workload": traffic_at},
         "kernel": "k_electron_round",
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
         "mean_launch_ms": round(mean_launch_s * 1e3, 4),
