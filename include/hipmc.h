@@ -197,6 +197,40 @@ int hmc_reset_maps(hmc_ctx *ctx);
 int hmc_set_quanta(hmc_ctx *ctx, const int64_t *quanta);
 int hmc_get_quanta(hmc_ctx *ctx, int64_t *quanta);
 
+/* Sampler draws for the device-level physics tests (no counterpart in the reference's
+ * engine API; the probe its own device sampler tests launch, tests/physics/
+ * test_warp_physics.py:54-120 and test_warp_gs_sampler.py:43-128). n independent
+ * streams (keys 0..n-1 of seed) each draw every scalar sampler the transport rounds
+ * use, with the rounds' own device functions; out is host float [n][HMC_SAMPLE_COLUMNS]:
+ *   0 free path at path_mu             1 channel (0 Compton .. 3 Rayleigh) from channel_mu
+ *   2 Compton ratio at compton_energy[0]  3 its photon cosine  4 its electron cosine
+ *   5 Compton ratio at compton_energy[1]  6 |isotropic|^2   7 isotropic z
+ *   8 |isotropic turned by (rotate_cos, rotate_phi)|^2
+ *   9 Moller delta energy at (moller_energy, moller_cut)  10 delta cosine  11 primary cosine
+ *  12 Gaussian hinge cosine at hinge_ms
+ *  13 bremsstrahlung emission probability and 14 local part for (brems_energy,
+ *     brems_radiative_loss, brems_pcut)   15 bremsstrahlung photon energy
+ *  16 coherent cosine at (coherent_energy, coherent_material) from the uploaded tables
+ *  17, 18 GS cosine at (gs_log_eta, gs_ms[0]) and (gs_log_eta, gs_ms[1]) from the uploaded grid
+ *  19 the uniform drawn after a GS sample, 20 after a Gaussian hinge, from the same state
+ *  21 z of (0, 0, 1) turned by (rotate_cos, rotate_phi) (the rotation's pole branch)
+ * Columns 16-19 are NaN without hmc_set_tables (and hmc_set_gs for 17-19). */
+#define HMC_SAMPLE_COLUMNS 22
+typedef struct hmc_sampler_args {
+  double path_mu;
+  double channel_mu[4];
+  double compton_energy[2];
+  double moller_energy, moller_cut;
+  double hinge_ms;
+  double brems_energy, brems_radiative_loss, brems_pcut;
+  double coherent_energy;
+  int coherent_material, reserved;
+  double gs_log_eta, gs_ms[2];
+  double rotate_cos, rotate_phi;
+} hmc_sampler_args;
+int hmc_sample_physics(hmc_ctx *ctx, uint64_t seed, int64_t n, const hmc_sampler_args *args,
+                       float *out);
+
 /* Host-side Goudsmit-Saunderson node grid (build_gs_grid, goudsmit_saunderson.py:
  * 1108-1219): fills out[(ix1-ix0+1)][(iy1-iy0+1)][n_u] with scaled-deflection tables
  * at eta = exp(i/bins), <theta^2> = exp(j/bins). n_threads <= 0 uses all cores. */

@@ -114,6 +114,21 @@ class WorkItem(Structure):
     _fields_ = [("key_base", c_int64), ("count", c_int64), ("column", c_int32), ("beamlet", c_int32)]
 
 
+class SamplerArgs(Structure):
+    """Inputs of ``hmc_sample_physics`` (include/hipmc.h documents the output columns)."""
+
+    _fields_ = [
+        ("path_mu", c_double), ("channel_mu", c_double * 4), ("compton_energy", c_double * 2),
+        ("moller_energy", c_double), ("moller_cut", c_double), ("hinge_ms", c_double),
+        ("brems_energy", c_double), ("brems_radiative_loss", c_double), ("brems_pcut", c_double),
+        ("coherent_energy", c_double), ("coherent_material", c_int), ("reserved", c_int),
+        ("gs_log_eta", c_double), ("gs_ms", c_double * 2), ("rotate_cos", c_double), ("rotate_phi", c_double),
+    ]
+
+
+SAMPLE_COLUMNS = 22
+
+
 class Ledger(Structure):
     _fields_ = [
         (name, c_int64)
@@ -176,6 +191,7 @@ def lib():
         "hmc_build_gs_grid": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, _f64p]),
         "hmc_gather_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p]),
         "hmc_scatter_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p, c_void_p]),
+        "hmc_sample_physics": (c_int, [ctx_p, c_uint64, c_int64, POINTER(SamplerArgs), _f32p]),
     }
     for name, (restype, argtypes) in sig.items():
         fn = getattr(h, name)

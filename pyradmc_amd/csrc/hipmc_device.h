@@ -234,6 +234,50 @@ __device__ inline void rotate(float& ux, float& uy, float& uz, float cos_t, floa
   }
 }
 
+// Photon channel from the four attenuation coefficients with one uniform, in the order
+// Compton, photoelectric, pair, Rayleigh (pyradmc/physics/channel.py:17-60).
+__device__ inline int select_process(float mu_c, float mu_p, float mu_pp, float mu_r, uint64_t& rng) {
+  const float threshold = uniform(rng) * (mu_c + mu_p + mu_pp + mu_r);
+  if (threshold < mu_c) return COMPTON;
+  if (threshold < mu_c + mu_p) return PHOTOELECTRIC;
+  if (threshold < mu_c + mu_p + mu_pp) return PAIR;
+  return RAYLEIGH;
+}
+
+// Isotropic unit direction from two uniforms (pyradmc/physics/direction.py:21-31).
+__device__ inline void sample_isotropic(uint64_t& rng, float& ax, float& ay, float& az) {
+  const float c = 1.0f - 2.0f * uniform(rng);
+  const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+  const float phi = kTwoPi * uniform(rng);
+  float sp, cp;
+  __sincosf(phi, &sp, &cp);
+  ax = s * cp;
+  ay = s * sp;
+  az = c;
+}
+
+// Bernoulli bremsstrahlung over a sub-step whose radiative loss is radiative_loss:
+// the emission probability that matches the radiative stopping power with photons
+// of log-uniform energy in (pcut, e), and the part below pcut deposited locally
+// (pyradmc/physics/brems.py:30-66). log_e and log_pcut are logf(e) and logf(pcut).
+__device__ inline void brems_step(float e, float radiative_loss, float pcut, float log_e, float log_pcut,
+                                  float& emit_probability, float& local) {
+  emit_probability = 0.0f;
+  local = radiative_loss;
+  if (e > pcut) {
+    const float emitted_fraction = 1.0f - pcut / e;
+    const float mean_k = (e - pcut) / (log_e - log_pcut);
+    emit_probability = radiative_loss * emitted_fraction / mean_k;
+    local = radiative_loss * (1.0f - emitted_fraction);
+  }
+}
+
+// Log-uniform bremsstrahlung photon energy k = pcut (e / pcut)^u
+// (pyradmc/physics/brems.py:69-84).
+__device__ inline float sample_brems_energy(float pcut, float log_e, float log_pcut, uint64_t& rng) {
+  return pcut * __expf(uniform(rng) * (log_e - log_pcut));
+}
+
 // Exponential free path -log(1-u)/mu (pyradmc/physics/path.py:15-35).
 __device__ inline float sample_path(float mu, uint64_t& rng) {
   return -__logf(1.0f - uniform(rng)) / mu;

@@ -401,12 +401,8 @@ __device__ inline void annihilate(const TransportParams& P, LaneTally& tal, cons
     deposit<kFast>(P, tal, base, x, y, z, w * 2.0f * kElectronMass, factor);
     return;
   }
-  const float c = 1.0f - 2.0f * uniform(rng);
-  const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
-  const float phi = kTwoPi * uniform(rng);
-  float sp, cp;
-  __sincosf(phi, &sp, &cp);
-  const float ax = s * cp, ay = s * sp, az = c;
+  float ax, ay, az;
+  sample_isotropic(rng, ax, ay, az);
   const uint64_t r1 = spawn_stream(rng);
   push(qp, PHOTON, column, kElectronMass, w, x, y, z, ax, ay, az, r1);
   const uint64_t r2 = spawn_stream(rng);
@@ -540,12 +536,7 @@ k_photon_round(TransportParams P, Queue qin,
     }
     if (uniform(rng) * mu_maj >= mu_real) continue;  // delta scattering
 
-    const float threshold = uniform(rng) * (mu_c + mu_p + mu_pp + mu_r);
-    int process;
-    if (threshold < mu_c) process = COMPTON;
-    else if (threshold < mu_c + mu_p) process = PHOTOELECTRIC;
-    else if (threshold < mu_c + mu_p + mu_pp) process = PAIR;
-    else process = RAYLEIGH;
+    const int process = select_process(mu_c, mu_p, mu_pp, mu_r, rng);
 
     if (process == RAYLEIGH) {
       const float cos_r = sample_coherent(P.t, mat, e, rng);
@@ -747,14 +738,8 @@ k_electron_round(TransportParams P, Queue qin,
     // Bernoulli bremsstrahlung matched to the radiative stopping power
     // (pyradmc/physics/brems.py:30-66).
     const float radiative_loss = ea.w * rho * s;
-    float emit_probability = 0.0f;
-    float local_brems = radiative_loss;
-    if (e > P.pcut) {
-      const float emitted_fraction = 1.0f - P.pcut / e;
-      const float mean_k = (e - P.pcut) / (log_e - P.log_pcut);
-      emit_probability = radiative_loss * emitted_fraction / mean_k;
-      local_brems = radiative_loss * (1.0f - emitted_fraction);
-    }
+    float emit_probability, local_brems;
+    brems_step(e, radiative_loss, P.pcut, log_e, P.log_pcut, emit_probability, local_brems);
     const float continuous = de + local_brems;
     const bool emit_brems = uniform(rng) < emit_probability;
     const float factor = spr<kFast>(P, e, mat);
@@ -791,7 +776,7 @@ k_electron_round(TransportParams P, Queue qin,
     rotate(ux, uy, uz, cos_hinge, phi);
 
     if (emit_brems && e > P.pcut) {
-      float k = P.pcut * __expf(uniform(rng) * (log_e_mid - P.log_pcut));
+      float k = sample_brems_energy(P.pcut, log_e_mid, P.log_pcut, rng);
       k = fminf(k, e - d2);
       if (k > P.pcut) {
         float photon_w = w;
@@ -1384,6 +1369,7 @@ struct hmc_ctx {
   double* d_out_dose = nullptr;
   double* d_out_sigma = nullptr;
   float* d_out32 = nullptr;  // float32 copies of the compacted dose and sigma (hmc_fetch_csc_f32)
+  float* d_samples = nullptr;  // hmc_sample_physics output
   long long trunc_total = 0;
   size_t out_cap = 0;
 };
@@ -1452,6 +1438,75 @@ int blocks_for(long long n, int block = 256, long long cap = 1 << 20) {
 
 }  // namespace
 
+// Draws of every scalar sampler the transport rounds use, one independent stream per
+// sample (key = sample index), for the sampler tests: out[i][column] as documented at
+// hmc_sample_physics. GS and coherent columns need the context's tables (NaN without).
+__global__ void __launch_bounds__(256) k_sample_physics(uint64_t seed, long long n, hmc_sampler_args a,
+                                                        DevTables t, int have_tables, int have_gs,
+                                                        float* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t rng = rng_init(seed, static_cast<uint64_t>(i));
+  float* o = out + i * HMC_SAMPLE_COLUMNS;
+  const float nan = __int_as_float(0x7fc00000);
+  o[0] = sample_path(static_cast<float>(a.path_mu), rng);
+  o[1] = static_cast<float>(select_process(static_cast<float>(a.channel_mu[0]), static_cast<float>(a.channel_mu[1]),
+                                           static_cast<float>(a.channel_mu[2]), static_cast<float>(a.channel_mu[3]),
+                                           rng));
+  const float e0 = static_cast<float>(a.compton_energy[0]);
+  const float r = sample_compton_ratio(e0, rng);
+  o[2] = r;
+  o[3] = compton_cos(e0, r);
+  o[4] = compton_electron_cos(e0, r);
+  o[5] = sample_compton_ratio(static_cast<float>(a.compton_energy[1]), rng);
+  float ax, ay, az;
+  sample_isotropic(rng, ax, ay, az);
+  o[6] = ax * ax + ay * ay + az * az;
+  o[7] = az;
+  float vx = ax, vy = ay, vz = az;
+  rotate(vx, vy, vz, static_cast<float>(a.rotate_cos), static_cast<float>(a.rotate_phi));
+  o[8] = vx * vx + vy * vy + vz * vz;
+  const float em = static_cast<float>(a.moller_energy);
+  const float w = sample_moller(em, static_cast<float>(a.moller_cut), rng);
+  o[9] = w;
+  float cd, cp;
+  moller_cosines(em, w, cd, cp);
+  o[10] = cd;
+  o[11] = cp;
+  o[12] = sample_gaussian_hinge(static_cast<float>(a.hinge_ms), rng);
+  const float eb = static_cast<float>(a.brems_energy), pc = static_cast<float>(a.brems_pcut);
+  const float log_eb = __logf(eb), log_pc = __logf(pc);
+  float prob, local;
+  brems_step(eb, static_cast<float>(a.brems_radiative_loss), pc, log_eb, log_pc, prob, local);
+  o[13] = prob;
+  o[14] = local;
+  o[15] = sample_brems_energy(pc, log_eb, log_pc, rng);
+  o[16] = have_tables ? sample_coherent(t, a.coherent_material, static_cast<float>(a.coherent_energy), rng) : nan;
+  // GS at (log eta, <theta^2>) with the round's keying; the draw after it and the draw
+  // after a Gaussian hinge from the same state (each consumes one uniform).
+  const float fx = static_cast<float>(a.gs_log_eta) * kGsBinsPerLog;
+  for (int k = 0; k < 2; ++k) {
+    const float ms = static_cast<float>(a.gs_ms[k]);
+    const float fy = __logf(fmaxf(ms, kGsTheta2Min)) * kGsBinsPerLog;
+    o[17 + k] = have_gs ? sample_gs(t, fx, fy, ms, rng) : nan;
+  }
+  uint64_t fork = rng;
+  const float ms0 = static_cast<float>(a.gs_ms[0]);
+  if (have_gs) {
+    (void)sample_gs(t, fx, __logf(fmaxf(ms0, kGsTheta2Min)) * kGsBinsPerLog, ms0, fork);
+    o[19] = uniform(fork);
+  } else {
+    o[19] = nan;
+  }
+  fork = rng;
+  (void)sample_gaussian_hinge(ms0, fork);
+  o[20] = uniform(fork);
+  // the pole guard: (0, 0, 1) turned by (rotate_cos, rotate_phi)
+  float px = 0.0f, py = 0.0f, pz = 1.0f;
+  rotate(px, py, pz, static_cast<float>(a.rotate_cos), static_cast<float>(a.rotate_phi));
+  o[21] = pz;
+}
+
 // ======================================================================== ABI
 
 extern "C" {
@@ -1497,7 +1552,7 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
                   ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
                   ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins,
-                  ctx->d_out32};
+                  ctx->d_out32,        ctx->d_samples};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
@@ -2325,6 +2380,33 @@ int hmc_fetch_csc(hmc_ctx* ctx, int64_t* indices, double* dose, double* sigma) {
                               ctx->stream));
   HMC_HIP(ctx, hipMemcpyAsync(sigma, ctx->d_out_sigma, sizeof(double) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HMC_OK;
+}
+
+int hmc_sample_physics(hmc_ctx* ctx, uint64_t seed, int64_t n, const hmc_sampler_args* args, float* out) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  if (!args || !out) return fail(ctx, HMC_EINVAL, "null argument");
+  if (n < 1 || n > (int64_t(1) << 24)) return fail(ctx, HMC_EINVAL, "sample count must be in [1, 2^24]");
+  if (!(args->path_mu > 0.0) || !(args->compton_energy[0] > 0.0) || !(args->compton_energy[1] > 0.0) ||
+      !(args->moller_energy > 2.0 * args->moller_cut) || !(args->moller_cut > 0.0) || args->hinge_ms < 0.0 ||
+      !(args->brems_pcut > 0.0) || !(args->brems_energy > 0.0) || !(args->coherent_energy > 0.0))
+    return fail(ctx, HMC_EINVAL, "sampler arguments out of range");
+  const double mu_sum = args->channel_mu[0] + args->channel_mu[1] + args->channel_mu[2] + args->channel_mu[3];
+  for (int k = 0; k < 4; ++k)
+    if (args->channel_mu[k] < 0.0) return fail(ctx, HMC_EINVAL, "negative channel coefficient");
+  if (!(mu_sum > 0.0)) return fail(ctx, HMC_EINVAL, "channel coefficients sum to zero");
+  if (ctx->has_tables && (args->coherent_material < 0 || args->coherent_material >= ctx->t.n_mat))
+    return fail(ctx, HMC_EINVAL, "coherent material has no table row");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  int rc;
+  const size_t count = static_cast<size_t>(n) * HMC_SAMPLE_COLUMNS;
+  if ((rc = dev_alloc(ctx, &ctx->d_samples, count))) return rc;
+  hipLaunchKernelGGL(k_sample_physics, dim3(blocks_for(n, 256, 1LL << 20)), dim3(256), 0, ctx->stream, seed,
+                     static_cast<long long>(n), *args, ctx->t, ctx->has_tables ? 1 : 0,
+                     (ctx->has_tables && ctx->has_gs) ? 1 : 0, ctx->d_samples);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipMemcpyAsync(out, ctx->d_samples, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HMC_OK;
 }
