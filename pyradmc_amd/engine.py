@@ -47,6 +47,10 @@ _CHUNK_MEMORY_FRACTION = 0.25
 _QUEUE_BYTES_PER_SLOT = 4 * 48
 _GROUP_MEMORY_FRACTION = 0.35
 _SLOT_MEMORY_FRACTION = 0.25
+# Host-sampled primaries: histories sampled per sample_batch call, and at most this many
+# primaries (36 B each on the device, about twice that on the host) per transport call.
+_SAMPLE_BLOCK = 1 << 20
+_SAMPLE_CALL = 4 << 20
 
 
 def _device_index(device) -> int:
@@ -426,6 +430,30 @@ class HipEngine:
         self._ctx.call("hmc_set_uploaded", int(kind.size), nat.ptr(kind, nat.c_int32),
                        *[nat.ptr(c, nat.ctypes.c_float) for c in cols])
 
+    def _presampled_transport(self, seed: int, items: list[tuple[int, int, int, int]], chunk: int,
+                              sample, books: dict) -> None:
+        """Transport host-sampled primaries a bounded piece at a time: each work item is
+        cut into sampling blocks of ``_SAMPLE_BLOCK`` histories aligned to its first key
+        (so what is sampled depends on the keys only, not on slots or chunk size), and
+        blocks go to the device in calls of at most ``_SAMPLE_CALL`` primaries, all
+        tallying into the same slots before their fold. Host and device memory for
+        primaries stay bounded whatever the run's size (the reference samples one chunk
+        at a time, pyradmc/backends/warp/engine.py:2506-2521). ``sample(key, n, beamlet)``
+        returns one block's columns."""
+        pieces = []
+        for key, count, column, beamlet in items:
+            for start in range(0, int(count), _SAMPLE_BLOCK):
+                pieces.append((int(key) + start, min(_SAMPLE_BLOCK, int(count) - start), column, beamlet))
+        call, size = [], 0
+        for piece in pieces + [None]:
+            if call and (piece is None or size + piece[1] > _SAMPLE_CALL):
+                self._upload_primaries([sample(key, n, beamlet) for key, n, _, beamlet in call])
+                self._merge(books, self._transport(seed, call, chunk))
+                call, size = [], 0
+            if piece is not None:
+                call.append(piece)
+                size += piece[1]
+
     def _transport(self, seed: int, items: list[tuple[int, int, int, int]], chunk: int) -> dict:
         """items: (key_base, count, flat tally column, beamlet parameter index)."""
         arr = (nat.WorkItem * len(items))()
@@ -546,9 +574,10 @@ class HipEngine:
             block = range(b0, min(b0 + slots, n_batches))
             items = [(b * per_batch, per_batch, b - b0, 0) for b in block]
             if presample:
-                self._upload_primaries([source.sample_batch(seed, key, n) for key, n, _, _ in items])
-            ledger = self._transport(seed, items, chunk)
-            self._merge(books, ledger)
+                self._presampled_transport(seed, items, chunk, lambda key, n, _: source.sample_batch(seed, key, n),
+                                           books)
+            else:
+                self._merge(books, self._transport(seed, items, chunk))
             deposited_quanta += self._fold(len(block), np.full(len(block), float(per_batch)))
             for _ in block:
                 emitter.tick(per_batch)
@@ -616,10 +645,11 @@ class HipEngine:
                     key = r0 if correlated else int(offsets[j]) + r0
                     items.append((key, int(per_batch[j]), (b - b0) * n_cols + (j - g0), j - g0))
             if not _in_kernel(source):
-                self._upload_primaries([source.sample_beamlet_batch(seed, key, n, g0 + local)
-                                        for key, n, _, local in items])
-            ledger = self._transport(seed, items, chunk)
-            self._merge(books, ledger)
+                self._presampled_transport(
+                    seed, items, chunk, lambda key, n, local: source.sample_beamlet_batch(seed, key, n, g0 + local),
+                    books)
+            else:
+                self._merge(books, self._transport(seed, items, chunk))
             nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
             deposited += self._fold(len(block), nhist)
         self._ctx.call("hmc_finalize", int(n_batches), None, None)

@@ -535,6 +535,51 @@ def test_presampled_source_matches_in_kernel_generator():
     assert _balance(b) < 1e-6
 
 
+class _RecordingParallel(_HostParallel):
+    """_HostParallel that records every (history offset, count) it is asked for."""
+
+    def __init__(self):
+        self.calls = []
+
+    def sample_batch(self, seed, history_offset, n):
+        self.calls.append((int(history_offset), int(n)))
+        return super().sample_batch(seed, history_offset, n)
+
+
+def test_presampled_source_is_sampled_in_bounded_blocks(monkeypatch):
+    # A host-sampled run far larger than one transport call's primaries: with blocks
+    # of 10,000 and calls of at most 30,000 primaries, 400,000 histories in 8 batches
+    # go through in many calls; every batch's keys are sampled exactly once, in blocks
+    # aligned to the batch's first key, so the result does not depend on the chunk
+    # size, and it matches the in-kernel generator's field statistically.
+    import pyradmc_amd.engine as engine_module
+    from pyradmc_amd import ParallelBeamSource, VoxelGrid
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    monkeypatch.setattr(engine_module, "_SAMPLE_BLOCK", 10_000)
+    monkeypatch.setattr(engine_module, "_SAMPLE_CALL", 30_000)
+    grid = VoxelGrid.uniform_water((16, 16, 16), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    eng = _engine(grid, xs)
+    src = _RecordingParallel()
+    r = eng.run(src, 400_000, 8, seed=6)
+    calls = sorted(src.calls)
+    expect = [(b * 50_000 + k * 10_000, 10_000) for b in range(8) for k in range(5)]
+    assert calls == expect
+    eng_small = _engine(grid, xs, chunk_size=7_000)
+    again = eng_small.run(_RecordingParallel(), 400_000, 8, seed=6)
+    ref = eng.run(ParallelBeamSource(2.0, -0.1, (2.0, 6.0), (2.0, 6.0)), 400_000, 8, seed=7)
+    eng.close()
+    eng_small.close()
+    np.testing.assert_array_equal(again.dose, r.dose)
+    assert r.energy_emitted == pytest.approx(400_000 * 2.0, rel=1e-12)
+    assert _balance(r) < 1e-6
+    pooled = 0.5 * (r.dose + ref.dose)
+    mask = pooled > 0.1 * pooled.max()
+    p, *_ = batched_chi2_pvalue(r.dose, r.dose_sigma, 8, ref.dose, ref.dose_sigma, 8, mask)
+    assert p > ALPHA
+
+
 class _MixedSource:
     """Photons, electrons and positrons (IAEA codes 1, 2, 3) with weights."""
     max_energy = 3.0

@@ -174,3 +174,41 @@ def test_denoiser_forward_flops_hand_count():
     base = denoise.forward_flops((1, 16, 16, 64))
     assert denoise.forward_flops((3, 16, 16, 64)) == 3 * base
     assert denoise.forward_flops((1, 32, 32, 128)) == 8 * base
+
+
+def test_presampled_transport_cuts_items_into_aligned_blocks_and_bounded_calls(monkeypatch):
+    # The host-sampling schedule of HipEngine, driven through stand-ins for the device
+    # calls: blocks of at most _SAMPLE_BLOCK histories aligned to each item's first key,
+    # calls of at most _SAMPLE_CALL primaries, every history sampled once, items kept
+    # in order with their tally column and beamlet.
+    import pyradmc_amd.engine as E
+
+    monkeypatch.setattr(E, "_SAMPLE_BLOCK", 100)
+    monkeypatch.setattr(E, "_SAMPLE_CALL", 250)
+
+    class Fake:
+        def __init__(self):
+            self.uploads, self.transports = [], []
+
+        def _upload_primaries(self, samples):
+            self.uploads.append(samples)
+
+        def _transport(self, seed, items, chunk):
+            self.transports.append(list(items))
+            return {"n": sum(n for _, n, _, _ in items)}
+
+        def _merge(self, books, ledger):
+            books["n"] = books.get("n", 0) + ledger["n"]
+
+    fake = Fake()
+    items = [(1000, 230, 0, 0), (5000, 90, 1, 3), (7000, 310, 2, 1)]
+    books: dict = {}
+    E.HipEngine._presampled_transport(fake, 11, items, 64, lambda key, n, beamlet: (key, n, beamlet), books)
+    assert books["n"] == 230 + 90 + 310
+    pieces = [p for call in fake.transports for p in call]
+    assert pieces == [(1000, 100, 0, 0), (1100, 100, 0, 0), (1200, 30, 0, 0), (5000, 90, 1, 3),
+                      (7000, 100, 2, 1), (7100, 100, 2, 1), (7200, 100, 2, 1), (7300, 10, 2, 1)]
+    assert all(sum(n for _, n, _, _ in call) <= 250 for call in fake.transports)
+    # each call uploads the samples of exactly its pieces, in order
+    for call, uploaded in zip(fake.transports, fake.uploads):
+        assert uploaded == [(k, n, b) for k, n, _, b in call]
