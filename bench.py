@@ -215,7 +215,7 @@ def _denoiser_roofline(model, shape: tuple[int, int, int, int], repeats: int = 5
             "ms_per_batch": round(seconds * 1e3, 3)}
 
 
-def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None = None) -> dict:
+def _denoise_leg(model_path: Path, low: int, audit: float, target: float, launch: dict | None = None) -> dict:
     """The 100-beamlet CT-like Dij at a dose-weighted RMS error target against a
     100x-history Monte Carlo reference: plain Monte Carlo (histories per beamlet set
     from a measured error at 1e5 and sigma ~ N^-1/2, then run and timed) versus
@@ -240,8 +240,8 @@ def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None 
     plain = engine.run_dij(src, n_plain, nb, seed=9, truncation=1e-4, value_dtype=np.float32)
     wall_plain = time.perf_counter() - started
     plain_err = _window_errors(plain, ref, src, grid)
-    builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, tolerance=target,
-                                         fallback_histories=n_plain, n_batches=nb)
+    builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, audit_fraction=audit,
+                                         tolerance=target, fallback_histories=n_plain, n_batches=nb)
     mfma = _denoiser_roofline(model, (32, grid.shape[2], denoise.WINDOW_XY, denoise.WINDOW_XY))
     builder.build(src, seed=11, value_dtype=np.float32)  # warm (kernels, allocator)
     started = time.perf_counter()
@@ -257,12 +257,19 @@ def _denoise_leg(model_path: Path, low: int, target: float, launch: dict | None 
         "plain_wall_s": round(wall_plain, 4),
         "plain_error_median": round(float(np.median(plain_err)), 5),
         "plain_error_p90": round(float(np.percentile(plain_err, 90)), 5),
-        "denoised_low_histories_per_beamlet": 2 * low,
+        "denoised_low_histories_per_beamlet": low,
+        "denoised_audit_fraction": audit,
         "denoised_wall_s": round(wall_den, 4),
         "denoised_seconds": {k: round(v, 4) for k, v in report.seconds.items()},
         "denoised_fallback_columns": int(report.fallback.sum()),
         "denoised_error_median": round(float(np.median(den_err)), 5),
         "denoised_error_p90": round(float(np.percentile(den_err, 90)), 5),
+        # the plain-MC audit's estimate of the denoised error, and the measured error of
+        # the columns that stayed denoised (against the 100x reference)
+        "audit_columns": int(report.audited.sum()),
+        "audit_error": round(float(report.audit_error), 5),
+        "denoised_only_error_median": (round(float(np.median(den_err[~report.fallback])), 5)
+                                       if (~report.fallback).any() else None),
         "inference_roofline": mfma,
         "speedup": round(wall_plain / wall_den, 3) if wall_den > 0 else None,
     }
@@ -469,6 +476,10 @@ def main() -> int:
     ap.add_argument("--denoise-model", default=str(ROOT / "pyradmc_amd" / "models" / "denoiser_ct.npz"),
                     help="trained denoiser weights for the denoised-Dij leg (skipped if absent)")
     ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--denoise-low", type=int, default=20_000,
+                    help="histories per beamlet of the denoised build's input run")
+    ap.add_argument("--denoise-audit", type=float, default=0.1,
+                    help="fraction of the denoised build's columns re-simulated with plain Monte Carlo as its judge")
     ap.add_argument("--no-plan", action="store_true", help="skip the full-plan Dij and product leg")
     ap.add_argument("--plan-probe", type=int, default=20_000,
                     help="probe histories per beamlet of the full-plan leg")
@@ -576,7 +587,8 @@ def main() -> int:
     pencil = None if args.no_pencil else _guarded("pencil", _pencil_leg)
     dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median, shape)
     dij_shape = shape if shape is not None else (dij or {}).get("launch")
-    denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), 10_000, 0.02, dij_shape)
+    denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), args.denoise_low,
+                         args.denoise_audit, 0.02, dij_shape)
                 if denoise_leg else None)
     plan, plan_dij = None, None
     if not args.no_plan:

@@ -189,6 +189,13 @@ int hmc_gather_windows(hmc_ctx *ctx, const int32_t *origins, int64_t n_cols, int
                        float *out);
 int hmc_scatter_windows(hmc_ctx *ctx, const int32_t *origins, int64_t n_cols, int wx, int wy,
                         const float *mean, const float *sigma);
+/* Overwrite mean and sigma of one column at n voxels (flat scoring indices) after
+ * hmc_finalize or hmc_scatter_windows, before hmc_truncate: the denoised builder puts
+ * the plain Monte Carlo entries of its audited columns there, so one device truncation
+ * and compaction yields the whole Dij in column order. Host arrays; indices outside the
+ * grid are refused. */
+int hmc_set_column_entries(hmc_ctx *ctx, int64_t column, int64_t n, const int64_t *indices,
+                           const double *dose, const double *sigma);
 /* Clear the dense maps and sums for a new group. */
 int hmc_reset_maps(hmc_ctx *ctx);
 /* Replace / read the dense int64 quanta maps [n_slots][n_columns][n_voxels]: the synthetic

@@ -191,6 +191,7 @@ def lib():
         "hmc_build_gs_grid": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, _f64p]),
         "hmc_gather_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p]),
         "hmc_scatter_windows": (c_int, [ctx_p, POINTER(c_int32), c_int64, c_int, c_int, c_void_p, c_void_p]),
+        "hmc_set_column_entries": (c_int, [ctx_p, c_int64, c_int64, POINTER(c_int64), _f64p, _f64p]),
         "hmc_sample_physics": (c_int, [ctx_p, c_uint64, c_int64, POINTER(SamplerArgs), _f32p]),
     }
     for name, (restype, argtypes) in sig.items():

@@ -1095,6 +1095,17 @@ __global__ void k_scatter_windows(double* __restrict__ mean, double* __restrict_
   }
 }
 
+// mean / sigma of one column at the given voxels (indices checked on the host).
+__global__ void k_set_entries(double* __restrict__ mean, double* __restrict__ sigma, long long base,
+                              long long n, const long long* __restrict__ idx, const double* __restrict__ dose,
+                              const double* __restrict__ sig) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    mean[base + idx[i]] = dose[i];
+    sigma[base + idx[i]] = sig[i];
+  }
+}
+
 __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v) {
   for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -1370,6 +1381,8 @@ struct hmc_ctx {
   double* d_out_sigma = nullptr;
   float* d_out32 = nullptr;  // float32 copies of the compacted dose and sigma (hmc_fetch_csc_f32)
   float* d_samples = nullptr;  // hmc_sample_physics output
+  long long* d_entry_idx = nullptr;  // hmc_set_column_entries staging
+  double* d_entry_val = nullptr;
   long long trunc_total = 0;
   size_t out_cap = 0;
 };
@@ -1552,7 +1565,7 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
                   ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
                   ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins,
-                  ctx->d_out32,        ctx->d_samples};
+                  ctx->d_out32,        ctx->d_samples,     ctx->d_entry_idx,  ctx->d_entry_val};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
@@ -2330,6 +2343,30 @@ int hmc_scatter_windows(hmc_ctx* ctx, const int32_t* origins, int64_t n_cols, in
   HMC_HIP(ctx, hipGetLastError());
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->finalized = true;  // s1 / s2 now hold mean / sigma for hmc_truncate
+  return HMC_OK;
+}
+
+int hmc_set_column_entries(hmc_ctx* ctx, int64_t column, int64_t n, const int64_t* indices, const double* dose,
+                           const double* sigma) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  if (!ctx->finalized) return fail(ctx, HMC_ESTATE, "finalize (or scatter windows) before setting entries");
+  if (column < 0 || column >= ctx->n_columns) return fail(ctx, HMC_EINVAL, "column outside the scoring columns");
+  if (n < 0 || (n > 0 && (!indices || !dose || !sigma))) return fail(ctx, HMC_EINVAL, "null entries");
+  if (n == 0) return HMC_OK;
+  for (int64_t i = 0; i < n; ++i)
+    if (indices[i] < 0 || indices[i] >= ctx->s.n_voxels) return fail(ctx, HMC_EINVAL, "entry index outside the grid");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_entry_idx, reinterpret_cast<const long long*>(indices), static_cast<size_t>(n))))
+    return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_entry_val, static_cast<size_t>(2 * n)))) return rc;
+  HMC_HIP(ctx, hipMemcpy(ctx->d_entry_val, dose, sizeof(double) * n, hipMemcpyHostToDevice));
+  HMC_HIP(ctx, hipMemcpy(ctx->d_entry_val + n, sigma, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_set_entries, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, ctx->stream, ctx->d_s1, ctx->d_s2,
+                     static_cast<long long>(column) * ctx->s.n_voxels, static_cast<long long>(n), ctx->d_entry_idx,
+                     ctx->d_entry_val, ctx->d_entry_val + n);
+  HMC_HIP(ctx, hipGetLastError());
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HMC_OK;
 }
 

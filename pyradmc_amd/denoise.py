@@ -4,10 +4,11 @@ BF16 MFMA kernels, trained on column pairs the engine itself generates.
 A beamlet column computed with few histories is denoised inside a beam-aligned
 window (64 x 64 voxels around the beamlet's central axis, full depth) from two
 input channels - the low-history dose and the density - into a prediction of the
-full-statistics column. Plain Monte Carlo stays the fallback and the judge: every
-column is also scored by a second, independent low-history run; when the two
-denoised estimates disagree by more than the error budget, the column is
-re-simulated with full statistics (``DenoisedDijBuilder``).
+full-statistics column. Plain Monte Carlo stays the fallback and the judge: a random
+sample of the plan's columns is also simulated with full statistics, the denoised
+error is estimated on it without bias, and if that estimate is not within the error
+budget every column is simulated with full statistics instead
+(``DenoisedDijBuilder``).
 
 The reference has no learned component; this module extends its Dij builder. The
 network's forward and backward passes run entirely in this build's HIP kernels
@@ -665,42 +666,107 @@ def evaluate(model: Conv3dDenoiser, pairs: list[ColumnPair], batch: int = 32) ->
 
 @dataclass
 class DenoiseReport:
-    """Per-column outcome of a denoised Dij build."""
+    """Outcome of a denoised Dij build."""
 
-    disagreement: np.ndarray   # dose-weighted RMS of (estimate A - estimate B) / 2
-    fallback: np.ndarray       # bool: column re-simulated with plain Monte Carlo
+    audited: np.ndarray          # bool per column: re-simulated with plain Monte Carlo as the judge's sample
+    estimated_error: np.ndarray  # per audited column, the denoised column's estimated error (NaN elsewhere)
+    audit_error: float           # RMS of the audited estimates: the judge's verdict on the network here
+    fallback: np.ndarray         # bool per column: the column in the result is plain Monte Carlo
     seconds: dict
+
+
+def _window_entries(dij, column: int, window, shape):
+    """The entries of one CSC column inside a beam-aligned window: local (x, y, z)
+    coordinates and their dose and sigma (float64), read from the column's entries
+    only."""
+    wx, wy = window
+    lo, hi = int(dij.indptr[column]), int(dij.indptr[column + 1])
+    idx = np.asarray(dij.indices[lo:hi], dtype=np.int64)
+    _, ny, nz = shape
+    x, rest = np.divmod(idx, ny * nz)
+    y, z = np.divmod(rest, nz)
+    inside = (x >= wx.start) & (x < wx.stop) & (y >= wy.start) & (y < wy.stop)
+    return (x[inside] - wx.start, y[inside] - wy.start, z[inside],
+            np.asarray(dij.dose[lo:hi], dtype=np.float64)[inside],
+            np.asarray(dij.sigma[lo:hi], dtype=np.float64)[inside])
 
 
 class DenoisedDijBuilder:
     """Dij from low-history Monte Carlo plus the denoiser, with plain Monte Carlo as the
     fallback and the judge.
 
-    Every beamlet is simulated twice with ``low_histories`` (independent seeds); both
-    runs are denoised in their beam-aligned windows and averaged. A column whose two
-    denoised estimates disagree by more than ``tolerance`` (half the dose-weighted RMS
-    of their difference relative to the column maximum: the error of their average,
-    were the two errors independent) is re-simulated with ``fallback_histories``. Outside the window the
-    column keeps the averaged low-history dose. Sigma of a denoised entry is half the
-    difference of the two denoised estimates.
+    Every beamlet is simulated once with ``low_histories`` (one batch: the network reads
+    only the mean) and denoised in its beam-aligned window. Plain Monte Carlo judges the
+    network on this plan first: a seeded random sample of ``audit_fraction`` of the
+    columns (at least one) is simulated both ways, plain at ``fallback_histories``, and
+    for each sampled column, with D the denoised window and M, s_M the plain run's mean
+    and batch sigma,
+
+        e^2 = sum_v D_v ((D_v - M_v)^2 - s_M,v^2) / sum_v D_v / max(D)^2
+
+    over voxels above 10 percent of max(D) estimates the squared dose-weighted RMS error
+    of D against the truth without bias (M is independent of D). A network bias shows in
+    it like noise does. If the RMS of the sample's estimates is not below ``tolerance``
+    the network is not trusted for this plan and the result is the plain Monte Carlo
+    Dij of every column (bit for bit ``run_dij(source, fallback_histories, n_batches,
+    seed + 2)``). Otherwise the sampled columns keep their plain Monte Carlo entries and
+    every other column its denoised window; dose outside a window is dropped, and a
+    denoised entry's sigma is the audit's RMS estimate times its column maximum.
     """
 
-    def __init__(self, engine, model: Conv3dDenoiser, low_histories: int = 10_000,
-                 tolerance: float = 0.02, fallback_histories: int = 1_000_000, n_batches: int = 10):
+    def __init__(self, engine, model: Conv3dDenoiser, low_histories: int = 20_000,
+                 tolerance: float = 0.02, fallback_histories: int = 1_000_000, n_batches: int = 10,
+                 audit_fraction: float = 0.1):
         self.engine = engine
         self.model = model
         self.low = int(low_histories)
         self.tolerance = float(tolerance)
         self.fallback = int(fallback_histories)
         self.n_batches = int(n_batches)
+        self.audit_fraction = float(audit_fraction)
+        if self.low < 1:
+            raise ValueError("the low-history run needs at least one history per beamlet")
+        if not 0.0 < self.audit_fraction <= 1.0:
+            raise ValueError(f"audit_fraction must lie in (0, 1], got {audit_fraction}")
+
+    def _denoise_group(self, source, group: range, windows, seed: int, books: dict, dens, batch: int):
+        """Low-history means of the group's columns (on the device, one batch), their
+        windows gathered and denoised: (denoised windows [n][z][x][y], origins, the
+        fold's deposited quanta, seconds of Monte Carlo, seconds of inference)."""
+        import time
+
+        torch = _torch()
+        t0 = time.perf_counter()
+        eng, model = self.engine, self.model
+        depth = eng.grid.shape[2]
+        ng = len(group)
+        org = np.ascontiguousarray([[windows[j][0].start, windows[j][1].start] for j in group], dtype=np.int32)
+        counts = np.full(int(source.n_beamlets), self.low, dtype=np.int64)
+        _, _, quanta = eng._dij_group(source, group, counts, 1, seed, True, eng._default_scoring(), books)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        wins = torch.empty((ng, depth, WINDOW_XY, WINDOW_XY), dtype=torch.float32, device=model.device)
+        eng._ctx.call("hmc_gather_windows", _native.ptr(org, ctypes.c_int32), ng, WINDOW_XY, WINDOW_XY,
+                      wins.data_ptr())
+        scale = torch.topk(wins.reshape(ng, -1), 64, dim=1).values.mean(dim=1).clamp_min(1e-30)
+        noisy = wins / scale[:, None, None, None]
+        dwin = torch.stack([dens[:, windows[j][0], windows[j][1]] for j in group])  # [n][z][x][y]
+        preds = []
+        for b0 in range(0, ng, batch):
+            x, _ = _pack_device(noisy[b0:b0 + batch].contiguous(), dwin[b0:b0 + batch].contiguous())
+            preds.append(model.predict_tensor(x))
+        pred = torch.cat(preds).reshape(ng, depth, WINDOW_XY, WINDOW_XY).clamp_min(0.0) * scale[:, None, None, None]
+        torch.cuda.synchronize()
+        return pred, org, quanta, t1 - t0, time.perf_counter() - t1
 
     def build(self, source, seed: int, truncation: float = 1e-4, batch: int = 64,
               group_size: int | None = None, value_dtype=np.float64):
-        """(DijResult, DenoiseReport) for the source's beamlets. Columns go through the
-        engine in groups; each group's low-history means stay on the device, its
-        windows are gathered straight into the network's input buffers, and the
-        denoised means and sigmas are written back for the engine's own truncation
-        and CSC compaction. ``value_dtype`` is the CSC dose and sigma type, float64 or
+        """(DijResult, DenoiseReport) for the source's beamlets. The judge's sample runs
+        first; then columns go through the engine in groups: each group's means stay on
+        the device, the windows are gathered straight into the network's input buffers,
+        and the denoised windows (and the sampled columns' plain entries) are written
+        back for the engine's own truncation and CSC compaction, which return the
+        columns in order. ``value_dtype`` is the CSC dose and sigma type, float64 or
         float32, as in ``HipEngine.run_dij``."""
         import time
 
@@ -716,61 +782,101 @@ class DenoisedDijBuilder:
         eng, model = self.engine, self.model
         grid = eng.grid
         shape = tuple(grid.shape)
-        depth = shape[2]
         n_cols = int(source.n_beamlets)
+        nb = self.n_batches
         windows = _windows_for(grid, source, range(n_cols))
-        origins = np.ascontiguousarray([[wx.start, wy.start] for wx, wy in windows], dtype=np.int32)
         scoring = eng._default_scoring()
         eng._prepare(source)  # default cuts and physics, as run_dij's defaults
-        if group_size is None:
-            free = int(_native.device_info(eng._index)["free_bytes"])
-            per_col = 8 * scoring.n_voxels * (self.n_batches + 3)
-            group_size = max(1, min(n_cols, int(0.25 * free) // max(per_col, 1)))
         dens = torch.from_numpy(np.ascontiguousarray(np.transpose(np.asarray(grid.density, np.float32),
                                                                   (2, 0, 1)))).to(model.device)  # [z][x][y]
-        counts = np.full(n_cols, self.low, dtype=np.int64)
+        seconds = {"audit_mc": 0.0, "audit": 0.0, "low_mc": 0.0, "denoise": 0.0, "fallback_mc": 0.0,
+                   "assemble": 0.0}
+
+        def subset(cols):
+            return SpectralBeamletSource(source.spectrum, source.focal_point, [source.centers[j] for j in cols],
+                                         source.width_u, source.width_v, source.u_axis, source.v_axis)
+
+        # The judge. Correlated keys: a column's bits do not depend on which other columns
+        # run with it, so the sampled columns' plain runs are those of the full plain Dij.
+        rng = np.random.default_rng([int(seed), 3])
+        n_audit = max(1, int(math.ceil(self.audit_fraction * n_cols)))
+        audit_cols = sorted(rng.choice(n_cols, size=n_audit, replace=False).tolist())
+        audited = np.zeros(n_cols, dtype=bool)
+        audited[audit_cols] = True
+        audit_books: dict = {}
+        plain = eng.run_dij(subset(audit_cols), self.fallback, nb, seed=seed + 2, truncation=truncation)
+        self._merge_books(audit_books, plain)
+        seconds["audit_mc"] = time.perf_counter() - t0
+        sample = subset(audit_cols)
+        sample_windows = [windows[j] for j in audit_cols]
+        pred, _, _, _, _ = self._denoise_group(sample, range(n_audit), sample_windows, seed, {}, dens, batch)
+        # sum_v w (D - M)^2 - w s^2 = sum w D^2 - 2 sum w D M + sum w (M^2 - s^2): the first
+        # sum over the dense window (on the device), the others over M's entries only
+        flat = pred.reshape(n_audit, -1).double()
+        dmax_t = flat.max(dim=1).values
+        w_t = torch.where(flat > 0.1 * dmax_t[:, None], flat, torch.zeros_like(flat))
+        s_w = w_t.sum(dim=1).cpu().numpy()
+        s_wdd = (w_t * flat * flat).sum(dim=1).cpu().numpy()
+        dmax = dmax_t.cpu().numpy()
+        sample_pred = pred.cpu().numpy()  # [n][z][x][y], float32
+        del pred, flat, w_t
+        estimate = np.full(n_cols, np.nan)
+        for k, j in enumerate(audit_cols):
+            if not dmax[k] > 0.0:
+                estimate[j] = np.inf  # nothing to judge: the network is not trusted with it
+                continue
+            x, y, z, m, s_m = _window_entries(plain, k, windows[j], shape)
+            d = sample_pred[k][z, x, y].astype(np.float64)
+            w = np.where(d > 0.1 * dmax[k], d, 0.0)
+            e2 = (s_wdd[k] - 2.0 * float(np.sum(w * d * m)) + float(np.sum(w * (m * m - s_m * s_m)))) / s_w[k]
+            estimate[j] = math.sqrt(max(e2, 0.0)) / dmax[k]
+        audit_error = float(np.sqrt(np.mean(estimate[audited] ** 2)))
+        seconds["audit"] = time.perf_counter() - t0
+
+        if not audit_error < self.tolerance:
+            # not trusted here: plain Monte Carlo for every column
+            te = time.perf_counter()
+            result = eng.run_dij(source, self.fallback, nb, seed=seed + 2, truncation=truncation,
+                                 value_dtype=value_dtype)
+            seconds["fallback_mc"] = time.perf_counter() - te
+            seconds["total"] = time.perf_counter() - t0
+            return result, DenoiseReport(audited=audited, estimated_error=estimate, audit_error=audit_error,
+                                         fallback=np.ones(n_cols, dtype=bool), seconds=seconds)
+
+        if group_size is None:
+            free = _native.device_free_bytes(eng._index)
+            per_col = 8 * scoring.n_voxels * 4  # one map slot, the sums, and the window buffers
+            group_size = max(1, min(n_cols, int(0.25 * free) // max(per_col, 1)))
         books: dict = {}
-        deposited_quanta = 0  # dose-to-medium: the fold's quanta (the ledger books only escapes)
-        disagreement = np.zeros(n_cols)
+        deposited_quanta = 0
         col_counts, col_idx, col_dose, col_sigma = [], [], [], []
-        seconds = {"low_mc": 0.0, "denoise": 0.0, "fallback_mc": 0.0, "assemble": 0.0}
         for g0 in range(0, n_cols, group_size):
             group = range(g0, min(g0 + group_size, n_cols))
             ng = len(group)
-            org = np.ascontiguousarray(origins[g0:group.stop])
-            org_p = _native.ptr(org, ctypes.c_int32)
-            wins = torch.empty((2, ng, depth, WINDOW_XY, WINDOW_XY), dtype=torch.float32, device=model.device)
-            ta = time.perf_counter()
-            for k, s_run in enumerate((seed, seed + 1)):
-                _, _, quanta = eng._dij_group(source, group, counts, self.n_batches, s_run, True, scoring, books)
-                deposited_quanta += quanta
-                torch.cuda.synchronize()
-                eng._ctx.call("hmc_gather_windows", org_p, ng, WINDOW_XY, WINDOW_XY, wins[k].data_ptr())
-            tb = time.perf_counter()
-            flat = wins.reshape(2, ng, -1)
-            scale = torch.topk(flat, 64, dim=2).values.mean(dim=2).clamp_min(1e-30)  # [2][ng]
-            noisy = (wins / scale[:, :, None, None, None]).reshape(2 * ng, depth, WINDOW_XY, WINDOW_XY)
-            dwin = torch.stack([dens[:, windows[j][0], windows[j][1]] for j in group])  # [ng][z][x][y]
-            dwin2 = torch.cat([dwin, dwin], dim=0)
-            preds = []
-            for b0 in range(0, 2 * ng, batch):
-                x, _ = _pack_device(noisy[b0:b0 + batch].contiguous(), dwin2[b0:b0 + batch].contiguous())
-                preds.append(model.predict_tensor(x))
-            pred = torch.cat(preds).reshape(2, ng, depth, WINDOW_XY, WINDOW_XY).clamp_min(0.0)
-            pred = pred * scale[:, :, None, None, None]
-            mean = 0.5 * (pred[0] + pred[1])
-            sigma = 0.5 * (pred[0] - pred[1]).abs()
-            # per column: dose-weighted RMS of (estimate - mean) over mean > 0.1 max, / max
-            m = mean.reshape(ng, -1)
-            d = (pred[0] - mean).reshape(ng, -1)
-            mmax = m.max(dim=1).values.clamp_min(1e-30)
-            sel = (m > 0.1 * mmax[:, None]).to(m.dtype) * m
-            rms = torch.sqrt((sel * d * d).sum(dim=1) / sel.sum(dim=1).clamp_min(1e-30)) / mmax
-            disagreement[g0:group.stop] = rms.double().cpu().numpy()
+            pred, org, quanta, t_mc, t_net = self._denoise_group(source, group, windows, seed, books, dens, batch)
+            seconds["low_mc"] += t_mc
+            seconds["denoise"] += t_net
+            deposited_quanta += quanta
+            local_audited = [j - g0 for j in group if audited[j]]
+            if local_audited:
+                pred[local_audited] = 0.0
+            col_max = pred.reshape(ng, -1).max(dim=1).values
+            sigma = (audit_error * col_max)[:, None, None, None].expand_as(pred).contiguous()
             torch.cuda.synchronize()
             tc = time.perf_counter()
-            eng._ctx.call("hmc_scatter_windows", org_p, ng, WINDOW_XY, WINDOW_XY, mean.contiguous().data_ptr(),
-                          sigma.contiguous().data_ptr())
+            eng._ctx.call("hmc_scatter_windows", _native.ptr(org, ctypes.c_int32), ng, WINDOW_XY, WINDOW_XY,
+                          pred.contiguous().data_ptr(), sigma.data_ptr())
+            for j in group:
+                if audited[j]:
+                    k = audit_cols.index(j)
+                    lo, hi = int(plain.indptr[k]), int(plain.indptr[k + 1])
+                    eng._ctx.call("hmc_set_column_entries", j - g0, hi - lo,
+                                  _native.ptr(np.ascontiguousarray(plain.indices[lo:hi], dtype=np.int64),
+                                              _native.c_int64),
+                                  _native.ptr(np.ascontiguousarray(plain.dose[lo:hi], dtype=np.float64),
+                                              ctypes.c_double),
+                                  _native.ptr(np.ascontiguousarray(plain.sigma[lo:hi], dtype=np.float64),
+                                              ctypes.c_double))
             kept = np.zeros(ng, dtype=np.int64)
             total = _native.c_int64()
             eng._ctx.call("hmc_truncate", float(truncation), _native.ptr(kept, _native.c_int64),
@@ -788,47 +894,24 @@ class DenoisedDijBuilder:
             col_idx.append(idx)
             col_dose.append(dose)
             col_sigma.append(sig)
-            td = time.perf_counter()
-            seconds["low_mc"] += tb - ta
-            seconds["denoise"] += tc - tb
-            seconds["assemble"] += td - tc
+            seconds["assemble"] += time.perf_counter() - tc
         kept_all = np.concatenate(col_counts)
         indptr = np.zeros(n_cols + 1, dtype=np.int64)
         np.cumsum(kept_all, out=indptr[1:])
         indices, dose, sigma = _joined(col_idx, np.int64), _joined(col_dose, value_dtype), _joined(col_sigma, value_dtype)
-        fallback = disagreement > self.tolerance
-        te = time.perf_counter()
-        if fallback.any():
-            redo = np.flatnonzero(fallback).tolist()
-            sub = SpectralBeamletSource(source.spectrum, source.focal_point, [source.centers[j] for j in redo],
-                                        source.width_u, source.width_v, source.u_axis, source.v_axis)
-            mc = eng.run_dij(sub, self.fallback, self.n_batches, seed=seed + 2, truncation=truncation,
-                             value_dtype=value_dtype)
-            parts_i, parts_d, parts_s, new_counts = [], [], [], kept_all.copy()
-            slot = {j: k for k, j in enumerate(redo)}
-            for j in range(n_cols):
-                if j in slot:
-                    k = slot[j]
-                    lo, hi = int(mc.indptr[k]), int(mc.indptr[k + 1])
-                    parts_i.append(mc.indices[lo:hi]); parts_d.append(mc.dose[lo:hi]); parts_s.append(mc.sigma[lo:hi])
-                    new_counts[j] = hi - lo
-                else:
-                    lo, hi = int(indptr[j]), int(indptr[j + 1])
-                    parts_i.append(indices[lo:hi]); parts_d.append(dose[lo:hi]); parts_s.append(sigma[lo:hi])
-            indices, dose, sigma = np.concatenate(parts_i), np.concatenate(parts_d), np.concatenate(parts_s)
-            np.cumsum(new_counts, out=indptr[1:])
-            self._merge_books(books, mc)
-        seconds["fallback_mc"] = time.perf_counter() - te
+        for name in ("emitted", "escaped", "unscored"):
+            books[name] = books.get(name, 0) + audit_books.get(name, 0)
         seconds["total"] = time.perf_counter() - t0
         q = 1.0e-9  # ledger quanta -> MeV
         result = DijResult(
-            grid_shape=shape, n_beamlets=n_cols, n_histories_per_beamlet=2 * self.low, n_batches=self.n_batches,
+            grid_shape=shape, n_beamlets=n_cols, n_histories_per_beamlet=self.low, n_batches=1,
             truncation=float(truncation), indptr=indptr, indices=indices, dose=dose, sigma=sigma,
             energy_emitted=float(books.get("emitted", 0)) * q,
-            energy_deposited=float(deposited_quanta + books.get("fallback_deposited", 0)) * q,
+            energy_deposited=float(deposited_quanta + audit_books.get("fallback_deposited", 0)) * q,
             energy_escaped=float(books.get("escaped", 0)) * q, energy_unscored=float(books.get("unscored", 0)) * q,
             correlated=True, scoring_mode="dose_to_medium", provenance=None, counters=books)
-        return result, DenoiseReport(disagreement=disagreement, fallback=fallback, seconds=seconds)
+        return result, DenoiseReport(audited=audited, estimated_error=estimate, audit_error=audit_error,
+                                     fallback=audited.copy(), seconds=seconds)
 
     @staticmethod
     def _merge_books(books: dict, dij) -> None:

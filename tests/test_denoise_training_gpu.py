@@ -15,6 +15,9 @@ committed network gains over the Monte Carlo it starts from.
 * The committed model (pyradmc_amd/models/denoiser_ct.npz), on columns of a phantom it
   never saw, has a dose-weighted RMS error against the 1e6-history target well below
   that of its 1e4-history input: the gain over plain Monte Carlo at the same histories.
+* Plain Monte Carlo judges the denoised build: its audit accepts the trained network on
+  an unseen phantom and rejects the same network with a bias added to its prediction,
+  whereupon the result is the plain Dij bit for bit.
 """
 
 from __future__ import annotations
@@ -210,3 +213,55 @@ def test_committed_model_beats_its_noisy_input_on_a_held_out_phantom(dn):
     assert (den < noisy).all(), (noisy, den)
     assert np.median(den) < np.median(noisy) / 3.0
     assert np.median(den) < 0.03
+
+
+def _ct_plan(n_beamlets: int = 10):
+    from pyradmc_amd import HipEngine, SpectralBeamletSource, workloads
+    from pyradmc_amd.data.analytic import AnalyticCrossSections
+
+    grid = workloads.ct_like_grid(seed=HELD_OUT_SEED + 1)
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    full = workloads.ct_like_beamlets()
+    pick = np.linspace(0, full.n_beamlets - 1, n_beamlets).round().astype(int).tolist()
+    src = SpectralBeamletSource(full.spectrum, full.focal_point, [full.centers[j] for j in pick],
+                                full.width_u, full.width_v)
+    return HipEngine(grid, xs), src
+
+
+def test_plain_monte_carlo_judge_accepts_the_trained_network(dn):
+    # On a phantom no training run used, the audit (plain Monte Carlo on 3 of 10 columns
+    # at 250,000 histories) estimates the trained network's error below the 2% budget:
+    # the audited columns come back as plain Monte Carlo, the others denoised.
+    eng, src = _ct_plan()
+    model = dn.load(MODEL)
+    builder = dn.DenoisedDijBuilder(eng, model, low_histories=20_000, tolerance=0.02,
+                                    fallback_histories=250_000, n_batches=10, audit_fraction=0.3)
+    dij, report = builder.build(src, seed=5, truncation=1e-4)
+    eng.close()
+    print(f"audit error {report.audit_error:.4f}: {np.round(report.estimated_error[report.audited], 4).tolist()}")
+    assert report.audited.sum() == 3 and np.array_equal(report.fallback, report.audited)
+    assert 0.0 < report.audit_error < 0.02
+    assert dij.indptr[-1] > 0 and np.all(np.diff(dij.indptr) > 0)
+
+
+def test_plain_monte_carlo_judge_rejects_a_biased_network(dn):
+    # The same network with a bias added to its prediction (+5% of each column's scale
+    # everywhere): two denoised runs of it would agree with each other, but the plain
+    # Monte Carlo audit sees the bias, and every column is re-simulated; the result is
+    # the plain Dij bit for bit.
+    eng, src = _ct_plan()
+    model = dn.load(MODEL)
+    head = model.layers["head"]
+    head.b[0] += 0.05
+    model._prepare(head)
+    builder = dn.DenoisedDijBuilder(eng, model, low_histories=20_000, tolerance=0.02,
+                                    fallback_histories=250_000, n_batches=10, audit_fraction=0.3)
+    dij, report = builder.build(src, seed=5, truncation=1e-4)
+    plain = eng.run_dij(src, 250_000, 10, seed=7, truncation=1e-4)
+    eng.close()
+    print(f"audit error {report.audit_error:.4f}")
+    assert report.audit_error > 0.02 and report.fallback.all()
+    np.testing.assert_array_equal(dij.indptr, plain.indptr)
+    np.testing.assert_array_equal(dij.indices, plain.indices)
+    np.testing.assert_array_equal(dij.dose, plain.dose)
+    np.testing.assert_array_equal(dij.sigma, plain.sigma)

@@ -212,3 +212,33 @@ def test_presampled_transport_cuts_items_into_aligned_blocks_and_bounded_calls(m
     # each call uploads the samples of exactly its pieces, in order
     for call, uploaded in zip(fake.transports, fake.uploads):
         assert uploaded == [(k, n, b) for k, n, _, b in call]
+
+
+def test_audit_window_entries_are_the_dense_column_in_the_window():
+    # The denoised builder's judge reads a plain column's dose and sigma inside a
+    # window straight from its CSC entries; scattered back they give the dense
+    # column's window.
+    from pyradmc_amd.denoise import _window_entries
+    from pyradmc_amd.scoring import DijResult
+
+    rng = np.random.default_rng(1)
+    shape = (10, 12, 7)
+    n_vox = int(np.prod(shape))
+    indptr, idx, dose, sig = [0], [], [], []
+    for _ in range(3):
+        k = np.sort(rng.choice(n_vox, 200, replace=False))
+        idx.append(k)
+        dose.append(rng.random(200))
+        sig.append(rng.random(200))
+        indptr.append(indptr[-1] + 200)
+    d = DijResult(grid_shape=shape, n_beamlets=3, n_histories_per_beamlet=10, n_batches=2, truncation=0.0,
+                  indptr=np.array(indptr), indices=np.concatenate(idx), dose=np.concatenate(dose),
+                  sigma=np.concatenate(sig), energy_emitted=1.0, energy_deposited=1.0, energy_escaped=0.0,
+                  energy_unscored=0.0, correlated=True, scoring_mode="dose_to_medium", provenance=None)
+    window = (slice(2, 8), slice(3, 10))
+    for c in range(3):
+        x, y, z, m, s = _window_entries(d, c, window, shape)
+        got_dose, got_sigma = np.zeros((6, 7, 7)), np.zeros((6, 7, 7))
+        got_dose[x, y, z], got_sigma[x, y, z] = m, s
+        np.testing.assert_array_equal(got_dose, d.column_dense(c)[window[0], window[1], :])
+        np.testing.assert_array_equal(got_sigma, d.sigma_dense(c)[window[0], window[1], :])
