@@ -50,7 +50,8 @@ enum {
   HMC_SRC_PARALLEL_SPECTRAL = 2, /* parallel field with spectral energies (extension) */
   HMC_SRC_SPECTRAL_FAN = 3,    /* SpectralBeamSource / SpectralBeamletSource */
   HMC_SRC_LATTICE = 4,         /* BeamletGridSource */
-  HMC_SRC_UPLOADED = 5         /* host-sampled primaries */
+  HMC_SRC_UPLOADED = 5,        /* host-sampled primaries */
+  HMC_SRC_RESIDENT = 6         /* records drawn in-kernel from a device-resident population */
 };
 
 typedef struct hmc_source {
@@ -128,6 +129,20 @@ int hmc_set_source(hmc_ctx *ctx, const hmc_source *source);
 int hmc_set_uploaded(hmc_ctx *ctx, int64_t n, const int32_t *particle, const float *energy,
                      const float *x, const float *y, const float *z, const float *ux,
                      const float *uy, const float *uz, const float *weight);
+/* A phase-space population kept in device memory for HMC_SRC_RESIDENT sources (an
+ * extension: the reference samples a file's records on the host for every chunk,
+ * pyradmc/backends/warp/engine.py:2506-2597, and keeps only its own head pre-solve
+ * output on the device, pyradmc/backends/warp/presolve.py:710-742). Writes rows
+ * [offset, offset + count) of a population of n records, in order: offset 0 starts a
+ * new population of n (and unsets a resident source until hmc_set_source is called
+ * again), each later call continues where the previous one ended, and a resident
+ * source can be set once all n rows are written. A history draws row floor(U * n), U
+ * from 64 bits of its own stream. Columns as hmc_set_uploaded (kinds 0 photon,
+ * 1 electron, 2 positron). */
+int hmc_set_population(hmc_ctx *ctx, int64_t n, int64_t offset, int64_t count,
+                       const int32_t *particle, const float *energy, const float *x,
+                       const float *y, const float *z, const float *ux, const float *uy,
+                       const float *uz, const float *weight);
 int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
 /* Time every transport launch with HIP events on the context's stream (no
  * reference counterpart: the measurement hook for the per-kernel roofline). */

@@ -60,6 +60,7 @@ class OmConfig(Structure):
         ("pcut", c_double), ("ecut", c_double), ("step_fraction", c_double),
         ("resolution", c_double), ("transport_electrons", c_int), ("msc_gs", c_int),
         ("srest", _dp), ("dose_to_water", c_int),
+        ("n_prim", c_int64), ("prim_kind", POINTER(c_int32)), ("prim", _dp),
     ]
 
 
@@ -180,7 +181,7 @@ class Oracle:
             c.gs = _p(values)
             c.gs_ix0, c.gs_iy0 = int(ix0), int(iy0)
             c.gs_neta, c.gs_nth, c.gs_nu = (int(v) for v in values.shape)
-        kinds = {"pencil": 0, "parallel": 1, "parallel_spectral": 2, "fan": 3, "lattice": 4}
+        kinds = {"pencil": 0, "parallel": 1, "parallel_spectral": 2, "fan": 3, "lattice": 4, "given": 5}
         c.src_kind = kinds[source["kind"]]
         c.particle = {"photon": 0, "electron": 1}[source.get("particle", "photon")]
         c.energy = float(source.get("energy", 0.0))
@@ -197,6 +198,19 @@ class Oracle:
             edges = self._hold(np.ascontiguousarray(source["edges"], dtype=np.float64))
             cdf = self._hold(np.ascontiguousarray(source["cdf"], dtype=np.float64))
             c.n_bins, c.edges, c.cdf = int(cdf.size), _p(edges), _p(cdf)
+        if source["kind"] == "given":
+            # source["primaries"]: sample_batch-style columns indexed by history key
+            # (IAEA codes 1/2/3 -> photon, electron, positron; energy, x, y, z, ux, uy,
+            # uz, weight).
+            given = source["primaries"]
+            code = np.asarray(given["particle_type"]).reshape(-1)
+            pk = self._hold(np.ascontiguousarray(code - 1, dtype=np.int32))
+            if np.any((pk < 0) | (pk > 2)):
+                raise ValueError("given primaries must be photons, electrons or positrons (codes 1-3)")
+            cols = np.stack([np.asarray(given[k], dtype=np.float64).reshape(-1)
+                             for k in ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")])
+            prim = self._hold(np.ascontiguousarray(cols))
+            c.n_prim, c.prim_kind, c.prim = int(code.size), _p(pk, c_int32), _p(prim)
         if "beamlets" in source:
             b = self._hold(np.ascontiguousarray(source["beamlets"], dtype=np.float64))
             c.n_beamlets, c.beamlet = int(b.shape[0]), _p(b)
@@ -221,8 +235,10 @@ class Oracle:
         books = OmBooks()
         f = np.ascontiguousarray(first, dtype=np.int64)
         k = np.ascontiguousarray(count, dtype=np.int64)
-        lib().om_batch(ctypes.byref(self.cfg), c_uint64(seed), n_cols, _p(f, c_int64),
-                       _p(k, c_int64), int(threads), _p(energy), ctypes.byref(books))
+        rc = lib().om_batch(ctypes.byref(self.cfg), c_uint64(seed), n_cols, _p(f, c_int64),
+                            _p(k, c_int64), int(threads), _p(energy), ctypes.byref(books))
+        if rc != 0:
+            raise ValueError("history keys beyond the given primaries")
         return energy.reshape(n_cols, self.n_vox), books
 
     @staticmethod

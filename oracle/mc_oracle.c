@@ -11,6 +11,9 @@
  *   table lookups       pyradmc/data/tables.py:83-133 (the kernels' data access,
  *                       pyradmc/backends/warp/kernels.py:1232-1260, 1646-1776)
  *   batched scoring     pyradmc/scoring/dose.py:127-207; Dij: ref/engine.py:296-463
+ *   given primaries     a phase space's sampled records, one per history key, each with
+ *                       its own kind and weight (pyradmc/geometry/phasespace.py:547-560,
+ *                       pyradmc/backends/warp/engine.py:2506-2597)
  * One sequential random stream per history serves its whole particle family, as in
  * ReferenceEngine (pyradmc/backends/ref/engine.py:241-278); the stream itself is
  * this file's own SplitMix64 (the reference's PCG64 streams are not reproduced).
@@ -46,7 +49,7 @@ typedef struct {
   const double *gs;
   int gs_ix0, gs_iy0, gs_neta, gs_nth, gs_nu;
   /* source */
-  int src_kind; /* 0 pencil, 1 parallel, 2 parallel spectral, 3 fan, 4 lattice */
+  int src_kind; /* 0 pencil, 1 parallel, 2 parallel spectral, 3 fan, 4 lattice, 5 given */
   int particle; /* 0 photon, 1 electron */
   double energy;
   double pos[3], dir[3];
@@ -62,6 +65,11 @@ typedef struct {
   /* dose to water: restricted stopping rows and the switch (scoring/dose_to_water.py) */
   const double *srest;
   int dose_to_water;
+  /* given primaries (src_kind 5), indexed by history key: kind 0 photon, 1 electron,
+   * 2 positron; prim [8][n_prim] = energy, x, y, z, ux, uy, uz, weight */
+  long long n_prim;
+  const int32_t *prim_kind;
+  const double *prim;
 } om_config;
 
 typedef struct {
@@ -468,7 +476,8 @@ static double spectrum_energy(const om_config *c, rng_t *r) {
 }
 
 /* Emit one primary for (beamlet) and transport its family; returns emitted energy. */
-static double history(const om_config *c, tally_t *t, stack_t *s, int beamlet, rng_t *r) {
+static double history(const om_config *c, tally_t *t, stack_t *s, int beamlet, long long key,
+                      rng_t *r) {
   particle_t q; double emitted;
   q.w = 1.0; q.kind = c->particle;
   memset(q.u, 0, sizeof(q.u)); q.u[2] = 1.0;
@@ -477,6 +486,10 @@ static double history(const om_config *c, tally_t *t, stack_t *s, int beamlet, r
     case 1: q.e = c->energy; q.p[0] = c->x0 + c->xe * uniform(r); q.p[1] = c->y0 + c->ye * uniform(r); q.p[2] = c->z0; break;
     case 2: q.kind = 0; q.e = spectrum_energy(c, r); q.p[0] = c->x0 + c->xe * uniform(r);
             q.p[1] = c->y0 + c->ye * uniform(r); q.p[2] = c->z0; break;
+    case 5: { const size_t n = (size_t)c->n_prim, k = (size_t)key;
+              q.kind = c->prim_kind[k]; q.e = c->prim[k];
+              for (int a = 0; a < 3; ++a) { q.p[a] = c->prim[(1 + a) * n + k]; q.u[a] = c->prim[(4 + a) * n + k]; }
+              q.w = c->prim[7 * n + k]; break; }
     case 4: { const double *b = c->beamlet + 4 * (size_t)beamlet;
               q.kind = 0; q.e = c->energy; q.p[0] = b[0] + b[1] * uniform(r); q.p[1] = b[2] + b[3] * uniform(r);
               q.p[2] = c->z0; break; }
@@ -523,7 +536,7 @@ static void *worker(void *arg) {
     rng_t r;
     rng_seed(&r, w->seed, (uint64_t)(w->first[j] + i));
     w->t.base = (size_t)j * (size_t)w->c->snx * w->c->sny * w->c->snz;
-    w->t.b.emitted += history(w->c, &w->t, &w->s, j, &r);
+    w->t.b.emitted += history(w->c, &w->t, &w->s, j, w->first[j] + i, &r);
   }
   return NULL;
 }
@@ -534,6 +547,9 @@ int om_batch(const om_config *c, uint64_t seed, int n_cols, const long long *fir
              const long long *count, int threads, double *batch_energy, om_books *books) {
   if (threads < 1) threads = 1;
   if (threads > 256) threads = 256;
+  if (c->src_kind == 5)
+    for (int j = 0; j < n_cols; ++j)
+      if (first[j] < 0 || count[j] < 0 || first[j] + count[j] > c->n_prim) return -1;
   size_t nv = (size_t)c->snx * c->sny * c->snz, cells = nv * (size_t)n_cols;
   long long *start = (long long *)malloc(sizeof(long long) * (size_t)(n_cols + 1));
   start[0] = 0;

@@ -29,6 +29,13 @@ from pyradmc_amd.geometry import (
     VoxelGrid,
     bremsstrahlung_6mv_like,
 )
+from pyradmc_amd.phasespace import (
+    IAEAPhaseSpace,
+    InMemoryPhaseSpaceSource,
+    PhaseSpaceSource,
+    read_iaea_header,
+    write_iaea_phase_space,
+)
 from pyradmc_amd.progress import ProgressEvent
 from pyradmc_amd.scoring import DijResult, RunProvenance, ScoringGrid, TransportResult
 
@@ -57,9 +64,12 @@ __all__ = [
     "ELECTRON_MASS_MEV",
     "GY_PER_MEV_PER_G",
     "HipEngine",
+    "IAEAPhaseSpace",
+    "InMemoryPhaseSpaceSource",
     "MATERIALS",
     "PCUT_MEV",
     "ParallelBeamSource",
+    "PhaseSpaceSource",
     "PencilBeamSource",
     "ProgressEvent",
     "RunProvenance",
@@ -74,4 +84,6 @@ __all__ = [
     "VoxelGrid",
     "WATER",
     "bremsstrahlung_6mv_like",
+    "read_iaea_header",
+    "write_iaea_phase_space",
 ]

@@ -38,6 +38,7 @@ SRC_PARALLEL_SPECTRAL = 2
 SRC_SPECTRAL_FAN = 3
 SRC_LATTICE = 4
 SRC_UPLOADED = 5
+SRC_RESIDENT = 6
 
 _f32p = POINTER(ctypes.c_float)
 _f64p = POINTER(c_double)
@@ -174,6 +175,7 @@ def lib():
         "hmc_set_gs": (c_int, [ctx_p, _f32p, c_int, c_int, c_int, c_int, c_int]),
         "hmc_set_source": (c_int, [ctx_p, POINTER(SourceDesc)]),
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
+        "hmc_set_population": (c_int, [ctx_p, c_int64, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
         "hmc_set_profiling": (c_int, [ctx_p, c_int]),
         "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int, c_int]),

@@ -948,8 +948,14 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
           book = true;
           break;
         }
-        default: {  // host-sampled primaries, indexed by the flat primary index
-          const long long j = g;
+        default: {  // host-sampled primaries by flat primary index, or a record of the
+                    // resident population drawn with 64 bits of the history's stream
+          long long j = g;
+          if (S.kind == HMC_SRC_RESIDENT) {
+            const uint64_t hi = rng_next(rng);
+            const uint64_t u = (hi << 32) | rng_next(rng);
+            j = static_cast<long long>(__umul64hi(u, static_cast<unsigned long long>(S.up_n)));
+          }
           kind = static_cast<uint32_t>(S.up_particle[j]);
           e = S.up[j];
           x = S.up[S.up_n + j];
@@ -973,7 +979,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
       p.meta = pack_meta(kind, 1u, column);
       p.pad = 0u;
     }
-    if (S.kind != HMC_SRC_UPLOADED) {
+    if (S.kind < HMC_SRC_UPLOADED) {
       // One kind per launch: fixed slots, no atomics.
       const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
       if (live && t < q.capacity) q.data[t] = p;
@@ -1008,7 +1014,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
     }
     emitted_sum += emitted;
   }
-  if (S.kind != HMC_SRC_UPLOADED && blockIdx.x == 0 && threadIdx.x == 0) {
+  if (S.kind < HMC_SRC_UPLOADED && blockIdx.x == 0 && threadIdx.x == 0) {
     const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
     *q.count = static_cast<unsigned int>(n_gen);
   }
@@ -1340,6 +1346,10 @@ struct hmc_ctx {
   float* d_beamlet = nullptr;
   int32_t* d_up_particle = nullptr;
   float* d_up = nullptr;
+  // resident population: [n] kinds and [8][n] columns, rows [0, pop_written) written
+  int32_t* d_pop_particle = nullptr;
+  float* d_pop = nullptr;
+  long long pop_n = 0, pop_written = 0;
   int src_beamlets = 0;
   bool has_source = false;
   // What d_edges, d_cdf and d_beamlet hold: a source set again with the same arrays
@@ -1565,7 +1575,8 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_q[3],         ctx->d_counts,      ctx->d_ledger,     ctx->d_items,
                   ctx->d_colmax,       ctx->d_chunk_counts, ctx->d_chunk_offsets,
                   ctx->d_out_idx,      ctx->d_out_dose,    ctx->d_out_sigma,  ctx->d_origins,
-                  ctx->d_out32,        ctx->d_samples,     ctx->d_entry_idx,  ctx->d_entry_val};
+                  ctx->d_out32,        ctx->d_samples,     ctx->d_entry_idx,  ctx->d_entry_val,
+                  ctx->d_pop_particle, ctx->d_pop};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
@@ -1791,8 +1802,10 @@ int hmc_set_gs(hmc_ctx* ctx, const float* values, int n_eta, int n_theta2, int n
 
 int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
   if (!ctx || !s) return fail(ctx, HMC_EINVAL, "null");
-  if (s->kind < HMC_SRC_PENCIL || s->kind > HMC_SRC_UPLOADED)
+  if (s->kind < HMC_SRC_PENCIL || s->kind > HMC_SRC_RESIDENT)
     return fail(ctx, HMC_EINVAL, "unknown source kind");
+  if (s->kind == HMC_SRC_RESIDENT && (ctx->pop_n < 1 || ctx->pop_written != ctx->pop_n))
+    return fail(ctx, HMC_EINVAL, "resident source without a complete population");
   if (s->particle < 0 || s->particle > 2) return fail(ctx, HMC_EINVAL, "unknown particle kind");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   DevSource& d = ctx->src;
@@ -1854,7 +1867,43 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
     ctx->src_beamlets = s->n_beamlets;
   }
   if (s->kind == HMC_SRC_UPLOADED) d.up_n = 0;  // primaries must be uploaded for this source
+  if (s->kind == HMC_SRC_RESIDENT) {
+    d.up_particle = ctx->d_pop_particle;
+    d.up = ctx->d_pop;
+    d.up_n = ctx->pop_n;
+  }
   ctx->has_source = true;
+  return HMC_OK;
+}
+
+int hmc_set_population(hmc_ctx* ctx, int64_t n, int64_t offset, int64_t count, const int32_t* particle,
+                       const float* energy, const float* x, const float* y, const float* z,
+                       const float* ux, const float* uy, const float* uz, const float* weight) {
+  if (!ctx || !particle || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
+    return fail(ctx, HMC_EINVAL, "population columns incomplete");
+  if (n < 1 || count < 1 || offset < 0 || offset > n - count)
+    return fail(ctx, HMC_EINVAL, "population rows out of range");
+  if (offset != (offset == 0 ? 0 : ctx->pop_written) || (offset > 0 && n != ctx->pop_n))
+    return fail(ctx, HMC_EINVAL, "population rows must be written in order");
+  for (int64_t i = 0; i < count; ++i)
+    if (particle[i] < 0 || particle[i] > 2) return fail(ctx, HMC_EINVAL, "bad particle kind");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no generator may be reading the rows
+  int rc;
+  if (offset == 0) {
+    ctx->pop_n = ctx->pop_written = 0;
+    if (ctx->src.kind == HMC_SRC_RESIDENT) ctx->src.up_n = 0;
+    if ((rc = dev_alloc(ctx, &ctx->d_pop_particle, static_cast<size_t>(n)))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_pop, static_cast<size_t>(n) * 8))) return rc;
+    ctx->pop_n = n;
+  }
+  HMC_HIP(ctx, hipMemcpy(ctx->d_pop_particle + offset, particle, sizeof(int32_t) * count,
+                         hipMemcpyHostToDevice));
+  const float* cols[8] = {energy, x, y, z, ux, uy, uz, weight};
+  for (int c = 0; c < 8; ++c)
+    HMC_HIP(ctx, hipMemcpy(ctx->d_pop + c * n + offset, cols[c], sizeof(float) * count,
+                           hipMemcpyHostToDevice));
+  ctx->pop_written = offset + count;
   return HMC_OK;
 }
 
@@ -1863,6 +1912,8 @@ int hmc_set_uploaded(hmc_ctx* ctx, int64_t n, const int32_t* particle, const flo
                      const float* uy, const float* uz, const float* weight) {
   if (!ctx || n < 1 || !particle || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
     return fail(ctx, HMC_EINVAL, "uploaded primaries incomplete");
+  if (ctx->src.kind != HMC_SRC_UPLOADED)
+    return fail(ctx, HMC_EINVAL, "primaries uploaded for a source that is not host-sampled");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   for (int64_t i = 0; i < n; ++i)
     if (particle[i] < 0 || particle[i] > 2) return fail(ctx, HMC_EINVAL, "bad particle kind");
@@ -2047,6 +2098,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
   }
   if (ctx->src.kind == HMC_SRC_UPLOADED && total > ctx->src.up_n)
     return fail(ctx, HMC_EINVAL, "more histories than uploaded primaries");
+  if (ctx->src.kind == HMC_SRC_RESIDENT && ctx->src.up_n < 1)
+    return fail(ctx, HMC_EINVAL, "resident source not set since its population was replaced");
   ctx->dirty_slots = std::max(ctx->dirty_slots, slots_written);
   int rc;
   if (static_cast<size_t>(n_items) > ctx->items_cap) {

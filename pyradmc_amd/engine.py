@@ -87,9 +87,9 @@ def _f32(a) -> np.ndarray:
 
 _BUILTIN_SOURCES = frozenset({
     "PencilBeamSource", "ParallelBeamSource", "SpectralParallelBeamSource",
-    "SpectralBeamSource", "BeamletGridSource", "SpectralBeamletSource",
+    "SpectralBeamSource", "BeamletGridSource", "SpectralBeamletSource", "DevicePhaseSpace",
 })
-_BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source"})
+_BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source", "pyradmc_amd.phasespace"})
 _UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
 
 
@@ -122,6 +122,18 @@ def _in_kernel(source) -> bool:
     included (it may override emit or sample_batch), is host pre-sampled."""
     t = type(source)
     return t.__name__ in _BUILTIN_SOURCES and t.__module__ in _BUILTIN_MODULES
+
+
+def _iaea_kinds(code) -> np.ndarray:
+    """IAEA particle codes (1 photon, 2 electron, 3 positron) as the kernels' kinds."""
+    code = np.asarray(code).reshape(-1)
+    kind = np.full(code.shape, -1, dtype=np.int32)
+    for iaea, k in ((1, 0), (2, 1), (3, 2)):
+        kind[code == iaea] = k
+    if np.any(kind < 0):
+        bad = sorted(set(np.asarray(code[kind < 0]).tolist()))[:5]
+        raise ValueError(f"sampled particle_type codes {bad} are not photon (1), electron (2) or positron (3)")
+    return kind
 
 
 def _type_name(obj) -> str:
@@ -171,6 +183,7 @@ class HipEngine:
         self._ctx = nat.Context(self._index)
         self._info = nat.device_info(self._index)
         self.launch = {"round_lanes_per_cu": 2048, "refill_block": 64, "push_chunk": 64, "use_graphs": True}
+        self._population_token = None
 
     def close(self) -> None:
         if self._ctx is not None:
@@ -389,6 +402,10 @@ class HipEngine:
             keep.append(params)
             d.n_beamlets = int(params.shape[0])
             d.beamlet_params = nat.ptr(params, nat.c_double)
+        elif name == "DevicePhaseSpace":
+            self._upload_population(source)
+            d.kind = nat.SRC_RESIDENT
+            d.particle = 0
         elif name == "BeamletGridSource":
             d.kind = nat.SRC_LATTICE
             d.particle = 0
@@ -413,16 +430,30 @@ class HipEngine:
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
         return analytic, energy
 
+    def _upload_population(self, source) -> None:
+        """Copy a DevicePhaseSpace's records to the device, once per population: the
+        context keeps the last population written, so running the same one again (a
+        loop over seeds) uploads nothing."""
+        if self._population_token == source.token:
+            return
+        self._population_token = None
+        n, offset = len(source), 0
+        for cols in source.population_blocks():
+            kind = _iaea_kinds(cols["particle_type"])
+            data = [np.ascontiguousarray(np.asarray(cols[c], dtype=np.float32).reshape(-1)) for c in _UPLOAD_COLUMNS]
+            if any(c.shape != kind.shape for c in data) or offset + kind.size > n:
+                raise ValueError("phase-space population blocks differ from its length")
+            self._ctx.call("hmc_set_population", int(n), int(offset), int(kind.size), nat.ptr(kind, nat.c_int32),
+                           *[nat.ptr(c, nat.ctypes.c_float) for c in data])
+            offset += kind.size
+        if offset != n:
+            raise ValueError(f"phase-space population gave {offset} records, expected {n}")
+        self._population_token = source.token
+
     def _upload_primaries(self, samples: list[dict]) -> None:
         """Upload host-sampled primaries (``sample_batch`` columns, IAEA particle codes
         1 photon, 2 electron, 3 positron), concatenated in work-item order."""
-        code = np.concatenate([np.asarray(b["particle_type"]).reshape(-1) for b in samples])
-        kind = np.full(code.shape, -1, dtype=np.int32)
-        for iaea, k in ((1, 0), (2, 1), (3, 2)):
-            kind[code == iaea] = k
-        if np.any(kind < 0):
-            bad = sorted(set(np.asarray(code[kind < 0]).tolist()))[:5]
-            raise ValueError(f"sampled particle_type codes {bad} are not photon (1), electron (2) or positron (3)")
+        kind = _iaea_kinds(np.concatenate([np.asarray(b["particle_type"]).reshape(-1) for b in samples]))
         cols = [np.ascontiguousarray(np.concatenate(
             [np.asarray(b[c], dtype=np.float32).reshape(-1) for b in samples])) for c in _UPLOAD_COLUMNS]
         if any(c.shape != kind.shape for c in cols):
@@ -562,6 +593,8 @@ class HipEngine:
         self._set_physics(pcut, ecut, step, resolution, transport_electrons, msc_gs, d2w)
         analytic, mono_energy = self._set_source(source, primary_kind)
         presample = not _in_kernel(source)
+        if _type_name(source) == "DevicePhaseSpace":
+            source.note_histories(n_histories)
         slots = self._batch_slots(n_batches, 1, scoring.n_voxels)
         self._set_scoring(scoring, 1, slots)
 
@@ -688,6 +721,10 @@ class HipEngine:
         quanta) and, as floats, the host wall split of this call: ``transport_s``
         (transport, fold and finalize), ``assemble_s`` (truncation, download, join) and
         ``groups`` (beamlet groups run)."""
+        if not hasattr(source, "n_beamlets"):
+            raise TypeError(
+                f"run_dij needs a beamlet source; {_type_name(source)} has no beamlets "
+                "(phase-space and other whole-field sources go through run)")
         value_dtype = np.dtype(value_dtype)
         if value_dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
             raise ValueError(f"value_dtype must be float64 or float32, got {value_dtype}")
