@@ -69,6 +69,36 @@ typedef struct hmc_source {
   const double *beamlet_params; /* lattice: [n][4] x_lo, x_ext, y_lo, y_ext; fan: [n][3] centre */
 } hmc_source;
 
+/* A deterministic per-primary weight factor applied by the generator to in-kernel
+ * sources (and resident phase spaces): the transmission of a beam-limiting stack,
+ * exp(-sum_i rho_i mu_i(E) t_i), or a 0..1 mask read where the ray crosses a plane
+ * rectangle. It replaces the host-side attenuation of sampled batches
+ * (pyradmc/geometry/collimation.py:1071-1151) for sources generated on the device;
+ * geometry, tables and mask conventions as pyradmc/geometry/collimation.py:280-489,
+ * 927-1068. */
+typedef struct hmc_ray_weights {
+  int kind;                     /* 0 none, 1 beam-limiting stack, 2 plane mask */
+  int n_devices;                /* stack: devices in order */
+  double origin[3];             /* focal spot (engine frame) */
+  double basis[9];              /* rows u, v, w of the beam frame (engine frame) */
+  const int32_t *device_kind;   /* [n_devices] 0 jaw pair, 1 MLC */
+  const int32_t *flags;         /* jaw: axis (0 u, 1 v) | focused << 1; MLC: focused sides << 1 */
+  const int32_t *n_pairs;       /* MLC leaf pairs */
+  const int32_t *edge_offset;   /* MLC: first strip edge in leaf_edges */
+  const int32_t *pair_offset;   /* MLC: first pair in tips_neg / tips_pos */
+  const double *z_top, *z_bottom;
+  const double *edge_neg, *edge_pos; /* jaw edges at the mid-plane; MLC tip radius in edge_neg */
+  const double *density;        /* g/cm^3 */
+  int n_leaf_edges, n_tips;
+  const double *leaf_edges, *tips_neg, *tips_pos;
+  int n_table;                  /* log-log total mu/rho per device */
+  const double *log_energies;   /* [n_table] increasing */
+  const double *log_mu;         /* [n_devices][n_table] log(mu/rho), cm^2/g */
+  double center[3], u_axis[3], v_axis[3], width_u, width_v; /* mask plane */
+  int mask_nu, mask_nv;
+  const double *mask;           /* [mask_nu][mask_nv], values in 0..1 */
+} hmc_ray_weights;
+
 typedef struct hmc_physics {
   double pcut, ecut, step_energy_fraction, deposit_resolution_cm;
   int transport_electrons, msc_gs, dose_to_water, reserved;
@@ -144,6 +174,13 @@ int hmc_set_population(hmc_ctx *ctx, int64_t n, int64_t offset, int64_t count,
                        const float *y, const float *z, const float *ux, const float *uy,
                        const float *uz, const float *weight);
 int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
+/* The ray weight model of the following transports (NULL or kind 0: none). */
+int hmc_set_ray_weights(hmc_ctx *ctx, const hmc_ray_weights *weights);
+/* The current model's factor for each of n rays (energy, position [n][3], direction
+ * [n][3], float32 as the generator holds a primary) into out[n] (a probe of the
+ * device evaluation; no reference counterpart). */
+int hmc_ray_weight_factors(hmc_ctx *ctx, int64_t n, const float *energy, const float *position,
+                           const float *direction, double *out);
 /* Time every transport launch with HIP events on the context's stream (no
  * reference counterpart: the measurement hook for the per-kernel roofline). */
 int hmc_set_profiling(hmc_ctx *ctx, int enabled);

@@ -73,6 +73,46 @@ class Tables(Structure):
     ]
 
 
+_i32p = POINTER(c_int32)
+
+
+class RayWeightsDesc(Structure):
+    """hmc_ray_weights: the generator's collimator or mask factor model."""
+
+    _fields_ = [
+        ("kind", c_int),
+        ("n_devices", c_int),
+        ("origin", c_double * 3),
+        ("basis", c_double * 9),
+        ("device_kind", _i32p),
+        ("flags", _i32p),
+        ("n_pairs", _i32p),
+        ("edge_offset", _i32p),
+        ("pair_offset", _i32p),
+        ("z_top", _f64p),
+        ("z_bottom", _f64p),
+        ("edge_neg", _f64p),
+        ("edge_pos", _f64p),
+        ("density", _f64p),
+        ("n_leaf_edges", c_int),
+        ("n_tips", c_int),
+        ("leaf_edges", _f64p),
+        ("tips_neg", _f64p),
+        ("tips_pos", _f64p),
+        ("n_table", c_int),
+        ("log_energies", _f64p),
+        ("log_mu", _f64p),
+        ("center", c_double * 3),
+        ("u_axis", c_double * 3),
+        ("v_axis", c_double * 3),
+        ("width_u", c_double),
+        ("width_v", c_double),
+        ("mask_nu", c_int),
+        ("mask_nv", c_int),
+        ("mask", _f64p),
+    ]
+
+
 class SourceDesc(Structure):
     _fields_ = [
         ("kind", c_int),
@@ -176,6 +216,8 @@ def lib():
         "hmc_set_source": (c_int, [ctx_p, POINTER(SourceDesc)]),
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_population": (c_int, [ctx_p, c_int64, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
+        "hmc_set_ray_weights": (c_int, [ctx_p, POINTER(RayWeightsDesc)]),
+        "hmc_ray_weight_factors": (c_int, [ctx_p, c_int64, _f32p, _f32p, _f32p, _f64p]),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
         "hmc_set_profiling": (c_int, [ctx_p, c_int]),
         "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int, c_int]),

@@ -26,9 +26,11 @@
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hipmc.h"
+#include "hipmc_collimation.h"
 #include "hipmc_device.h"
 
 using namespace hmc;
@@ -873,9 +875,9 @@ __device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
 // reserve slots in the queue of their kind with one atomic per queue per block and
 // pass. The emitted energy is reduced over the block and added with one atomic per
 // block (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
-__global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* items, int n_items,
-                                                  long long p0, long long n_gen, uint64_t seed,
-                                                  Queue qp, Queue qe,
+__global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW, const DevItem* items,
+                                                  int n_items, long long p0, long long n_gen,
+                                                  uint64_t seed, Queue qp, Queue qe,
                                                   unsigned long long* ledger) {
   __shared__ unsigned int wave_count[2][4];
   __shared__ unsigned int block_base[2];
@@ -968,6 +970,11 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, const DevItem* it
           book = true;
           break;
         }
+      }
+      if (RW.kind != 0 && S.kind != HMC_SRC_UPLOADED) {
+        // collimator or mask factor of the primary's ray (no random number drawn)
+        w *= static_cast<float>(ray_weight(RW, e, x, y, z, ux, uy, uz));
+        book = true;
       }
       if (book) {
         const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
@@ -1346,6 +1353,9 @@ struct hmc_ctx {
   float* d_beamlet = nullptr;
   int32_t* d_up_particle = nullptr;
   float* d_up = nullptr;
+  // ray weight model of the generator and the device buffers it points into
+  DevRayWeights rw{};
+  std::vector<void*> rw_buffers;
   // resident population: [n] kinds and [8][n] columns, rows [0, pop_written) written
   int32_t* d_pop_particle = nullptr;
   float* d_pop = nullptr;
@@ -1579,6 +1589,7 @@ void hmc_destroy(hmc_ctx* ctx) {
                   ctx->d_pop_particle, ctx->d_pop};
   for (void* p : ptrs) free_ptr(p);
   for (void* p : ctx->table_buffers) free_ptr(p);
+  for (void* p : ctx->rw_buffers) free_ptr(p);
   for (hipEvent_t e : ctx->ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (ctx->round_graph) (void)hipGraphExecDestroy(ctx->round_graph);
@@ -1733,6 +1744,7 @@ int hmc_set_tables(hmc_ctx* ctx, const hmc_tables* tb) {
     return fail(ctx, HMC_EINVAL, "table sizes out of range (n_points>=2, 1<=n_mat<=256)");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   for (void* p : ctx->table_buffers) free_ptr(p);
+  for (void* p : ctx->rw_buffers) free_ptr(p);
   ctx->table_buffers.clear();
   const size_t rows = static_cast<size_t>(nm) * n;
   std::vector<float4> photon(rows), ea(rows), eb(rows);
@@ -1927,6 +1939,126 @@ int hmc_set_uploaded(hmc_ctx* ctx, int64_t n, const int32_t* particle, const flo
   ctx->src.up = ctx->d_up;
   ctx->src.up_n = n;
   return HMC_OK;
+}
+
+__global__ void k_ray_weights(DevRayWeights RW, long long n, const float* e, const float* pos,
+                              const float* dir, double* out) {
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = ray_weight(RW, e[i], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], dir[3 * i], dir[3 * i + 1],
+                        dir[3 * i + 2]);
+}
+
+int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no generator may be reading the buffers
+  for (void* p : ctx->rw_buffers) free_ptr(p);
+  ctx->rw_buffers.clear();
+  ctx->rw = DevRayWeights{};
+  if (!w || w->kind == 0) return HMC_OK;
+  DevRayWeights r{};
+  int rc;
+  auto put = [&](const auto* host, size_t count, const auto** dst) -> int {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
+    T* d = nullptr;
+    const int code = upload(ctx, &d, host, count);
+    if (d) ctx->rw_buffers.push_back(d);
+    *dst = d;
+    return code;
+  };
+  if (w->kind == 1) {
+    const int n = w->n_devices;
+    if (n < 1 || w->n_table < 2 || !w->device_kind || !w->flags || !w->n_pairs || !w->edge_offset ||
+        !w->pair_offset || !w->z_top || !w->z_bottom || !w->edge_neg || !w->edge_pos || !w->density ||
+        !w->log_energies || !w->log_mu || w->n_leaf_edges < 1 || w->n_tips < 1 || !w->leaf_edges ||
+        !w->tips_neg || !w->tips_pos)
+      return fail(ctx, HMC_EINVAL, "beam-limiting stack incomplete");
+    for (int j = 0; j < n; ++j) {
+      if (w->device_kind[j] != 0 && w->device_kind[j] != 1) return fail(ctx, HMC_EINVAL, "unknown device kind");
+      if (!(w->z_top[j] > 0.0) || !(w->z_bottom[j] > w->z_top[j]))
+        return fail(ctx, HMC_EINVAL, "device slab must lie downstream of the focal spot");
+      if (w->device_kind[j] == 1) {
+        const int np = w->n_pairs[j];
+        if (np < 1 || w->edge_offset[j] < 0 || w->pair_offset[j] < 0 ||
+            w->edge_offset[j] + np + 1 > w->n_leaf_edges || w->pair_offset[j] + np > w->n_tips)
+          return fail(ctx, HMC_EINVAL, "MLC leaf arrays out of range");
+        for (int i = 0; i < np; ++i)
+          if (!(w->leaf_edges[w->edge_offset[j] + i + 1] > w->leaf_edges[w->edge_offset[j] + i]))
+            return fail(ctx, HMC_EINVAL, "MLC strip edges must increase");
+      }
+    }
+    for (int k = 1; k < w->n_table; ++k)
+      if (!(w->log_energies[k] > w->log_energies[k - 1])) return fail(ctx, HMC_EINVAL, "energies must increase");
+    r.n_dev = n;
+    r.n_table = w->n_table;
+    for (int k = 0; k < 3; ++k) r.o[k] = w->origin[k];
+    for (int k = 0; k < 9; ++k) r.b[k] = w->basis[k];
+    if ((rc = put(w->device_kind, n, &r.dkind)) || (rc = put(w->flags, n, &r.flags)) ||
+        (rc = put(w->n_pairs, n, &r.n_pairs)) || (rc = put(w->edge_offset, n, &r.off)) ||
+        (rc = put(w->pair_offset, n, &r.poff)) || (rc = put(w->z_top, n, &r.z_top)) ||
+        (rc = put(w->z_bottom, n, &r.z_bot)) || (rc = put(w->edge_neg, n, &r.e_neg)) ||
+        (rc = put(w->edge_pos, n, &r.e_pos)) || (rc = put(w->density, n, &r.dens)) ||
+        (rc = put(w->leaf_edges, w->n_leaf_edges, &r.edges)) || (rc = put(w->tips_neg, w->n_tips, &r.tn)) ||
+        (rc = put(w->tips_pos, w->n_tips, &r.tp)) || (rc = put(w->log_energies, w->n_table, &r.log_e)) ||
+        (rc = put(w->log_mu, static_cast<size_t>(n) * w->n_table, &r.log_mu)))
+      return rc;
+    r.le0 = w->log_energies[0];
+    r.inv_dle = (w->n_table - 1) / (w->log_energies[w->n_table - 1] - w->log_energies[0]);
+  } else if (w->kind == 2) {
+    if (w->mask_nu < 1 || w->mask_nv < 1 || !w->mask || !(w->width_u > 0.0) || !(w->width_v > 0.0))
+      return fail(ctx, HMC_EINVAL, "mask incomplete");
+    const size_t cells = static_cast<size_t>(w->mask_nu) * w->mask_nv;
+    for (size_t i = 0; i < cells; ++i)
+      if (!(w->mask[i] >= 0.0 && w->mask[i] <= 1.0)) return fail(ctx, HMC_EINVAL, "mask values must lie in 0..1");
+    if ((rc = put(w->mask, cells, &r.mask))) return rc;
+    r.nu = w->mask_nu;
+    r.nv = w->mask_nv;
+    r.wu = w->width_u;
+    r.wv = w->width_v;
+    for (int k = 0; k < 3; ++k) {
+      r.c[k] = w->center[k];
+      r.u[k] = w->u_axis[k];
+      r.v[k] = w->v_axis[k];
+    }
+    r.nrm[0] = r.u[1] * r.v[2] - r.u[2] * r.v[1];
+    r.nrm[1] = r.u[2] * r.v[0] - r.u[0] * r.v[2];
+    r.nrm[2] = r.u[0] * r.v[1] - r.u[1] * r.v[0];
+  } else {
+    return fail(ctx, HMC_EINVAL, "unknown ray weight kind");
+  }
+  r.kind = w->kind;
+  ctx->rw = r;
+  return HMC_OK;
+}
+
+int hmc_ray_weight_factors(hmc_ctx* ctx, int64_t n, const float* energy, const float* position,
+                           const float* direction, double* out) {
+  if (!ctx || n < 1 || !energy || !position || !direction || !out) return fail(ctx, HMC_EINVAL, "null or empty");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  float *d_e = nullptr, *d_p = nullptr, *d_d = nullptr;
+  double* d_out = nullptr;
+  int rc = HMC_OK;
+  if ((rc = upload(ctx, &d_e, energy, static_cast<size_t>(n))) ||
+      (rc = upload(ctx, &d_p, position, static_cast<size_t>(n) * 3)) ||
+      (rc = upload(ctx, &d_d, direction, static_cast<size_t>(n) * 3)) ||
+      (rc = dev_alloc(ctx, &d_out, static_cast<size_t>(n)))) {
+  } else {
+    hipLaunchKernelGGL(k_ray_weights, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, ctx->rw,
+                       static_cast<long long>(n), d_e, d_p, d_d, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = fail(ctx, HMC_EHIP, hipGetErrorString(e));
+  }
+  for (void* p : {static_cast<void*>(d_e), static_cast<void*>(d_p), static_cast<void*>(d_d),
+                  static_cast<void*>(d_out)}) {
+    if (p) {
+      ctx->alloc_bytes.erase(p);
+      free_ptr(p);
+    }
+  }
+  return rc;
 }
 
 int hmc_set_physics(hmc_ctx* ctx, const hmc_physics* p) {
@@ -2167,7 +2299,7 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     const int pa = 0, ea = 1;  // queue roles at the start of every group of rounds
     Queue qpa{ctx->d_q[pa], counts + pa, cap}, qea{ctx->d_q[ea], counts + ea, cap};
     hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
-                       ctx->stream, ctx->src, ctx->d_items, n_items, p0, n_gen,
+                       ctx->stream, ctx->src, ctx->rw, ctx->d_items, n_items, p0, n_gen,
                        static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());
     while (true) {

@@ -116,10 +116,29 @@ def _joined(parts: list, dtype) -> np.ndarray:
     return np.concatenate(parts, out=host_array(sum(len(p) for p in parts), parts[0].dtype))
 
 
+_WEIGHTED_SOURCES = frozenset({
+    "CollimatedSource", "CollimatedBeamletSource", "TransmissionMaskSource", "TransmissionMaskBeamletSource",
+})
+
+
+def _weighted_parts(source) -> tuple:
+    """(inner source, ray weight model) of this package's collimated and mask-shaped
+    sources (exact types), else (source, None)."""
+    t = type(source)
+    if t.__name__ in _WEIGHTED_SOURCES and t.__module__ == "pyradmc_amd.collimation":
+        return source.inner, source.model
+    return source, None
+
+
 def _in_kernel(source) -> bool:
     """Exact-type routing (pyradmc/backends/warp/engine.py:523-531): only the built-in
     source types themselves get the in-kernel generator; any other source, a subclass
-    included (it may override emit or sample_batch), is host pre-sampled."""
+    included (it may override emit or sample_batch), is host pre-sampled. A collimated
+    or mask-shaped source around a built-in source (not around another wrapper) stays
+    in-kernel: the generator applies the ray factor on the device."""
+    inner, model = _weighted_parts(source)
+    if model is not None:
+        return _weighted_parts(inner)[1] is None and _in_kernel(inner)
     t = type(source)
     return t.__name__ in _BUILTIN_SOURCES and t.__module__ in _BUILTIN_MODULES
 
@@ -184,6 +203,7 @@ class HipEngine:
         self._info = nat.device_info(self._index)
         self.launch = {"round_lanes_per_cu": 2048, "refill_block": 64, "push_chunk": 64, "use_graphs": True}
         self._population_token = None
+        self._ray_model = None
 
     def close(self) -> None:
         if self._ctx is not None:
@@ -340,6 +360,12 @@ class HipEngine:
         """Describe the source to the generator; returns (analytic_emitted, energy).
         Sources without an in-kernel generator are set up for host pre-sampling: their
         primaries are uploaded per transport call (``_upload_primaries``)."""
+        inner, model = _weighted_parts(source)
+        if model is not None and _in_kernel(source):
+            self._set_source(inner, primary_kind, beamlets)
+            self._set_ray_weights(model)
+            return False, 0.0  # emitted energy is booked per weighted primary on the device
+        self._set_ray_weights(None)
         d = nat.SourceDesc()
         keep = []
         name = _type_name(source)
@@ -429,6 +455,57 @@ class HipEngine:
             )
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
         return analytic, energy
+
+    def _set_ray_weights(self, model) -> None:
+        """Hand the generator a collimator (StackTransmission) or mask
+        (MaskTransmission) factor model, or none; a model already set is not sent
+        again."""
+        if model is self._ray_model:
+            return
+        self._ray_model = None
+        if model is None:
+            self._ctx.call("hmc_set_ray_weights", None)
+            return
+        d = nat.RayWeightsDesc()
+        keep = []
+
+        def arr(a, dtype, ctype):
+            a = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1))
+            keep.append(a)
+            return nat.ptr(a, ctype)
+
+        name = type(model).__name__
+        if name == "StackTransmission":
+            cs = model.stack.compile()
+            d.kind = 1
+            d.n_devices = int(cs.kind.size)
+            d.origin[:] = [float(c) for c in cs.origin]
+            d.basis[:] = [float(c) for c in cs.basis.reshape(-1)]
+            for field_name, values in (("device_kind", cs.kind), ("flags", cs.flags), ("n_pairs", cs.n_pairs),
+                                       ("edge_offset", cs.offset), ("pair_offset", cs.pair_offset)):
+                setattr(d, field_name, arr(values, np.int32, nat.c_int32))
+            for field_name, values in (("z_top", cs.z_top), ("z_bottom", cs.z_bottom), ("edge_neg", cs.edge_neg),
+                                       ("edge_pos", cs.edge_pos), ("density", model.densities)):
+                setattr(d, field_name, arr(values, np.float64, nat.c_double))
+            d.n_leaf_edges, d.n_tips = int(cs.leaf_edges.size), int(cs.tips_pos.size)
+            d.leaf_edges = arr(cs.leaf_edges, np.float64, nat.c_double)
+            d.tips_neg = arr(cs.tips_neg, np.float64, nat.c_double)
+            d.tips_pos = arr(cs.tips_pos, np.float64, nat.c_double)
+            d.n_table = int(model.log_energies.size)
+            d.log_energies = arr(model.log_energies, np.float64, nat.c_double)
+            d.log_mu = arr(model.log_mu, np.float64, nat.c_double)
+        elif name == "MaskTransmission":
+            d.kind = 2
+            d.center[:] = [float(c) for c in model.center]
+            d.u_axis[:] = [float(c) for c in model.u_axis]
+            d.v_axis[:] = [float(c) for c in model.v_axis]
+            d.width_u, d.width_v = model.widths
+            d.mask_nu, d.mask_nv = (int(n) for n in model.mask.shape)
+            d.mask = arr(model.mask, np.float64, nat.c_double)
+        else:
+            raise ValueError(f"no device evaluation for the ray weight model {name}")
+        self._ctx.call("hmc_set_ray_weights", nat.ctypes.byref(d))
+        self._ray_model = model
 
     def _upload_population(self, source) -> None:
         """Copy a DevicePhaseSpace's records to the device, once per population: the

@@ -271,3 +271,31 @@ def test_projection_between_planes():
     np.testing.assert_allclose(project_between_planes(np.array([2.0, -4.0]), 100.0, 50.0), [1.0, -2.0])
     with pytest.raises(ValueError):
         project_between_planes(1.0, 0.0, 1.0)
+
+
+def test_engine_routing_of_weighted_sources():
+    from pyradmc_amd import AnalyticCrossSections, BeamletGridSource, ParallelBeamSource
+    from pyradmc_amd.collimation import CollimatedBeamletSource, TransmissionMaskBeamletSource
+    from pyradmc_amd.engine import _in_kernel
+
+    xs = AnalyticCrossSections()
+    stack = BeamLimitingStack(BeamFrame((0, 0, -100.0)), (JawPair("u", 20.0, 28.0, -3.0, 4.0, 0, 19.3),))
+    beam = ParallelBeamSource(2.0, -0.1, (0.0, 4.0), (0.0, 4.0))
+    lattice = BeamletGridSource(2.0, -0.1, (0.0, 4.0), (0.0, 4.0), 2, 2)
+    assert _in_kernel(CollimatedSource(beam, stack, xs))
+    assert _in_kernel(CollimatedBeamletSource(lattice, stack, xs))
+    assert _in_kernel(TransmissionMaskBeamletSource(lattice, mask=np.ones((2, 2)), plane_center=(2, 2, 0),
+                                                    width_u=4, width_v=4))
+    nested = TransmissionMaskSource(CollimatedSource(beam, stack, xs), mask=np.ones((2, 2)),
+                                    plane_center=(2, 2, 0), width_u=4, width_v=4)
+    assert not _in_kernel(nested)  # one device model per launch: nesting is host-sampled
+
+    class Host:
+        max_energy = 2.0
+
+        def sample_batch(self, seed, offset, n):
+            raise AssertionError
+
+    assert not _in_kernel(CollimatedSource(Host(), stack, xs))
+    with pytest.raises(TypeError, match="generated on the device"):
+        nested.sample_batch(1, 0, 10)
