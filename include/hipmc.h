@@ -51,7 +51,8 @@ enum {
   HMC_SRC_SPECTRAL_FAN = 3,    /* SpectralBeamSource / SpectralBeamletSource */
   HMC_SRC_LATTICE = 4,         /* BeamletGridSource */
   HMC_SRC_UPLOADED = 5,        /* host-sampled primaries */
-  HMC_SRC_RESIDENT = 6         /* records drawn in-kernel from a device-resident population */
+  HMC_SRC_RESIDENT = 6,        /* records drawn in-kernel from a device-resident population */
+  HMC_SRC_SPOT_FAN = 7         /* GaussianSpotBeamSource / GaussianSpotBeamletSource */
 };
 
 typedef struct hmc_source {
@@ -66,7 +67,8 @@ typedef struct hmc_source {
   const double *edges; /* [n_bins + 1] */
   const double *cdf;   /* [n_bins] */
   int n_beamlets;      /* per-column parameters below */
-  const double *beamlet_params; /* lattice: [n][4] x_lo, x_ext, y_lo, y_ext; fan: [n][3] centre */
+  const double *beamlet_params; /* lattice: [n][4] x_lo, x_ext, y_lo, y_ext; fans: [n][3] centre */
+  double sigma_u, sigma_v;     /* Gaussian focal spot (spot fan) */
 } hmc_source;
 
 /* A deterministic per-primary weight factor applied by the generator to in-kernel

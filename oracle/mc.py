@@ -61,6 +61,7 @@ class OmConfig(Structure):
         ("resolution", c_double), ("transport_electrons", c_int), ("msc_gs", c_int),
         ("srest", _dp), ("dose_to_water", c_int),
         ("n_prim", c_int64), ("prim_kind", POINTER(c_int32)), ("prim", _dp),
+        ("sigma_u", c_double), ("sigma_v", c_double),
     ]
 
 
@@ -181,14 +182,15 @@ class Oracle:
             c.gs = _p(values)
             c.gs_ix0, c.gs_iy0 = int(ix0), int(iy0)
             c.gs_neta, c.gs_nth, c.gs_nu = (int(v) for v in values.shape)
-        kinds = {"pencil": 0, "parallel": 1, "parallel_spectral": 2, "fan": 3, "lattice": 4, "given": 5}
+        kinds = {"pencil": 0, "parallel": 1, "parallel_spectral": 2, "fan": 3, "lattice": 4, "given": 5,
+                 "spot": 6}
         c.src_kind = kinds[source["kind"]]
         c.particle = {"photon": 0, "electron": 1}[source.get("particle", "photon")]
         c.energy = float(source.get("energy", 0.0))
         if "position" in source:
             c.pos[:] = source["position"]
             c.dir[:] = source["direction"]
-        for key in ("z0", "x0", "xe", "y0", "ye", "wu", "wv"):
+        for key in ("z0", "x0", "xe", "y0", "ye", "wu", "wv", "sigma_u", "sigma_v"):
             if key in source:
                 setattr(c, key, float(source[key]))
         for key in ("focal", "ua", "va"):
@@ -348,6 +350,11 @@ def source_dict(source) -> dict:
                     x0=source.x_range[0], xe=source.x_range[1] - source.x_range[0],
                     y0=source.y_range[0], ye=source.y_range[1] - source.y_range[0],
                     edges=np.asarray(source.spectrum.edges), cdf=np.asarray(source.spectrum.cdf))
+    if name in ("GaussianSpotBeamSource", "GaussianSpotBeamletSource"):
+        return dict(kind="spot", max_energy=source.spectrum.max_energy, focal=list(source.focal_point),
+                    ua=list(source.u_axis), va=list(source.v_axis), wu=source.width_u, wv=source.width_v,
+                    sigma_u=source.sigma_u, sigma_v=source.sigma_v, edges=np.asarray(source.spectrum.edges),
+                    cdf=np.asarray(source.spectrum.cdf), beamlets=np.asarray(source.centers, dtype=np.float64))
     if name in ("SpectralBeamSource", "SpectralBeamletSource"):
         centers = source.centers
         return dict(kind="fan", max_energy=source.spectrum.max_energy, focal=list(source.focal_point),

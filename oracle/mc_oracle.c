@@ -49,7 +49,7 @@ typedef struct {
   const double *gs;
   int gs_ix0, gs_iy0, gs_neta, gs_nth, gs_nu;
   /* source */
-  int src_kind; /* 0 pencil, 1 parallel, 2 parallel spectral, 3 fan, 4 lattice, 5 given */
+  int src_kind; /* 0 pencil, 1 parallel, 2 parallel spectral, 3 fan, 4 lattice, 5 given, 6 spot fan */
   int particle; /* 0 photon, 1 electron */
   double energy;
   double pos[3], dir[3];
@@ -70,6 +70,8 @@ typedef struct {
   long long n_prim;
   const int32_t *prim_kind;
   const double *prim;
+  /* Gaussian focal spot of the spot fan (src_kind 6; apertures in beamlet [n][3]) */
+  double sigma_u, sigma_v;
 } om_config;
 
 typedef struct {
@@ -493,6 +495,21 @@ static double history(const om_config *c, tally_t *t, stack_t *s, int beamlet, l
     case 4: { const double *b = c->beamlet + 4 * (size_t)beamlet;
               q.kind = 0; q.e = c->energy; q.p[0] = b[0] + b[1] * uniform(r); q.p[1] = b[2] + b[3] * uniform(r);
               q.p[2] = c->z0; break; }
+    case 6: { /* spot fan: energy, rectangle offset, Box-Muller spot; starts on the rectangle
+               * (pyradmc/geometry/source.py:760-792) */
+              const double *cc = c->beamlet + 3 * (size_t)beamlet;
+              q.kind = 0; q.e = spectrum_energy(c, r);
+              double du = (uniform(r) - 0.5) * c->wu, dv = (uniform(r) - 0.5) * c->wv;
+              double u4 = uniform(r); if (u4 < 1e-12) u4 = 1e-12;
+              double rad = sqrt(-2.0 * log(u4)), ang = TWO_PI * uniform(r);
+              double su = c->sigma_u * rad * cos(ang), sv = c->sigma_v * rad * sin(ang), d[3];
+              for (int a = 0; a < 3; ++a) {
+                q.p[a] = cc[a] + du * c->ua[a] + dv * c->va[a];
+                d[a] = q.p[a] - (c->focal[a] + su * c->ua[a] + sv * c->va[a]);
+              }
+              double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+              for (int a = 0; a < 3; ++a) q.u[a] = d[a] / nrm;
+              break; }
     default: { const double *cc = c->beamlet + 3 * (size_t)beamlet;
                q.kind = 0; q.e = spectrum_energy(c, r);
                double du = (uniform(r) - 0.5) * c->wu, dv = (uniform(r) - 0.5) * c->wv, d[3];

@@ -18,8 +18,21 @@ from pyradmc_amd.constants import (
 from pyradmc_amd.data.analytic import AnalyticCrossSections
 from pyradmc_amd.data.materials import MATERIALS, WATER
 from pyradmc_amd.data.tabulated import TabulatedCrossSections, TabulatedData
+from pyradmc_amd.collimation import (
+    MLC,
+    BeamFrame,
+    BeamLimitingStack,
+    CollimatedBeamletSource,
+    CollimatedSource,
+    JawPair,
+    TransmissionMaskBeamletSource,
+    TransmissionMaskSource,
+    project_between_planes,
+)
 from pyradmc_amd.geometry import (
     BeamletGridSource,
+    GaussianSpotBeamletSource,
+    GaussianSpotBeamSource,
     ParallelBeamSource,
     PencilBeamSource,
     SpectralBeamletSource,
@@ -56,16 +69,24 @@ def __getattr__(name: str):
 
 __all__ = [
     "AnalyticCrossSections",
+    "BeamFrame",
+    "BeamLimitingStack",
     "BeamletGridSource",
+    "CollimatedBeamletSource",
+    "CollimatedSource",
     "DIJ_TRUNCATION_RELATIVE",
     "DeviceDij",
     "DijResult",
     "ECUT_MEV",
     "ELECTRON_MASS_MEV",
     "GY_PER_MEV_PER_G",
+    "GaussianSpotBeamSource",
+    "GaussianSpotBeamletSource",
     "HipEngine",
     "IAEAPhaseSpace",
     "InMemoryPhaseSpaceSource",
+    "JawPair",
+    "MLC",
     "MATERIALS",
     "PCUT_MEV",
     "ParallelBeamSource",
@@ -80,10 +101,13 @@ __all__ = [
     "Spectrum",
     "TabulatedCrossSections",
     "TabulatedData",
+    "TransmissionMaskBeamletSource",
+    "TransmissionMaskSource",
     "TransportResult",
     "VoxelGrid",
     "WATER",
     "bremsstrahlung_6mv_like",
+    "project_between_planes",
     "read_iaea_header",
     "write_iaea_phase_space",
 ]

@@ -39,6 +39,7 @@ SRC_SPECTRAL_FAN = 3
 SRC_LATTICE = 4
 SRC_UPLOADED = 5
 SRC_RESIDENT = 6
+SRC_SPOT_FAN = 7
 
 _f32p = POINTER(ctypes.c_float)
 _f64p = POINTER(c_double)
@@ -135,6 +136,8 @@ class SourceDesc(Structure):
         ("cdf", _f64p),
         ("n_beamlets", c_int),
         ("beamlet_params", _f64p),
+        ("sigma_u", c_double),
+        ("sigma_v", c_double),
     ]
 
 

@@ -835,6 +835,7 @@ struct DevSource {
   float px, py, pz, dx, dy, dz;
   float z0, x0, xe, y0, ye;
   float fx, fy, fz, uax, uay, uaz, vax, vay, vaz, width_u, width_v;
+  float sigma_u, sigma_v;  // spot fan
   int n_bins;
   const float* edges;
   const float* cdf;
@@ -950,6 +951,29 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
           book = true;
           break;
         }
+        case HMC_SRC_SPOT_FAN: {
+          // Six uniforms: energy (two), rectangle offset (two), Box-Muller spot (two);
+          // the primary starts on the rectangle, aimed from the sampled spot point
+          // (pyradmc/geometry/source.py:760-792).
+          e = sample_spectrum(S, rng);
+          const float du = (uniform(rng) - 0.5f) * S.width_u;
+          const float dv = (uniform(rng) - 0.5f) * S.width_v;
+          const float radius = sqrtf(-2.0f * logf(fmaxf(uniform(rng), 1.0e-12f)));
+          const float angle = kTwoPi * uniform(rng);
+          const float su = S.sigma_u * radius * cosf(angle), sv = S.sigma_v * radius * sinf(angle);
+          const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
+          x = c[0] + du * S.uax + dv * S.vax;
+          y = c[1] + du * S.uay + dv * S.vay;
+          z = c[2] + du * S.uaz + dv * S.vaz;
+          const float dx = x - (S.fx + su * S.uax + sv * S.vax);
+          const float dy = y - (S.fy + su * S.uay + sv * S.vay);
+          const float dz = z - (S.fz + su * S.uaz + sv * S.vaz);
+          const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+          ux = dx * inv; uy = dy * inv; uz = dz * inv;
+          kind = PHOTON;
+          book = true;
+          break;
+        }
         default: {  // host-sampled primaries by flat primary index, or a record of the
                     // resident population drawn with 64 bits of the history's stream
           long long j = g;
@@ -986,7 +1010,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
       p.meta = pack_meta(kind, 1u, column);
       p.pad = 0u;
     }
-    if (S.kind < HMC_SRC_UPLOADED) {
+    if (S.kind != HMC_SRC_UPLOADED && S.kind != HMC_SRC_RESIDENT) {
       // One kind per launch: fixed slots, no atomics.
       const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
       if (live && t < q.capacity) q.data[t] = p;
@@ -1021,7 +1045,7 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
     }
     emitted_sum += emitted;
   }
-  if (S.kind < HMC_SRC_UPLOADED && blockIdx.x == 0 && threadIdx.x == 0) {
+  if (S.kind != HMC_SRC_UPLOADED && S.kind != HMC_SRC_RESIDENT && blockIdx.x == 0 && threadIdx.x == 0) {
     const Queue& q = S.particle == PHOTON || S.kind >= HMC_SRC_PARALLEL_SPECTRAL ? qp : qe;
     *q.count = static_cast<unsigned int>(n_gen);
   }
@@ -1814,8 +1838,10 @@ int hmc_set_gs(hmc_ctx* ctx, const float* values, int n_eta, int n_theta2, int n
 
 int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
   if (!ctx || !s) return fail(ctx, HMC_EINVAL, "null");
-  if (s->kind < HMC_SRC_PENCIL || s->kind > HMC_SRC_RESIDENT)
+  if (s->kind < HMC_SRC_PENCIL || s->kind > HMC_SRC_SPOT_FAN)
     return fail(ctx, HMC_EINVAL, "unknown source kind");
+  if (s->kind == HMC_SRC_SPOT_FAN && !(s->sigma_u >= 0.0 && s->sigma_v >= 0.0))
+    return fail(ctx, HMC_EINVAL, "spot sigma must be non-negative");
   if (s->kind == HMC_SRC_RESIDENT && (ctx->pop_n < 1 || ctx->pop_written != ctx->pop_n))
     return fail(ctx, HMC_EINVAL, "resident source without a complete population");
   if (s->particle < 0 || s->particle > 2) return fail(ctx, HMC_EINVAL, "unknown particle kind");
@@ -1848,7 +1874,10 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
   d.width_v = static_cast<float>(s->width_v);
   d.n_bins = s->n_bins;
   int rc;
-  const bool spectral = s->kind == HMC_SRC_PARALLEL_SPECTRAL || s->kind == HMC_SRC_SPECTRAL_FAN;
+  d.sigma_u = static_cast<float>(s->sigma_u);
+  d.sigma_v = static_cast<float>(s->sigma_v);
+  const bool spectral = s->kind == HMC_SRC_PARALLEL_SPECTRAL || s->kind == HMC_SRC_SPECTRAL_FAN ||
+                        s->kind == HMC_SRC_SPOT_FAN;
   if (spectral) {
     if (s->n_bins < 1 || !s->edges || !s->cdf) return fail(ctx, HMC_EINVAL, "spectrum missing");
     std::vector<float> e(s->edges, s->edges + s->n_bins + 1), c(s->cdf, s->cdf + s->n_bins);
@@ -1865,7 +1894,7 @@ int hmc_set_source(hmc_ctx* ctx, const hmc_source* s) {
     d.edges = ctx->d_edges;
     d.cdf = ctx->d_cdf;
   }
-  if (s->kind == HMC_SRC_LATTICE || s->kind == HMC_SRC_SPECTRAL_FAN) {
+  if (s->kind == HMC_SRC_LATTICE || s->kind == HMC_SRC_SPECTRAL_FAN || s->kind == HMC_SRC_SPOT_FAN) {
     const int stride = s->kind == HMC_SRC_LATTICE ? 4 : 3;
     if (s->n_beamlets < 1 || !s->beamlet_params)
       return fail(ctx, HMC_EINVAL, "beamlet parameters missing");
@@ -2219,7 +2248,8 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
         items[i].column >= ctx->n_columns * ctx->n_slots)
       return fail(ctx, HMC_EINVAL, "work item count/column out of range");
     slots_written = std::max(slots_written, static_cast<int>(items[i].column / ctx->n_columns) + 1);
-    const bool per_beamlet = ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN;
+    const bool per_beamlet = ctx->src.kind == HMC_SRC_LATTICE || ctx->src.kind == HMC_SRC_SPECTRAL_FAN ||
+                             ctx->src.kind == HMC_SRC_SPOT_FAN;
     if (items[i].beamlet < 0 || (per_beamlet && items[i].beamlet >= ctx->src_beamlets))
       return fail(ctx, HMC_EINVAL, "work item beamlet out of range");
     dev_items[i].key_base = items[i].key_base;

@@ -88,6 +88,7 @@ def _f32(a) -> np.ndarray:
 _BUILTIN_SOURCES = frozenset({
     "PencilBeamSource", "ParallelBeamSource", "SpectralParallelBeamSource",
     "SpectralBeamSource", "BeamletGridSource", "SpectralBeamletSource", "DevicePhaseSpace",
+    "GaussianSpotBeamSource", "GaussianSpotBeamletSource",
 })
 _BUILTIN_MODULES = frozenset({"pyradmc_amd.geometry", "pyradmc.geometry.source", "pyradmc_amd.phasespace"})
 _UPLOAD_COLUMNS = ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")
@@ -414,15 +415,19 @@ class HipEngine:
             d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
             d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
             spectrum_arrays(source.spectrum)
-        elif name in ("SpectralBeamSource", "SpectralBeamletSource"):
-            d.kind = nat.SRC_SPECTRAL_FAN
+        elif name in ("SpectralBeamSource", "SpectralBeamletSource", "GaussianSpotBeamSource",
+                      "GaussianSpotBeamletSource"):
+            spot = name.startswith("GaussianSpot")
+            d.kind = nat.SRC_SPOT_FAN if spot else nat.SRC_SPECTRAL_FAN
+            if spot:
+                d.sigma_u, d.sigma_v = float(source.sigma_u), float(source.sigma_v)
             d.particle = 0
             spectrum_arrays(source.spectrum)
             d.focal[:] = [float(c) for c in source.focal_point]
             d.u_axis[:] = [float(c) for c in source.u_axis]
             d.v_axis[:] = [float(c) for c in source.v_axis]
             d.width_u, d.width_v = float(source.width_u), float(source.width_v)
-            centers = source.centers if name == "SpectralBeamletSource" else (source.center,)
+            centers = source.centers if name.endswith("BeamletSource") else (source.center,)
             sel = list(beamlets) if beamlets is not None else list(range(len(centers)))
             params = np.ascontiguousarray(np.array([centers[j] for j in sel], dtype=np.float64).reshape(-1, 3))
             keep.append(params)
@@ -450,8 +455,7 @@ class HipEngine:
         else:
             raise ValueError(
                 f"source type {name} has no in-kernel generator in this engine; supported: "
-                "PencilBeamSource, ParallelBeamSource, SpectralParallelBeamSource, "
-                "SpectralBeamSource, BeamletGridSource, SpectralBeamletSource"
+                + ", ".join(sorted(_BUILTIN_SOURCES))
             )
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
         return analytic, energy

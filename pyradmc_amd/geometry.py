@@ -417,3 +417,170 @@ class SpectralBeamletSource(BeamletSource):
     @property
     def v_axis(self):
         return self._fan.v_axis
+
+
+# Floor of the Box-Muller uniform, so an exact zero draw gives a finite radius
+# (pyradmc/geometry/source.py:731).
+BOX_MULLER_FLOOR = 1.0e-12
+
+
+class _SpotFan:
+    """A divergent fan from a 2D-Gaussian focal spot (pyradmc/geometry/source.py:
+    735-835): a primary starts on an aperture rectangle and travels along the line from
+    a spot point ``focal + s_u u + s_v v`` (independent Gaussians of sigmas
+    ``sigma_u, sigma_v`` by Box-Muller) through its start point. Six uniforms per
+    history in fixed order: two for the energy, two for the offset in the rectangle, two
+    for the spot (drawn at zero sigma too), so a zero-sigma source starts on the fan's
+    own lines."""
+
+    def __init__(self, fan: _Fan, sigma_u: float, sigma_v: float) -> None:
+        if sigma_u < 0.0 or sigma_v < 0.0:
+            raise ValueError(f"spot sigma must be non-negative, got ({sigma_u}, {sigma_v})")
+        self.fan = fan
+        self.sigma_u = float(sigma_u)
+        self.sigma_v = float(sigma_v)
+
+    def sample(self, spectrum: Spectrum, aperture: int, seed: int, history_offset: int, n: int) -> dict:
+        """Host-sampled columns of histories ``[history_offset, history_offset + n)``:
+        draws ``6h .. 6h + 5`` of ``Generator(PCG64(seed))`` for history ``h``, whatever
+        the chunking and the aperture (pyradmc/geometry/source.py:795-835)."""
+        bits = np.random.PCG64(int(seed))
+        bits.advance(6 * int(history_offset))
+        u = np.random.Generator(bits).random((int(n), 6))
+        f = self.fan
+        energy = spectrum.sample_energies(u[:, 0], u[:, 1])
+        du = (u[:, 2] - 0.5) * f.width_u
+        dv = (u[:, 3] - 0.5) * f.width_v
+        radius = np.sqrt(-2.0 * np.log(np.maximum(u[:, 4], BOX_MULLER_FLOOR)))
+        angle = 2.0 * np.pi * u[:, 5]
+        su, sv = self.sigma_u * radius * np.cos(angle), self.sigma_v * radius * np.sin(angle)
+        ua, va = np.asarray(f.u_axis), np.asarray(f.v_axis)
+        start = np.asarray(f.centers[aperture]) + du[:, None] * ua + dv[:, None] * va
+        spot = np.asarray(f.focal_point) + su[:, None] * ua + sv[:, None] * va
+        d = start - spot
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        return {"particle_type": np.ones(int(n), np.int32), "energy": energy.astype(np.float32),
+                "x": start[:, 0].astype(np.float32), "y": start[:, 1].astype(np.float32),
+                "z": start[:, 2].astype(np.float32), "ux": d[:, 0].astype(np.float32),
+                "uy": d[:, 1].astype(np.float32), "uz": d[:, 2].astype(np.float32),
+                "weight": np.ones(int(n), np.float32)}
+
+
+class GaussianSpotBeamSource(Source):
+    """Photons born on a plane rectangle, aimed from a 2D-Gaussian focal spot
+    (pyradmc/geometry/source.py:838-881): the head-input source. Generated in-kernel by
+    the engine (six uniforms of the history's stream); ``sample_batch`` gives the same
+    distribution on the host (for the head pre-solve or a host-side wrapper)."""
+
+    def __init__(self, spectrum: Spectrum, focal_point, center, width_u, width_v, sigma_u, sigma_v,
+                 u_axis=(1.0, 0.0, 0.0), v_axis=(0.0, 1.0, 0.0)) -> None:
+        self._spectrum = spectrum
+        self._spot = _SpotFan(_Fan(focal_point, (center,), width_u, width_v, u_axis, v_axis), sigma_u, sigma_v)
+
+    @property
+    def max_energy(self) -> float:
+        return self._spectrum.max_energy
+
+    @property
+    def spectrum(self) -> Spectrum:
+        return self._spectrum
+
+    @property
+    def focal_point(self):
+        return self._spot.fan.focal_point
+
+    @property
+    def center(self):
+        return self._spot.fan.centers[0]
+
+    @property
+    def centers(self):
+        return self._spot.fan.centers
+
+    @property
+    def width_u(self) -> float:
+        return self._spot.fan.width_u
+
+    @property
+    def width_v(self) -> float:
+        return self._spot.fan.width_v
+
+    @property
+    def u_axis(self):
+        return self._spot.fan.u_axis
+
+    @property
+    def v_axis(self):
+        return self._spot.fan.v_axis
+
+    @property
+    def sigma_u(self) -> float:
+        return self._spot.sigma_u
+
+    @property
+    def sigma_v(self) -> float:
+        return self._spot.sigma_v
+
+    def sample_batch(self, seed: int, history_offset: int, n: int) -> dict:
+        return self._spot.sample(self._spectrum, 0, seed, history_offset, n)
+
+
+class GaussianSpotBeamletSource(BeamletSource):
+    """One plane rectangle per beamlet, aimed from a 2D-Gaussian focal spot
+    (pyradmc/geometry/source.py:884-944); the draws never see the beamlet, so
+    correlated Dij sampling replays energy, offset and spot point in every column."""
+
+    def __init__(self, spectrum: Spectrum, focal_point, centers: Sequence, width_u, width_v, sigma_u, sigma_v,
+                 u_axis=(1.0, 0.0, 0.0), v_axis=(0.0, 1.0, 0.0)) -> None:
+        self._spectrum = spectrum
+        self._spot = _SpotFan(_Fan(focal_point, tuple(centers), width_u, width_v, u_axis, v_axis), sigma_u,
+                              sigma_v)
+
+    @property
+    def max_energy(self) -> float:
+        return self._spectrum.max_energy
+
+    @property
+    def n_beamlets(self) -> int:
+        return len(self._spot.fan.centers)
+
+    @property
+    def spectrum(self) -> Spectrum:
+        return self._spectrum
+
+    @property
+    def focal_point(self):
+        return self._spot.fan.focal_point
+
+    @property
+    def centers(self):
+        return self._spot.fan.centers
+
+    @property
+    def width_u(self) -> float:
+        return self._spot.fan.width_u
+
+    @property
+    def width_v(self) -> float:
+        return self._spot.fan.width_v
+
+    @property
+    def u_axis(self):
+        return self._spot.fan.u_axis
+
+    @property
+    def v_axis(self):
+        return self._spot.fan.v_axis
+
+    @property
+    def sigma_u(self) -> float:
+        return self._spot.sigma_u
+
+    @property
+    def sigma_v(self) -> float:
+        return self._spot.sigma_v
+
+    def sample_beamlet_batch(self, seed: int, history_offset: int, n: int, beamlet: int) -> dict:
+        if not 0 <= beamlet < self.n_beamlets:
+            raise IndexError(f"beamlet {beamlet} outside fan of {self.n_beamlets}")
+        return self._spot.sample(self._spectrum, beamlet, seed, history_offset, n)
