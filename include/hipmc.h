@@ -176,6 +176,40 @@ int hmc_set_population(hmc_ctx *ctx, int64_t n, int64_t offset, int64_t count,
                        const float *y, const float *z, const float *ux, const float *uy,
                        const float *uz, const float *weight);
 int hmc_set_physics(hmc_ctx *ctx, const hmc_physics *physics);
+
+/* Treatment-head pre-solve (pyradmc/backends/warp/presolve.py:572-687, physics of
+ * pyradmc/geometry/head.py:377-583). Per photon primary: Beer-Lambert transmission
+ * through the stack, optionally a forced first Compton interaction at the mid-plane of
+ * a device chosen by its share of the optical depth (the scattered photon attenuated
+ * on its way out), an air column (attenuation, and a forced Compton photon and recoil
+ * electron at an exponentially sampled depth), fair roulette of forced secondaries
+ * below weight_floor, then every forward particle at or above its cutoff moved to the
+ * exit plane w = exit_z. The population (weights scaled by kept / primaries) becomes
+ * the context's resident population in primary order (stack photon, air photon, air
+ * electron, transmitted photon of each primary), ready for a HMC_SRC_RESIDENT source. */
+typedef struct hmc_presolve {
+  int first_compton;            /* 0 attenuation only, 1 forced first Compton */
+  int has_air;
+  double exit_z;                /* local w of the exit plane */
+  double air_z_start, air_z_end;
+  double pcut, ecut, weight_floor;
+  double origin[3], basis[9];   /* the beam frame (the stack's, when there is a stack) */
+  int n_table;                  /* shared energy grid of the attenuation tables */
+  int n_rows;                   /* stack devices in order, then the air column */
+  const double *log_energies;   /* [n_table] increasing */
+  const double *log_mu_total;   /* [n_rows][n_table] log of the macroscopic mu (1/cm) */
+  const double *log_mu_compton; /* [n_rows][n_table] */
+} hmc_presolve;
+/* n photon primaries (sample_batch columns); stack: a kind-1 model (its tables unused)
+ * or NULL; *n_out receives the population size. At most 16 devices. */
+int hmc_presolve_head(hmc_ctx *ctx, uint64_t seed, int64_t n, const float *energy, const float *x,
+                      const float *y, const float *z, const float *ux, const float *uy,
+                      const float *uz, const float *weight, const hmc_ray_weights *stack,
+                      const hmc_presolve *presolve, int64_t *n_out);
+/* Rows [offset, offset + count) of the resident population (kinds 0 photon, 1 electron,
+ * 2 positron) into host columns. */
+int hmc_get_population(hmc_ctx *ctx, int64_t offset, int64_t count, int32_t *particle, float *energy,
+                       float *x, float *y, float *z, float *ux, float *uy, float *uz, float *weight);
 /* The ray weight model of the following transports (NULL or kind 0: none). */
 int hmc_set_ray_weights(hmc_ctx *ctx, const hmc_ray_weights *weights);
 /* The current model's factor for each of n rays (energy, position [n][3], direction

@@ -42,7 +42,9 @@ from pyradmc_amd.geometry import (
     VoxelGrid,
     bremsstrahlung_6mv_like,
 )
+from pyradmc_amd.head import AirColumn, presolve_head
 from pyradmc_amd.phasespace import (
+    DevicePhaseSpace,
     IAEAPhaseSpace,
     InMemoryPhaseSpaceSource,
     PhaseSpaceSource,
@@ -68,6 +70,7 @@ def __getattr__(name: str):
 
 
 __all__ = [
+    "AirColumn",
     "AnalyticCrossSections",
     "BeamFrame",
     "BeamLimitingStack",
@@ -76,6 +79,7 @@ __all__ = [
     "CollimatedSource",
     "DIJ_TRUNCATION_RELATIVE",
     "DeviceDij",
+    "DevicePhaseSpace",
     "DijResult",
     "ECUT_MEV",
     "ELECTRON_MASS_MEV",
@@ -107,6 +111,7 @@ __all__ = [
     "VoxelGrid",
     "WATER",
     "bremsstrahlung_6mv_like",
+    "presolve_head",
     "project_between_planes",
     "read_iaea_header",
     "write_iaea_phase_space",

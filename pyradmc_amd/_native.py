@@ -114,6 +114,28 @@ class RayWeightsDesc(Structure):
     ]
 
 
+class PresolveDesc(Structure):
+    """hmc_presolve: the head pre-solve's settings and attenuation tables."""
+
+    _fields_ = [
+        ("first_compton", c_int),
+        ("has_air", c_int),
+        ("exit_z", c_double),
+        ("air_z_start", c_double),
+        ("air_z_end", c_double),
+        ("pcut", c_double),
+        ("ecut", c_double),
+        ("weight_floor", c_double),
+        ("origin", c_double * 3),
+        ("basis", c_double * 9),
+        ("n_table", c_int),
+        ("n_rows", c_int),
+        ("log_energies", _f64p),
+        ("log_mu_total", _f64p),
+        ("log_mu_compton", _f64p),
+    ]
+
+
 class SourceDesc(Structure):
     _fields_ = [
         ("kind", c_int),
@@ -220,6 +242,9 @@ def lib():
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_population": (c_int, [ctx_p, c_int64, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_ray_weights": (c_int, [ctx_p, POINTER(RayWeightsDesc)]),
+        "hmc_presolve_head": (c_int, [ctx_p, c_uint64, c_int64] + [_f32p] * 8
+                              + [POINTER(RayWeightsDesc), POINTER(PresolveDesc), POINTER(c_int64)]),
+        "hmc_get_population": (c_int, [ctx_p, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_ray_weight_factors": (c_int, [ctx_p, c_int64, _f32p, _f32p, _f32p, _f64p]),
         "hmc_set_physics": (c_int, [ctx_p, POINTER(Physics)]),
         "hmc_set_profiling": (c_int, [ctx_p, c_int]),

@@ -52,25 +52,27 @@ __host__ __device__ inline double clip_affine(double s_lo, double s_hi, double a
   return span(s_lo, fmin(s_hi, root));
 }
 
-// Slab interval of the ray; false when it misses the slab.
-__host__ __device__ inline bool slab_interval(double p_w, double d_w, double zt, double zb, double& lo,
-                                              double& hi) {
+// Slab interval of the ray (the forward half-line only if from_origin); false when it
+// misses the slab.
+__host__ __device__ inline bool slab_interval(double p_w, double d_w, double zt, double zb, bool from_origin,
+                                              double& lo, double& hi) {
   if (d_w == 0.0) {
     if (p_w < zt || p_w > zb) return false;
     lo = -INFINITY;
     hi = INFINITY;
-    return true;
+  } else {
+    const double t1 = (zt - p_w) / d_w, t2 = (zb - p_w) / d_w;
+    lo = fmin(t1, t2);
+    hi = fmax(t1, t2);
   }
-  const double t1 = (zt - p_w) / d_w, t2 = (zb - p_w) / d_w;
-  lo = fmin(t1, t2);
-  hi = fmax(t1, t2);
+  if (from_origin) lo = fmax(lo, 0.0);
   return hi > lo;
 }
 
 __host__ __device__ inline double jaw_chord(const double* p, const double* d, double zt, double zb,
-                                            double en, double ep, int flags) {
+                                            double en, double ep, int flags, bool from_origin = false) {
   double s_lo, s_hi;
-  if (!slab_interval(p[2], d[2], zt, zb, s_lo, s_hi)) return 0.0;
+  if (!slab_interval(p[2], d[2], zt, zb, from_origin, s_lo, s_hi)) return 0.0;
   const int k = flags & 1;
   double ap, bp, an, bn;
   if (flags & 2) {
@@ -138,10 +140,11 @@ __host__ __device__ inline int upper_bound(const double* e, int lo, int hi, doub
   return lo;
 }
 
-__host__ __device__ inline double mlc_chord(const DevRayWeights& W, int j, const double* p, const double* d) {
+__host__ __device__ inline double mlc_chord(const DevRayWeights& W, int j, const double* p, const double* d,
+                                            bool from_origin = false) {
   double s_lo, s_hi;
   const double zt = W.z_top[j], zb = W.z_bot[j];
-  if (!slab_interval(p[2], d[2], zt, zb, s_lo, s_hi)) return 0.0;
+  if (!slab_interval(p[2], d[2], zt, zb, from_origin, s_lo, s_hi)) return 0.0;
   const double zm = 0.5 * (zt + zb), r = W.e_neg[j];
   const bool focused = (W.flags[j] & 2) != 0;
   const int n = W.n_pairs[j];
@@ -198,19 +201,29 @@ __host__ __device__ inline double mlc_chord(const DevRayWeights& W, int j, const
   return total;
 }
 
-// exp(interp(log E)) of one table row, clamped at the ends (numpy.interp's rule).
-__host__ __device__ inline double table_mu(const DevRayWeights& W, int j, double le) {
-  const double* x = W.log_e;
-  const double* y = W.log_mu + static_cast<long long>(j) * W.n_table;
-  const int n = W.n_table;
+// exp(interp(le)) of a table row y over increasing nodes x, clamped at the ends
+// (numpy.interp's rule); the node is guessed from the mean spacing and corrected.
+__host__ __device__ inline double log_table(const double* x, const double* y, int n, double le0, double inv_dle,
+                                            double le) {
   if (le <= x[0]) return exp(y[0]);
   if (le >= x[n - 1]) return exp(y[n - 1]);
-  int k = static_cast<int>((le - W.le0) * W.inv_dle);
+  int k = static_cast<int>((le - le0) * inv_dle);
   k = min(max(k, 0), n - 2);
   while (k > 0 && x[k] > le) --k;
   while (k < n - 2 && x[k + 1] <= le) ++k;
   const double f = (le - x[k]) / (x[k + 1] - x[k]);
   return exp(y[k] + f * (y[k + 1] - y[k]));
+}
+
+__host__ __device__ inline double table_mu(const DevRayWeights& W, int j, double le) {
+  return log_table(W.log_e, W.log_mu + static_cast<long long>(j) * W.n_table, W.n_table, W.le0, W.inv_dle, le);
+}
+
+// Chord of one ray (beam frame) through device j.
+__host__ __device__ inline double device_chord(const DevRayWeights& W, int j, const double* p, const double* d,
+                                               bool from_origin) {
+  return W.dkind[j] == 0 ? jaw_chord(p, d, W.z_top[j], W.z_bot[j], W.e_neg[j], W.e_pos[j], W.flags[j], from_origin)
+                         : mlc_chord(W, j, p, d, from_origin);
 }
 
 __host__ __device__ inline double stack_factor(const DevRayWeights& W, double energy, const double* pe,
@@ -224,8 +237,7 @@ __host__ __device__ inline double stack_factor(const DevRayWeights& W, double en
   const double le = log(energy);
   double tau = 0.0;
   for (int j = 0; j < W.n_dev; ++j) {
-    const double t = W.dkind[j] == 0 ? jaw_chord(p, d, W.z_top[j], W.z_bot[j], W.e_neg[j], W.e_pos[j], W.flags[j])
-                                     : mlc_chord(W, j, p, d);
+    const double t = device_chord(W, j, p, d, false);
     if (t > 0.0) tau += W.dens[j] * table_mu(W, j, le) * t;
   }
   return exp(-tau);

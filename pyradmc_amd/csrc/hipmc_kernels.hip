@@ -1768,7 +1768,6 @@ int hmc_set_tables(hmc_ctx* ctx, const hmc_tables* tb) {
     return fail(ctx, HMC_EINVAL, "table sizes out of range (n_points>=2, 1<=n_mat<=256)");
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   for (void* p : ctx->table_buffers) free_ptr(p);
-  for (void* p : ctx->rw_buffers) free_ptr(p);
   ctx->table_buffers.clear();
   const size_t rows = static_cast<size_t>(nm) * n;
   std::vector<float4> photon(rows), ea(rows), eb(rows);
@@ -1978,30 +1977,29 @@ __global__ void k_ray_weights(DevRayWeights RW, long long n, const float* e, con
                         dir[3 * i + 2]);
 }
 
-int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
-  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
-  HMC_HIP(ctx, hipSetDevice(ctx->device));
-  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no generator may be reading the buffers
-  for (void* p : ctx->rw_buffers) free_ptr(p);
-  ctx->rw_buffers.clear();
-  ctx->rw = DevRayWeights{};
-  if (!w || w->kind == 0) return HMC_OK;
-  DevRayWeights r{};
+// Validate a ray weight model and copy its arrays to the device (buffers appended to
+// bufs, which the caller frees); with need_tables false a stack's tables are not read.
+int build_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w, bool need_tables, DevRayWeights& r,
+                      std::vector<void*>& bufs) {
+  r = DevRayWeights{};
   int rc;
   auto put = [&](const auto* host, size_t count, const auto** dst) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
     T* d = nullptr;
     const int code = upload(ctx, &d, host, count);
-    if (d) ctx->rw_buffers.push_back(d);
+    if (d) {
+      bufs.push_back(d);
+      ctx->alloc_bytes.erase(d);  // owned by bufs, freed with free_ptr
+    }
     *dst = d;
     return code;
   };
   if (w->kind == 1) {
     const int n = w->n_devices;
-    if (n < 1 || w->n_table < 2 || !w->device_kind || !w->flags || !w->n_pairs || !w->edge_offset ||
-        !w->pair_offset || !w->z_top || !w->z_bottom || !w->edge_neg || !w->edge_pos || !w->density ||
-        !w->log_energies || !w->log_mu || w->n_leaf_edges < 1 || w->n_tips < 1 || !w->leaf_edges ||
-        !w->tips_neg || !w->tips_pos)
+    if (n < 1 || !w->device_kind || !w->flags || !w->n_pairs || !w->edge_offset || !w->pair_offset ||
+        !w->z_top || !w->z_bottom || !w->edge_neg || !w->edge_pos || !w->density || w->n_leaf_edges < 1 ||
+        w->n_tips < 1 || !w->leaf_edges || !w->tips_neg || !w->tips_pos ||
+        (need_tables && (w->n_table < 2 || !w->log_energies || !w->log_mu)))
       return fail(ctx, HMC_EINVAL, "beam-limiting stack incomplete");
     for (int j = 0; j < n; ++j) {
       if (w->device_kind[j] != 0 && w->device_kind[j] != 1) return fail(ctx, HMC_EINVAL, "unknown device kind");
@@ -2017,10 +2015,10 @@ int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
             return fail(ctx, HMC_EINVAL, "MLC strip edges must increase");
       }
     }
-    for (int k = 1; k < w->n_table; ++k)
-      if (!(w->log_energies[k] > w->log_energies[k - 1])) return fail(ctx, HMC_EINVAL, "energies must increase");
+    if (need_tables)
+      for (int k = 1; k < w->n_table; ++k)
+        if (!(w->log_energies[k] > w->log_energies[k - 1])) return fail(ctx, HMC_EINVAL, "energies must increase");
     r.n_dev = n;
-    r.n_table = w->n_table;
     for (int k = 0; k < 3; ++k) r.o[k] = w->origin[k];
     for (int k = 0; k < 9; ++k) r.b[k] = w->basis[k];
     if ((rc = put(w->device_kind, n, &r.dkind)) || (rc = put(w->flags, n, &r.flags)) ||
@@ -2029,11 +2027,16 @@ int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
         (rc = put(w->z_bottom, n, &r.z_bot)) || (rc = put(w->edge_neg, n, &r.e_neg)) ||
         (rc = put(w->edge_pos, n, &r.e_pos)) || (rc = put(w->density, n, &r.dens)) ||
         (rc = put(w->leaf_edges, w->n_leaf_edges, &r.edges)) || (rc = put(w->tips_neg, w->n_tips, &r.tn)) ||
-        (rc = put(w->tips_pos, w->n_tips, &r.tp)) || (rc = put(w->log_energies, w->n_table, &r.log_e)) ||
-        (rc = put(w->log_mu, static_cast<size_t>(n) * w->n_table, &r.log_mu)))
+        (rc = put(w->tips_pos, w->n_tips, &r.tp)))
       return rc;
-    r.le0 = w->log_energies[0];
-    r.inv_dle = (w->n_table - 1) / (w->log_energies[w->n_table - 1] - w->log_energies[0]);
+    if (need_tables) {
+      r.n_table = w->n_table;
+      if ((rc = put(w->log_energies, w->n_table, &r.log_e)) ||
+          (rc = put(w->log_mu, static_cast<size_t>(n) * w->n_table, &r.log_mu)))
+        return rc;
+      r.le0 = w->log_energies[0];
+      r.inv_dle = (w->n_table - 1) / (w->log_energies[w->n_table - 1] - w->log_energies[0]);
+    }
   } else if (w->kind == 2) {
     if (w->mask_nu < 1 || w->mask_nv < 1 || !w->mask || !(w->width_u > 0.0) || !(w->width_v > 0.0))
       return fail(ctx, HMC_EINVAL, "mask incomplete");
@@ -2057,6 +2060,24 @@ int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
     return fail(ctx, HMC_EINVAL, "unknown ray weight kind");
   }
   r.kind = w->kind;
+  return HMC_OK;
+}
+
+int hmc_set_ray_weights(hmc_ctx* ctx, const hmc_ray_weights* w) {
+  if (!ctx) return fail(ctx, HMC_EINVAL, "null");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no generator may be reading the buffers
+  for (void* p : ctx->rw_buffers) free_ptr(p);
+  ctx->rw_buffers.clear();
+  ctx->rw = DevRayWeights{};
+  if (!w || w->kind == 0) return HMC_OK;
+  DevRayWeights r{};
+  const int rc = build_ray_weights(ctx, w, true, r, ctx->rw_buffers);
+  if (rc) {
+    for (void* p : ctx->rw_buffers) free_ptr(p);
+    ctx->rw_buffers.clear();
+    return rc;
+  }
   ctx->rw = r;
   return HMC_OK;
 }
@@ -2088,6 +2109,336 @@ int hmc_ray_weight_factors(hmc_ctx* ctx, int64_t n, const float* energy, const f
     }
   }
   return rc;
+}
+
+// ------------------------------------------------------------ head pre-solve
+
+constexpr int kPresolveMaxDevices = 16;
+constexpr int kPresolveSlots = 4;           // stack photon, air photon, air electron, transmitted
+constexpr uint64_t kPresolveSalt = 0x48454144ULL;
+
+struct DevPresolve {
+  int first_compton, has_air, n_table, air_row;
+  double exit_z, air_z0, air_z1, pcut, ecut, floor;
+  double o[3], b[9];
+  const double* log_e;
+  const double* log_tot;  // [rows][n_table]
+  const double* log_com;
+  double le0, inv_dle;
+};
+
+struct PresolveRecord {
+  float e, x, y, z, ux, uy, uz, w;
+  int kind;  // -1: not kept
+};
+
+__device__ inline double presolve_mu(const DevPresolve& P, const double* tab, int row, double e) {
+  return log_table(P.log_e, tab + static_cast<long long>(row) * P.n_table, P.n_table, P.le0, P.inv_dle, log(e));
+}
+
+__device__ inline double roulette_below(double w, double floor, uint64_t& rng) {
+  if (floor <= 0.0 || w <= 0.0 || w >= floor) return w;
+  return static_cast<double>(uniform(rng)) < w / floor ? floor : 0.0;
+}
+
+// Move a particle along its ray to the exit plane; kept when forward (local w), at or
+// above the cutoff, with positive weight and the plane ahead (head.py:563-583).
+__device__ inline int exit_record(const DevPresolve& P, PresolveRecord& r, int kind, double e, const double* s,
+                                  float dx, float dy, float dz, double w, double cutoff) {
+  r.kind = -1;
+  if (!(w > 0.0) || !(e >= cutoff)) return 0;
+  const double d_w = P.b[6] * dx + P.b[7] * dy + P.b[8] * dz;
+  if (!(d_w > 0.0)) return 0;
+  const double p_w = P.b[6] * (s[0] - P.o[0]) + P.b[7] * (s[1] - P.o[1]) + P.b[8] * (s[2] - P.o[2]);
+  const double t = (P.exit_z - p_w) / d_w;
+  if (!(t >= 0.0)) return 0;
+  r.e = static_cast<float>(e);
+  r.x = static_cast<float>(s[0] + t * dx);
+  r.y = static_cast<float>(s[1] + t * dy);
+  r.z = static_cast<float>(s[2] + t * dz);
+  r.ux = dx;
+  r.uy = dy;
+  r.uz = dz;
+  r.w = static_cast<float>(w);
+  r.kind = kind;
+  return 1;
+}
+
+__device__ inline void to_beam(const DevPresolve& P, const double* q, const double* d, double* pl, double* dl) {
+  const double r[3] = {q[0] - P.o[0], q[1] - P.o[1], q[2] - P.o[2]};
+  for (int i = 0; i < 3; ++i) {
+    pl[i] = P.b[3 * i] * r[0] + P.b[3 * i + 1] * r[1] + P.b[3 * i + 2] * r[2];
+    dl[i] = P.b[3 * i] * d[0] + P.b[3 * i + 1] * d[1] + P.b[3 * i + 2] * d[2];
+  }
+}
+
+// One lane per primary (pyradmc/backends/warp/presolve.py:356-541). Records go to the
+// primary's fixed slots; the block's kept count goes to block_counts for the
+// deterministic compaction that follows.
+__global__ void __launch_bounds__(256) k_presolve(DevRayWeights G, DevPresolve P, long long n, uint64_t seed,
+                                                  const float* energy, const float* px, const float* py,
+                                                  const float* pz, const float* pux, const float* puy,
+                                                  const float* puz, const float* pw, PresolveRecord* out,
+                                                  long long* block_counts) {
+  __shared__ int kept_block;
+  if (threadIdx.x == 0) kept_block = 0;
+  __syncthreads();
+  const long long h = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int kept = 0;
+  if (h < n) {
+    PresolveRecord* rec = out + kPresolveSlots * h;
+    uint64_t rng = rng_init(seed ^ kPresolveSalt, static_cast<uint64_t>(h));
+    const double e = energy[h], w0 = pw[h];
+    const double p[3] = {px[h], py[h], pz[h]};
+    const double d[3] = {pux[h], puy[h], puz[h]};
+    double pl[3], dl[3];
+    to_beam(P, p, d, pl, dl);
+    double tau[kPresolveMaxDevices];
+    double tau_total = 0.0;
+    for (int j = 0; j < G.n_dev; ++j) {
+      const double t = device_chord(G, j, pl, dl, false);
+      tau[j] = t > 0.0 ? presolve_mu(P, P.log_tot, j, e) * t : 0.0;
+      tau_total += tau[j];
+    }
+    double transmitted = w0 * exp(-tau_total);
+    rec[0].kind = rec[1].kind = rec[2].kind = -1;
+    if (P.first_compton && tau_total > 0.0 && isfinite(tau_total) && dl[2] != 0.0) {
+      // the device by its share of the optical depth (one the ray crosses)
+      const double target = static_cast<double>(uniform(rng)) * tau_total;
+      double cum = 0.0;
+      int jd = -1;
+      for (int j = 0; j < G.n_dev; ++j) {
+        cum += tau[j];
+        if (jd < 0 && tau[j] > 0.0 && cum >= target) jd = j;
+      }
+      if (jd < 0) jd = G.n_dev - 1;
+      const double cf = presolve_mu(P, P.log_com, jd, e) / presolve_mu(P, P.log_tot, jd, e);
+      const double t_mid = (0.5 * (G.z_top[jd] + G.z_bot[jd]) - pl[2]) / dl[2];
+      const double site[3] = {p[0] + t_mid * d[0], p[1] + t_mid * d[1], p[2] + t_mid * d[2]};
+      const float ratio = sample_compton_ratio(static_cast<float>(e), rng);
+      const double e_out = static_cast<double>(ratio) * e;
+      float ox = static_cast<float>(d[0]), oy = static_cast<float>(d[1]), oz = static_cast<float>(d[2]);
+      rotate(ox, oy, oz, compton_cos(static_cast<float>(e), ratio), kTwoPi * uniform(rng));
+      const double dout[3] = {ox, oy, oz};
+      double sl[3], sdl[3];
+      to_beam(P, site, dout, sl, sdl);
+      double tau_out = 0.0;
+      for (int j = 0; j < G.n_dev; ++j) {
+        const double t = device_chord(G, j, sl, sdl, true);
+        if (t > 0.0) tau_out += presolve_mu(P, P.log_tot, j, e_out) * t;
+      }
+      const double ws = roulette_below(w0 * (1.0 - exp(-tau_total)) * cf * exp(-tau_out), P.floor, rng);
+      kept += exit_record(P, rec[0], PHOTON, e_out, site, ox, oy, oz, ws, P.pcut);
+    }
+    if (P.has_air) {
+      const double length = dl[2] > 0.0 ? (P.air_z1 - P.air_z0) / dl[2] : 0.0;
+      const double mu_air = presolve_mu(P, P.log_tot, P.air_row, e);
+      const double tau_air = mu_air * length;
+      const double before = transmitted;
+      transmitted *= exp(-tau_air);
+      if (P.first_compton && dl[2] > 0.0 && tau_air > 0.0) {
+        const double pair_w = before * (1.0 - exp(-tau_air)) * presolve_mu(P, P.log_com, P.air_row, e) / mu_air;
+        const double u = uniform(rng);
+        const double depth = -log(1.0 - u * (1.0 - exp(-tau_air))) / mu_air;
+        const double ts = (P.air_z0 - pl[2]) / dl[2] + depth;
+        const double site[3] = {p[0] + ts * d[0], p[1] + ts * d[1], p[2] + ts * d[2]};
+        const float fe = static_cast<float>(e);
+        const float ratio = sample_compton_ratio(fe, rng);
+        const double e_ph = static_cast<double>(ratio) * e;
+        const float phi = kTwoPi * uniform(rng);
+        float ax = static_cast<float>(d[0]), ay = static_cast<float>(d[1]), az = static_cast<float>(d[2]);
+        float bx = ax, by = ay, bz = az;
+        rotate(ax, ay, az, compton_cos(fe, ratio), phi);
+        rotate(bx, by, bz, compton_electron_cos(fe, ratio), phi + kPi);
+        kept += exit_record(P, rec[1], PHOTON, e_ph, site, ax, ay, az, roulette_below(pair_w, P.floor, rng), P.pcut);
+        kept += exit_record(P, rec[2], ELECTRON, e - e_ph, site, bx, by, bz, roulette_below(pair_w, P.floor, rng),
+                            P.ecut);
+      }
+    }
+    kept += exit_record(P, rec[3], PHOTON, e, p, static_cast<float>(d[0]), static_cast<float>(d[1]),
+                        static_cast<float>(d[2]), transmitted, P.pcut);
+  }
+  if (kept) atomicAdd(&kept_block, kept);
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = kept_block;
+}
+
+// Kept records of each block, in (primary, slot) order, from the block's offset on;
+// weights scaled per primary. Same grid as k_presolve.
+__global__ void __launch_bounds__(256) k_presolve_compact(const PresolveRecord* rec, long long n,
+                                                          const long long* block_offsets, int32_t* out_kind,
+                                                          float* out, long long pop_n, float scale) {
+  __shared__ int scan[256];
+  const int t = threadIdx.x;
+  const long long h = static_cast<long long>(blockIdx.x) * blockDim.x + t;
+  int mine = 0;
+  if (h < n)
+    for (int k = 0; k < kPresolveSlots; ++k) mine += rec[kPresolveSlots * h + k].kind >= 0;
+  scan[t] = mine;
+  __syncthreads();
+  for (int step = 1; step < 256; step <<= 1) {  // inclusive Hillis-Steele scan
+    const int add = t >= step ? scan[t - step] : 0;
+    __syncthreads();
+    scan[t] += add;
+    __syncthreads();
+  }
+  if (h >= n || mine == 0) return;
+  long long j = block_offsets[blockIdx.x] + scan[t] - mine;
+  for (int k = 0; k < kPresolveSlots; ++k) {
+    const PresolveRecord& r = rec[kPresolveSlots * h + k];
+    if (r.kind < 0) continue;
+    out_kind[j] = r.kind;
+    const float cols[8] = {r.e, r.x, r.y, r.z, r.ux, r.uy, r.uz, r.w * scale};
+    for (int c = 0; c < 8; ++c) out[c * pop_n + j] = cols[c];
+    ++j;
+  }
+}
+
+int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const float* energy, const float* x, const float* y,
+                      const float* z, const float* ux, const float* uy, const float* uz, const float* weight,
+                      const hmc_ray_weights* stack, const hmc_presolve* q, int64_t* n_out) {
+  if (!ctx || !q || !n_out || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
+    return fail(ctx, HMC_EINVAL, "null");
+  if (n < 1 || n > (static_cast<int64_t>(1) << 31) - 1) return fail(ctx, HMC_EINVAL, "primaries out of range");
+  const int n_dev = stack && stack->kind == 1 ? stack->n_devices : 0;
+  if (stack && stack->kind != 0 && stack->kind != 1) return fail(ctx, HMC_EINVAL, "the stack must be kind 1");
+  if (n_dev > kPresolveMaxDevices) return fail(ctx, HMC_EINVAL, "at most 16 devices in a pre-solve stack");
+  const int rows = n_dev + (q->has_air ? 1 : 0);
+  if (q->n_rows != rows) return fail(ctx, HMC_EINVAL, "table rows must be the devices then the air column");
+  if (rows > 0 && (q->n_table < 2 || !q->log_energies || !q->log_mu_total || !q->log_mu_compton))
+    return fail(ctx, HMC_EINVAL, "attenuation tables incomplete");
+  for (int k = 1; k < q->n_table && rows > 0; ++k)
+    if (!(q->log_energies[k] > q->log_energies[k - 1])) return fail(ctx, HMC_EINVAL, "energies must increase");
+  if (q->has_air && !(q->air_z_end > q->air_z_start)) return fail(ctx, HMC_EINVAL, "empty air column");
+  // Kahn's rejection accepts only for a positive finite energy
+  for (int64_t i = 0; i < n; ++i)
+    if (!(energy[i] > 0.0f) || !std::isfinite(energy[i]) || !std::isfinite(weight[i]) || !(weight[i] >= 0.0f))
+      return fail(ctx, HMC_EINVAL, "pre-solve primaries need positive finite energies and weights");
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<void*> bufs;
+  auto release = [&]() {
+    for (void* p : bufs) free_ptr(p);
+    bufs.clear();
+  };
+  auto take = [&](auto** ptr, const auto* host, size_t count) -> int {
+    const int code = upload(ctx, ptr, host, count);
+    if (*ptr) {
+      bufs.push_back(*ptr);
+      ctx->alloc_bytes.erase(*ptr);
+    }
+    return code;
+  };
+  DevRayWeights G{};
+  int rc = HMC_OK;
+  if (n_dev > 0 && (rc = build_ray_weights(ctx, stack, false, G, bufs))) {
+    release();
+    return rc;
+  }
+  DevPresolve P{};
+  P.first_compton = q->first_compton ? 1 : 0;
+  P.has_air = q->has_air ? 1 : 0;
+  P.air_row = n_dev;
+  P.exit_z = q->exit_z;
+  P.air_z0 = q->air_z_start;
+  P.air_z1 = q->air_z_end;
+  P.pcut = q->pcut;
+  P.ecut = q->ecut;
+  P.floor = q->weight_floor;
+  for (int k = 0; k < 3; ++k) P.o[k] = q->origin[k];
+  for (int k = 0; k < 9; ++k) P.b[k] = q->basis[k];
+  double *d_le = nullptr, *d_tot = nullptr, *d_com = nullptr;
+  if (rows > 0) {
+    P.n_table = q->n_table;
+    const size_t cells = static_cast<size_t>(rows) * q->n_table;
+    if ((rc = take(&d_le, q->log_energies, q->n_table)) || (rc = take(&d_tot, q->log_mu_total, cells)) ||
+        (rc = take(&d_com, q->log_mu_compton, cells))) {
+      release();
+      return rc;
+    }
+    P.log_e = d_le;
+    P.log_tot = d_tot;
+    P.log_com = d_com;
+    P.le0 = q->log_energies[0];
+    P.inv_dle = (q->n_table - 1) / (q->log_energies[q->n_table - 1] - q->log_energies[0]);
+  }
+  const float* host_cols[8] = {energy, x, y, z, ux, uy, uz, weight};
+  float* dev_cols[8] = {};
+  for (int c = 0; c < 8; ++c)
+    if ((rc = take(&dev_cols[c], host_cols[c], static_cast<size_t>(n)))) {
+      release();
+      return rc;
+    }
+  const long long blocks = (n + 255) / 256;
+  PresolveRecord* d_rec = nullptr;
+  long long* d_counts = nullptr;
+  if ((rc = take(&d_rec, static_cast<const PresolveRecord*>(nullptr), static_cast<size_t>(n) * kPresolveSlots)) ||
+      (rc = take(&d_counts, static_cast<const long long*>(nullptr), static_cast<size_t>(blocks)))) {
+    release();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_presolve, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream, G, P,
+                     static_cast<long long>(n), seed, dev_cols[0], dev_cols[1], dev_cols[2], dev_cols[3],
+                     dev_cols[4], dev_cols[5], dev_cols[6], dev_cols[7], d_rec, d_counts);
+  hipError_t e = hipGetLastError();
+  std::vector<long long> offsets(static_cast<size_t>(blocks));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(offsets.data(), d_counts, sizeof(long long) * blocks, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    release();
+    return fail(ctx, HMC_EHIP, hipGetErrorString(e));
+  }
+  long long total = 0;
+  for (auto& c : offsets) {
+    const long long k = c;
+    c = total;
+    total += k;
+  }
+  if (total == 0) {
+    release();
+    return fail(ctx, HMC_EINVAL, "the pre-solve kept no particles; check the geometry and cutoffs");
+  }
+  // the population replaces the resident one
+  ctx->pop_n = ctx->pop_written = 0;
+  if (ctx->src.kind == HMC_SRC_RESIDENT) ctx->src.up_n = 0;
+  if ((rc = dev_alloc(ctx, &ctx->d_pop_particle, static_cast<size_t>(total))) ||
+      (rc = dev_alloc(ctx, &ctx->d_pop, static_cast<size_t>(total) * 8))) {
+    release();
+    return rc;
+  }
+  long long* d_offsets = nullptr;
+  if ((rc = take(&d_offsets, offsets.data(), offsets.size()))) {
+    release();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_presolve_compact, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream, d_rec,
+                     static_cast<long long>(n), d_offsets, ctx->d_pop_particle, ctx->d_pop, total,
+                     static_cast<float>(static_cast<double>(total) / static_cast<double>(n)));
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  release();
+  if (e != hipSuccess) return fail(ctx, HMC_EHIP, hipGetErrorString(e));
+  ctx->pop_n = ctx->pop_written = total;
+  *n_out = total;
+  return HMC_OK;
+}
+
+int hmc_get_population(hmc_ctx* ctx, int64_t offset, int64_t count, int32_t* particle, float* energy, float* x,
+                       float* y, float* z, float* ux, float* uy, float* uz, float* weight) {
+  if (!ctx || !particle || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
+    return fail(ctx, HMC_EINVAL, "null");
+  if (count < 0 || offset < 0 || offset > ctx->pop_written - count)
+    return fail(ctx, HMC_EINVAL, "population rows out of range");
+  if (count == 0) return HMC_OK;
+  HMC_HIP(ctx, hipSetDevice(ctx->device));
+  HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  HMC_HIP(ctx, hipMemcpy(particle, ctx->d_pop_particle + offset, sizeof(int32_t) * count, hipMemcpyDeviceToHost));
+  float* cols[8] = {energy, x, y, z, ux, uy, uz, weight};
+  for (int c = 0; c < 8; ++c)
+    HMC_HIP(ctx, hipMemcpy(cols[c], ctx->d_pop + c * ctx->pop_n + offset, sizeof(float) * count,
+                           hipMemcpyDeviceToHost));
+  return HMC_OK;
 }
 
 int hmc_set_physics(hmc_ctx* ctx, const hmc_physics* p) {
