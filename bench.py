@@ -120,6 +120,81 @@ def _pencil_leg(repeats: int = 7) -> dict:
     return out
 
 
+def _head_leg(n_presolve: int = 8_000_000, histories: int = 8_000_000, repeats: int = 3) -> dict:
+    """The treatment head on one field: a 6 MV-like Gaussian-spot source through two jaw
+    pairs and a 40-pair rounded-tip MLC onto 64^3 water at 5 mm. Times (median of
+    ``repeats``, whole calls on the host clock): the device pre-solve (forced first
+    Compton, air column) with its primaries drawn in the kernel, and with them sampled on
+    the host; transport of its exit population left in HBM; the same field as a
+    collimated in-kernel source and open; and, for contrast, the copied-back population
+    on the host-sampled route."""
+    import statistics
+    import warnings
+
+    from pyradmc_amd import (MLC, AirColumn, AnalyticCrossSections, BeamFrame, BeamLimitingStack,
+                             CollimatedSource, GaussianSpotBeamSource, HipEngine, JawPair, VoxelGrid,
+                             bremsstrahlung_6mv_like, presolve_head)
+
+    grid = VoxelGrid.uniform_water((64, 64, 64), (0.5, 0.5, 0.5))
+    xs = AnalyticCrossSections(geometry_densities=grid.max_density_by_material())
+    focal = (16.0, 16.0, -100.0)
+    leaves = np.linspace(-2.0, 2.0, 41)
+    mid = 0.5 * (leaves[:-1] + leaves[1:])
+    # the analytic source is water only: the devices are water (material 0) at tungsten's density
+    stack = BeamLimitingStack(BeamFrame(focal), (
+        JawPair("v", 20.0, 27.8, -2.2, 2.2, 0, 19.3, focused=True),
+        JawPair("u", 28.0, 35.8, -2.2, 2.2, 0, 19.3, focused=True),
+        MLC(40.0, 46.5, tuple(leaves), tuple(-1.2 - 0.6 * np.cos(mid)), tuple(0.9 + 0.7 * np.sin(mid) ** 2), 8.0,
+            0, 18.0, focused_sides=True),
+    ))
+    src = GaussianSpotBeamSource(bremsstrahlung_6mv_like(), focal, (16.0, 16.0, -85.0), 3.0, 3.0, 0.15, 0.15)
+    air = AirColumn(50.0, 95.0, material=0)  # water at air's density (the analytic source is water only)
+    engine = HipEngine(grid, xs)
+
+    def median_wall(fn):
+        walls = []
+        for _ in range(repeats):
+            started = time.perf_counter()
+            fn()
+            walls.append(time.perf_counter() - started)
+        return statistics.median(walls)
+
+    class HostSampledSpot(GaussianSpotBeamSource):
+        """A subclass: the pre-solve samples its primaries on the host."""
+
+    host_src = HostSampledSpot(bremsstrahlung_6mv_like(), focal, (16.0, 16.0, -85.0), 3.0, 3.0, 0.15, 0.15)
+    kw = dict(cross_sections=xs, exit_z=97.0, stack=stack, air=air, engine=engine, resident=True)
+    presolve_head(src, n_histories=100_000, seed=1, **kw)  # warm
+    presolve_wall = median_wall(lambda: presolve_head(src, n_histories=n_presolve, seed=2, **kw))
+    host_presolve_wall = median_wall(lambda: presolve_head(host_src, n_histories=n_presolve, seed=2, **kw))
+    population = presolve_head(src, n_histories=n_presolve, seed=3, **kw)
+    engine.run(population, 1_000_000, 10, seed=4)  # warm
+    resident = median_wall(lambda: engine.run(population, histories, 10, seed=5))
+    collimated_src = CollimatedSource(src, stack, xs)
+    engine.run(collimated_src, 1_000_000, 10, seed=6)
+    collimated = median_wall(lambda: engine.run(collimated_src, histories, 10, seed=7))
+    engine.run(src, 1_000_000, 10, seed=8)
+    open_field = median_wall(lambda: engine.run(src, histories, 10, seed=9))
+    host_n = 2_000_000
+    in_memory = population.to_phase_space()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        host = median_wall(lambda: engine.run(in_memory, host_n, 10, seed=10))
+    engine.close()
+    return {
+        "workload": "6 MV-like Gaussian spot (0.15 cm) through two focused jaw pairs and a 40-pair "
+                    "rounded-tip MLC (water at tungsten density), air column; 64^3 water at 5 mm",
+        "presolve": {"primaries": n_presolve, "exit_particles": len(population), "wall_s": round(presolve_wall, 4),
+                     "primaries_per_s": round(n_presolve / presolve_wall, 1),
+                     "host_sampled_primaries_wall_s": round(host_presolve_wall, 4)},
+        "histories_per_s": {"resident_phase_space": round(histories / resident, 1),
+                            "collimated_in_kernel": round(histories / collimated, 1),
+                            "open_spot_field": round(histories / open_field, 1),
+                            "host_sampled_phase_space": round(host_n / host, 1)},
+        "histories": histories,
+    }
+
+
 def _dij_leg(median: float, launch: dict | None = None) -> dict:
     """The batched uneven Dij: 128x128x96 CT-like phantom at 2.5 mm, 100 spectral
     beamlets of 5x5 mm from 100 cm SAD, lognormal per-beamlet counts, truncation 1e-4,
@@ -471,6 +546,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pencil", action="store_true", help="skip the quick-start pencil leg")
     ap.add_argument("--no-dij", action="store_true", help="skip the batched Dij leg")
+    ap.add_argument("--no-head", action="store_true", help="skip the treatment-head leg")
     ap.add_argument("--dij-median", type=float, default=2e5,
                     help="median per-beamlet history count of the Dij leg")
     ap.add_argument("--denoise-model", default=str(ROOT / "pyradmc_amd" / "models" / "denoiser_ct.npz"),
@@ -586,6 +662,7 @@ def main() -> int:
     shape = dict(engine.launch) if (args.launch or args.no_autotune) else None
     pencil = None if args.no_pencil else _guarded("pencil", _pencil_leg)
     dij = None if args.no_dij else _guarded("dij", _dij_leg, args.dij_median, shape)
+    head = None if args.no_head else _guarded("head", _head_leg)
     dij_shape = shape if shape is not None else (dij or {}).get("launch")
     denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), args.denoise_low,
                          args.denoise_audit, 0.02, dij_shape)
@@ -637,6 +714,7 @@ def main() -> int:
         "dij": dij,
         "denoised_dij": denoised,
         "full_plan": plan,
+        "head": head,
         "events_per_history": _events(counters, total),
     }
     engine.close()

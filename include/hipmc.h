@@ -200,12 +200,15 @@ typedef struct hmc_presolve {
   const double *log_mu_total;   /* [n_rows][n_table] log of the macroscopic mu (1/cm) */
   const double *log_mu_compton; /* [n_rows][n_table] */
 } hmc_presolve;
-/* n photon primaries (sample_batch columns); stack: a kind-1 model (its tables unused)
- * or NULL; *n_out receives the population size. At most 16 devices. */
-int hmc_presolve_head(hmc_ctx *ctx, uint64_t seed, int64_t n, const float *energy, const float *x,
-                      const float *y, const float *z, const float *ux, const float *uy,
-                      const float *uz, const float *weight, const hmc_ray_weights *stack,
-                      const hmc_presolve *presolve, int64_t *n_out);
+/* n photon primaries: drawn in-kernel from generator (an open-field photon source:
+ * pencil, parallel, spectral parallel, or a one-aperture spectral or spot fan; it
+ * becomes the context's source) on history h's stream keyed (seed, h), or, with
+ * generator NULL, given as columns (sample_batch); stack: a kind-1 model (its tables
+ * unused) or NULL; *n_out receives the population size. At most 16 devices. */
+int hmc_presolve_head(hmc_ctx *ctx, uint64_t seed, int64_t n, const hmc_source *generator,
+                      const float *energy, const float *x, const float *y, const float *z,
+                      const float *ux, const float *uy, const float *uz, const float *weight,
+                      const hmc_ray_weights *stack, const hmc_presolve *presolve, int64_t *n_out);
 /* Rows [offset, offset + count) of the resident population (kinds 0 photon, 1 electron,
  * 2 positron) into host columns. */
 int hmc_get_population(hmc_ctx *ctx, int64_t offset, int64_t count, int32_t *particle, float *energy,

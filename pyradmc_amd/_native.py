@@ -242,7 +242,7 @@ def lib():
         "hmc_set_uploaded": (c_int, [ctx_p, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_population": (c_int, [ctx_p, c_int64, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_set_ray_weights": (c_int, [ctx_p, POINTER(RayWeightsDesc)]),
-        "hmc_presolve_head": (c_int, [ctx_p, c_uint64, c_int64] + [_f32p] * 8
+        "hmc_presolve_head": (c_int, [ctx_p, c_uint64, c_int64, POINTER(SourceDesc)] + [_f32p] * 8
                               + [POINTER(RayWeightsDesc), POINTER(PresolveDesc), POINTER(c_int64)]),
         "hmc_get_population": (c_int, [ctx_p, c_int64, c_int64, POINTER(c_int32)] + [_f32p] * 8),
         "hmc_ray_weight_factors": (c_int, [ctx_p, c_int64, _f32p, _f32p, _f32p, _f64p]),

@@ -870,6 +870,109 @@ __device__ inline float sample_spectrum(const DevSource& S, uint64_t& rng) {
 }
 
 
+// One primary of the source from the history's stream rng (the source's draws come
+// first on it): beamlet indexes per-beamlet parameters, g is the flat primary index of
+// host-sampled primaries. Mono sources leave book false: their emitted energy is
+// analytic (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
+struct Primary {
+  float e, x, y, z, ux, uy, uz, w;
+  uint32_t kind;
+  bool book;
+};
+
+__device__ inline Primary make_primary(const DevSource& S, int beamlet, long long g, uint64_t& rng) {
+  float e = S.energy, w = 1.0f;
+  float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
+  bool book = false;
+  uint32_t kind = static_cast<uint32_t>(S.particle);
+  switch (S.kind) {
+    case HMC_SRC_PENCIL:
+      x = S.px; y = S.py; z = S.pz;
+      ux = S.dx; uy = S.dy; uz = S.dz;
+      break;
+    case HMC_SRC_PARALLEL:
+      x = S.x0 + S.xe * uniform(rng);
+      y = S.y0 + S.ye * uniform(rng);
+      z = S.z0;
+      break;
+    case HMC_SRC_PARALLEL_SPECTRAL:
+      e = sample_spectrum(S, rng);
+      x = S.x0 + S.xe * uniform(rng);
+      y = S.y0 + S.ye * uniform(rng);
+      z = S.z0;
+      kind = PHOTON;
+      book = true;
+      break;
+    case HMC_SRC_LATTICE: {
+      const float* b = S.beamlet + 4 * static_cast<long long>(beamlet);
+      x = b[0] + b[1] * uniform(rng);
+      y = b[2] + b[3] * uniform(rng);
+      z = S.z0;
+      kind = PHOTON;
+      break;
+    }
+    case HMC_SRC_SPECTRAL_FAN: {
+      e = sample_spectrum(S, rng);
+      const float du = (uniform(rng) - 0.5f) * S.width_u;
+      const float dv = (uniform(rng) - 0.5f) * S.width_v;
+      const float* c = S.beamlet + 3 * static_cast<long long>(beamlet);
+      const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
+      const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
+      const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
+      const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+      x = S.fx; y = S.fy; z = S.fz;
+      ux = dx * inv; uy = dy * inv; uz = dz * inv;
+      kind = PHOTON;
+      book = true;
+      break;
+    }
+    case HMC_SRC_SPOT_FAN: {
+      // Six uniforms: energy (two), rectangle offset (two), Box-Muller spot (two);
+      // the primary starts on the rectangle, aimed from the sampled spot point
+      // (pyradmc/geometry/source.py:760-792).
+      e = sample_spectrum(S, rng);
+      const float du = (uniform(rng) - 0.5f) * S.width_u;
+      const float dv = (uniform(rng) - 0.5f) * S.width_v;
+      const float radius = sqrtf(-2.0f * logf(fmaxf(uniform(rng), 1.0e-12f)));
+      const float angle = kTwoPi * uniform(rng);
+      const float su = S.sigma_u * radius * cosf(angle), sv = S.sigma_v * radius * sinf(angle);
+      const float* c = S.beamlet + 3 * static_cast<long long>(beamlet);
+      x = c[0] + du * S.uax + dv * S.vax;
+      y = c[1] + du * S.uay + dv * S.vay;
+      z = c[2] + du * S.uaz + dv * S.vaz;
+      const float dx = x - (S.fx + su * S.uax + sv * S.vax);
+      const float dy = y - (S.fy + su * S.uay + sv * S.vay);
+      const float dz = z - (S.fz + su * S.uaz + sv * S.vaz);
+      const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+      ux = dx * inv; uy = dy * inv; uz = dz * inv;
+      kind = PHOTON;
+      book = true;
+      break;
+    }
+    default: {  // host-sampled primaries by flat primary index, or a record of the
+                // resident population drawn with 64 bits of the history's stream
+      long long j = g;
+      if (S.kind == HMC_SRC_RESIDENT) {
+        const uint64_t hi = rng_next(rng);
+        const uint64_t u = (hi << 32) | rng_next(rng);
+        j = static_cast<long long>(__umul64hi(u, static_cast<unsigned long long>(S.up_n)));
+      }
+      kind = static_cast<uint32_t>(S.up_particle[j]);
+      e = S.up[j];
+      x = S.up[S.up_n + j];
+      y = S.up[2 * S.up_n + j];
+      z = S.up[3 * S.up_n + j];
+      ux = S.up[4 * S.up_n + j];
+      uy = S.up[5 * S.up_n + j];
+      uz = S.up[6 * S.up_n + j];
+      w = S.up[7 * S.up_n + j];
+      book = true;
+      break;
+    }
+  }
+  return Primary{e, x, y, z, ux, uy, uz, w, kind, book};
+}
+
 // Primaries [p0, p0 + n_gen) of the flattened work items, grid-stride (at most a few
 // thousand blocks). Single-kind sources write fixed slots of the target queue
 // (primary t -> slot t; the count is set once); host-sampled mixed-kind primaries
@@ -907,94 +1010,10 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
       const long long i = g - item.start;
       const uint32_t column = static_cast<uint32_t>(item.column);
       uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
-      float e = S.energy, w = 1.0f;
-      float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1;
-      bool book = false;
-      switch (S.kind) {
-        case HMC_SRC_PENCIL:
-          x = S.px; y = S.py; z = S.pz;
-          ux = S.dx; uy = S.dy; uz = S.dz;
-          break;
-        case HMC_SRC_PARALLEL:
-          x = S.x0 + S.xe * uniform(rng);
-          y = S.y0 + S.ye * uniform(rng);
-          z = S.z0;
-          break;
-        case HMC_SRC_PARALLEL_SPECTRAL:
-          e = sample_spectrum(S, rng);
-          x = S.x0 + S.xe * uniform(rng);
-          y = S.y0 + S.ye * uniform(rng);
-          z = S.z0;
-          kind = PHOTON;
-          book = true;
-          break;
-        case HMC_SRC_LATTICE: {
-          const float* b = S.beamlet + 4 * static_cast<long long>(item.beamlet);
-          x = b[0] + b[1] * uniform(rng);
-          y = b[2] + b[3] * uniform(rng);
-          z = S.z0;
-          kind = PHOTON;
-          break;
-        }
-        case HMC_SRC_SPECTRAL_FAN: {
-          e = sample_spectrum(S, rng);
-          const float du = (uniform(rng) - 0.5f) * S.width_u;
-          const float dv = (uniform(rng) - 0.5f) * S.width_v;
-          const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
-          const float dx = c[0] + du * S.uax + dv * S.vax - S.fx;
-          const float dy = c[1] + du * S.uay + dv * S.vay - S.fy;
-          const float dz = c[2] + du * S.uaz + dv * S.vaz - S.fz;
-          const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-          x = S.fx; y = S.fy; z = S.fz;
-          ux = dx * inv; uy = dy * inv; uz = dz * inv;
-          kind = PHOTON;
-          book = true;
-          break;
-        }
-        case HMC_SRC_SPOT_FAN: {
-          // Six uniforms: energy (two), rectangle offset (two), Box-Muller spot (two);
-          // the primary starts on the rectangle, aimed from the sampled spot point
-          // (pyradmc/geometry/source.py:760-792).
-          e = sample_spectrum(S, rng);
-          const float du = (uniform(rng) - 0.5f) * S.width_u;
-          const float dv = (uniform(rng) - 0.5f) * S.width_v;
-          const float radius = sqrtf(-2.0f * logf(fmaxf(uniform(rng), 1.0e-12f)));
-          const float angle = kTwoPi * uniform(rng);
-          const float su = S.sigma_u * radius * cosf(angle), sv = S.sigma_v * radius * sinf(angle);
-          const float* c = S.beamlet + 3 * static_cast<long long>(item.beamlet);
-          x = c[0] + du * S.uax + dv * S.vax;
-          y = c[1] + du * S.uay + dv * S.vay;
-          z = c[2] + du * S.uaz + dv * S.vaz;
-          const float dx = x - (S.fx + su * S.uax + sv * S.vax);
-          const float dy = y - (S.fy + su * S.uay + sv * S.vay);
-          const float dz = z - (S.fz + su * S.uaz + sv * S.vaz);
-          const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-          ux = dx * inv; uy = dy * inv; uz = dz * inv;
-          kind = PHOTON;
-          book = true;
-          break;
-        }
-        default: {  // host-sampled primaries by flat primary index, or a record of the
-                    // resident population drawn with 64 bits of the history's stream
-          long long j = g;
-          if (S.kind == HMC_SRC_RESIDENT) {
-            const uint64_t hi = rng_next(rng);
-            const uint64_t u = (hi << 32) | rng_next(rng);
-            j = static_cast<long long>(__umul64hi(u, static_cast<unsigned long long>(S.up_n)));
-          }
-          kind = static_cast<uint32_t>(S.up_particle[j]);
-          e = S.up[j];
-          x = S.up[S.up_n + j];
-          y = S.up[2 * S.up_n + j];
-          z = S.up[3 * S.up_n + j];
-          ux = S.up[4 * S.up_n + j];
-          uy = S.up[5 * S.up_n + j];
-          uz = S.up[6 * S.up_n + j];
-          w = S.up[7 * S.up_n + j];
-          book = true;
-          break;
-        }
-      }
+      Primary q = make_primary(S, item.beamlet, g, rng);
+      float e = q.e, w = q.w, x = q.x, y = q.y, z = q.z, ux = q.ux, uy = q.uy, uz = q.uz;
+      bool book = q.book;
+      kind = q.kind;
       if (RW.kind != 0 && S.kind != HMC_SRC_UPLOADED) {
         // collimator or mask factor of the primary's ray (no random number drawn)
         w *= static_cast<float>(ray_weight(RW, e, x, y, z, ux, uy, uz));
@@ -2175,11 +2194,11 @@ __device__ inline void to_beam(const DevPresolve& P, const double* q, const doub
 // One lane per primary (pyradmc/backends/warp/presolve.py:356-541). Records go to the
 // primary's fixed slots; the block's kept count goes to block_counts for the
 // deterministic compaction that follows.
-__global__ void __launch_bounds__(256) k_presolve(DevRayWeights G, DevPresolve P, long long n, uint64_t seed,
-                                                  const float* energy, const float* px, const float* py,
-                                                  const float* pz, const float* pux, const float* puy,
-                                                  const float* puz, const float* pw, PresolveRecord* out,
-                                                  long long* block_counts) {
+__global__ void __launch_bounds__(256) k_presolve(DevRayWeights G, DevPresolve P, DevSource S, int generate,
+                                                  long long n, uint64_t seed, const float* energy,
+                                                  const float* px, const float* py, const float* pz,
+                                                  const float* pux, const float* puy, const float* puz,
+                                                  const float* pw, PresolveRecord* out, long long* block_counts) {
   __shared__ int kept_block;
   if (threadIdx.x == 0) kept_block = 0;
   __syncthreads();
@@ -2187,10 +2206,22 @@ __global__ void __launch_bounds__(256) k_presolve(DevRayWeights G, DevPresolve P
   int kept = 0;
   if (h < n) {
     PresolveRecord* rec = out + kPresolveSlots * h;
-    uint64_t rng = rng_init(seed ^ kPresolveSalt, static_cast<uint64_t>(h));
-    const double e = energy[h], w0 = pw[h];
-    const double p[3] = {px[h], py[h], pz[h]};
-    const double d[3] = {pux[h], puy[h], puz[h]};
+    uint64_t rng;
+    double e, w0, p[3], d[3];
+    if (generate) {  // the source's draws first, then the head's, on the history's stream
+      rng = rng_init(seed, static_cast<uint64_t>(h));
+      const Primary q = make_primary(S, 0, h, rng);
+      e = q.e;
+      w0 = q.w;
+      p[0] = q.x; p[1] = q.y; p[2] = q.z;
+      d[0] = q.ux; d[1] = q.uy; d[2] = q.uz;
+    } else {
+      rng = rng_init(seed ^ kPresolveSalt, static_cast<uint64_t>(h));
+      e = energy[h];
+      w0 = pw[h];
+      p[0] = px[h]; p[1] = py[h]; p[2] = pz[h];
+      d[0] = pux[h]; d[1] = puy[h]; d[2] = puz[h];
+    }
     double pl[3], dl[3];
     to_beam(P, p, d, pl, dl);
     double tau[kPresolveMaxDevices];
@@ -2294,11 +2325,13 @@ __global__ void __launch_bounds__(256) k_presolve_compact(const PresolveRecord* 
   }
 }
 
-int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const float* energy, const float* x, const float* y,
-                      const float* z, const float* ux, const float* uy, const float* uz, const float* weight,
-                      const hmc_ray_weights* stack, const hmc_presolve* q, int64_t* n_out) {
-  if (!ctx || !q || !n_out || !energy || !x || !y || !z || !ux || !uy || !uz || !weight)
-    return fail(ctx, HMC_EINVAL, "null");
+int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const hmc_source* generator, const float* energy,
+                      const float* x, const float* y, const float* z, const float* ux, const float* uy,
+                      const float* uz, const float* weight, const hmc_ray_weights* stack, const hmc_presolve* q,
+                      int64_t* n_out) {
+  if (!ctx || !q || !n_out) return fail(ctx, HMC_EINVAL, "null");
+  if (!generator && (!energy || !x || !y || !z || !ux || !uy || !uz || !weight))
+    return fail(ctx, HMC_EINVAL, "pre-solve primaries need a generator or all eight columns");
   if (n < 1 || n > (static_cast<int64_t>(1) << 31) - 1) return fail(ctx, HMC_EINVAL, "primaries out of range");
   const int n_dev = stack && stack->kind == 1 ? stack->n_devices : 0;
   if (stack && stack->kind != 0 && stack->kind != 1) return fail(ctx, HMC_EINVAL, "the stack must be kind 1");
@@ -2311,9 +2344,26 @@ int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const float* energ
     if (!(q->log_energies[k] > q->log_energies[k - 1])) return fail(ctx, HMC_EINVAL, "energies must increase");
   if (q->has_air && !(q->air_z_end > q->air_z_start)) return fail(ctx, HMC_EINVAL, "empty air column");
   // Kahn's rejection accepts only for a positive finite energy
-  for (int64_t i = 0; i < n; ++i)
-    if (!(energy[i] > 0.0f) || !std::isfinite(energy[i]) || !std::isfinite(weight[i]) || !(weight[i] >= 0.0f))
-      return fail(ctx, HMC_EINVAL, "pre-solve primaries need positive finite energies and weights");
+  if (generator) {
+    const int k = generator->kind;
+    const bool mono = k == HMC_SRC_PENCIL || k == HMC_SRC_PARALLEL;
+    const bool fan = k == HMC_SRC_SPECTRAL_FAN || k == HMC_SRC_SPOT_FAN;
+    if (!mono && !fan && k != HMC_SRC_PARALLEL_SPECTRAL)
+      return fail(ctx, HMC_EINVAL, "the pre-solve generates open-field photon sources only");
+    if (mono && (generator->particle != 0 || !(generator->energy > 0.0) || !std::isfinite(generator->energy)))
+      return fail(ctx, HMC_EINVAL, "the pre-solve needs photons of positive energy");
+    if (fan && generator->n_beamlets != 1) return fail(ctx, HMC_EINVAL, "the pre-solve takes one aperture");
+    if (!mono && (generator->n_bins < 1 || !generator->edges || !(generator->edges[0] > 0.0)))
+      return fail(ctx, HMC_EINVAL, "the pre-solve needs a spectrum of positive energies");
+  } else {
+    for (int64_t i = 0; i < n; ++i)
+      if (!(energy[i] > 0.0f) || !std::isfinite(energy[i]) || !std::isfinite(weight[i]) || !(weight[i] >= 0.0f))
+        return fail(ctx, HMC_EINVAL, "pre-solve primaries need positive finite energies and weights");
+  }
+  if (generator) {
+    const int src_rc = hmc_set_source(ctx, generator);
+    if (src_rc) return src_rc;
+  }
   HMC_HIP(ctx, hipSetDevice(ctx->device));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<void*> bufs;
@@ -2364,7 +2414,7 @@ int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const float* energ
   }
   const float* host_cols[8] = {energy, x, y, z, ux, uy, uz, weight};
   float* dev_cols[8] = {};
-  for (int c = 0; c < 8; ++c)
+  for (int c = 0; c < 8 && !generator; ++c)
     if ((rc = take(&dev_cols[c], host_cols[c], static_cast<size_t>(n)))) {
       release();
       return rc;
@@ -2377,8 +2427,9 @@ int hmc_presolve_head(hmc_ctx* ctx, uint64_t seed, int64_t n, const float* energ
     release();
     return rc;
   }
-  hipLaunchKernelGGL(k_presolve, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream, G, P,
-                     static_cast<long long>(n), seed, dev_cols[0], dev_cols[1], dev_cols[2], dev_cols[3],
+  hipLaunchKernelGGL(k_presolve, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream, G, P, ctx->src,
+                     generator ? 1 : 0, static_cast<long long>(n), seed, dev_cols[0], dev_cols[1], dev_cols[2],
+                     dev_cols[3],
                      dev_cols[4], dev_cols[5], dev_cols[6], dev_cols[7], d_rec, d_counts);
   hipError_t e = hipGetLastError();
   std::vector<long long> offsets(static_cast<size_t>(blocks));

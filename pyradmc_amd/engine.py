@@ -160,6 +160,85 @@ def _type_name(obj) -> str:
     return type(obj).__name__
 
 
+def generator_desc(source, primary_kind: str = "photon", beamlets: range | None = None):
+    """The in-kernel generator's description of a built-in source (hmc_source):
+    (descriptor, arrays it points into, analytic emitted energy?, mono energy)."""
+    d = nat.SourceDesc()
+    keep: list = []
+    name = _type_name(source)
+    d.particle = {"photon": 0, "electron": 1, "positron": 2}[primary_kind]
+    analytic = False
+    energy = 0.0
+
+    def spectrum_arrays(spectrum):
+        edges = np.ascontiguousarray(np.asarray(spectrum.edges, dtype=np.float64))
+        cdf = np.ascontiguousarray(np.asarray(spectrum.cdf, dtype=np.float64))
+        keep.extend([edges, cdf])
+        d.n_bins = int(cdf.size)
+        d.edges = nat.ptr(edges, nat.c_double)
+        d.cdf = nat.ptr(cdf, nat.c_double)
+
+    if name == "PencilBeamSource":
+        d.kind = nat.SRC_PENCIL
+        d.energy = float(source.energy)
+        d.position[:] = [float(c) for c in source.position]
+        d.direction[:] = [float(c) for c in source.direction]
+        analytic, energy = True, float(source.energy)
+    elif name == "ParallelBeamSource":
+        d.kind = nat.SRC_PARALLEL
+        d.energy = float(source.energy)
+        d.z0 = float(source.z)
+        d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
+        d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
+        analytic, energy = True, float(source.energy)
+    elif name == "SpectralParallelBeamSource":
+        d.kind = nat.SRC_PARALLEL_SPECTRAL
+        d.particle = 0
+        d.z0 = float(source.z)
+        d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
+        d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
+        spectrum_arrays(source.spectrum)
+    elif name in ("SpectralBeamSource", "SpectralBeamletSource", "GaussianSpotBeamSource",
+                  "GaussianSpotBeamletSource"):
+        spot = name.startswith("GaussianSpot")
+        d.kind = nat.SRC_SPOT_FAN if spot else nat.SRC_SPECTRAL_FAN
+        if spot:
+            d.sigma_u, d.sigma_v = float(source.sigma_u), float(source.sigma_v)
+        d.particle = 0
+        spectrum_arrays(source.spectrum)
+        d.focal[:] = [float(c) for c in source.focal_point]
+        d.u_axis[:] = [float(c) for c in source.u_axis]
+        d.v_axis[:] = [float(c) for c in source.v_axis]
+        d.width_u, d.width_v = float(source.width_u), float(source.width_v)
+        centers = source.centers if name.endswith("BeamletSource") else (source.center,)
+        sel = list(beamlets) if beamlets is not None else list(range(len(centers)))
+        params = np.ascontiguousarray(np.array([centers[j] for j in sel], dtype=np.float64).reshape(-1, 3))
+        keep.append(params)
+        d.n_beamlets = int(params.shape[0])
+        d.beamlet_params = nat.ptr(params, nat.c_double)
+    elif name == "BeamletGridSource":
+        d.kind = nat.SRC_LATTICE
+        d.particle = 0
+        d.energy = float(source.energy)
+        d.z0 = float(source.z)
+        sel = list(beamlets) if beamlets is not None else list(range(source.n_beamlets))
+        rows = []
+        for j in sel:
+            x_lo, x_hi, y_lo, y_hi = source.beamlet_bounds(j)
+            rows.append((x_lo, x_hi - x_lo, y_lo, y_hi - y_lo))
+        params = np.ascontiguousarray(np.array(rows, dtype=np.float64))
+        keep.append(params)
+        d.n_beamlets = int(params.shape[0])
+        d.beamlet_params = nat.ptr(params, nat.c_double)
+        analytic, energy = True, float(source.energy)
+    else:
+        raise ValueError(
+            f"source type {name} has no in-kernel generator in this engine; supported: "
+            + ", ".join(sorted(_BUILTIN_SOURCES))
+        )
+    return d, keep, analytic, energy
+
+
 @dataclass
 class HipEngine:
     """One-GPU transport-and-scoring engine over a voxel grid."""
@@ -368,12 +447,7 @@ class HipEngine:
             return False, 0.0  # emitted energy is booked per weighted primary on the device
         self._set_ray_weights(None)
         d = nat.SourceDesc()
-        keep = []
         name = _type_name(source)
-        particle = {"photon": 0, "electron": 1, "positron": 2}[primary_kind]
-        d.particle = particle
-        analytic = False
-        energy = 0.0
         if not _in_kernel(source):
             sampler = "sample_beamlet_batch" if beamlets is not None else "sample_batch"
             if not callable(getattr(source, sampler, None)):
@@ -386,78 +460,15 @@ class HipEngine:
             d.particle = 0
             self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
             return False, 0.0
-
-        def spectrum_arrays(spectrum):
-            edges = np.ascontiguousarray(np.asarray(spectrum.edges, dtype=np.float64))
-            cdf = np.ascontiguousarray(np.asarray(spectrum.cdf, dtype=np.float64))
-            keep.extend([edges, cdf])
-            d.n_bins = int(cdf.size)
-            d.edges = nat.ptr(edges, nat.c_double)
-            d.cdf = nat.ptr(cdf, nat.c_double)
-
-        if name == "PencilBeamSource":
-            d.kind = nat.SRC_PENCIL
-            d.energy = float(source.energy)
-            d.position[:] = [float(c) for c in source.position]
-            d.direction[:] = [float(c) for c in source.direction]
-            analytic, energy = True, float(source.energy)
-        elif name == "ParallelBeamSource":
-            d.kind = nat.SRC_PARALLEL
-            d.energy = float(source.energy)
-            d.z0 = float(source.z)
-            d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
-            d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
-            analytic, energy = True, float(source.energy)
-        elif name == "SpectralParallelBeamSource":
-            d.kind = nat.SRC_PARALLEL_SPECTRAL
-            d.particle = 0
-            d.z0 = float(source.z)
-            d.x0, d.x_extent = float(source.x_range[0]), float(source.x_range[1] - source.x_range[0])
-            d.y0, d.y_extent = float(source.y_range[0]), float(source.y_range[1] - source.y_range[0])
-            spectrum_arrays(source.spectrum)
-        elif name in ("SpectralBeamSource", "SpectralBeamletSource", "GaussianSpotBeamSource",
-                      "GaussianSpotBeamletSource"):
-            spot = name.startswith("GaussianSpot")
-            d.kind = nat.SRC_SPOT_FAN if spot else nat.SRC_SPECTRAL_FAN
-            if spot:
-                d.sigma_u, d.sigma_v = float(source.sigma_u), float(source.sigma_v)
-            d.particle = 0
-            spectrum_arrays(source.spectrum)
-            d.focal[:] = [float(c) for c in source.focal_point]
-            d.u_axis[:] = [float(c) for c in source.u_axis]
-            d.v_axis[:] = [float(c) for c in source.v_axis]
-            d.width_u, d.width_v = float(source.width_u), float(source.width_v)
-            centers = source.centers if name.endswith("BeamletSource") else (source.center,)
-            sel = list(beamlets) if beamlets is not None else list(range(len(centers)))
-            params = np.ascontiguousarray(np.array([centers[j] for j in sel], dtype=np.float64).reshape(-1, 3))
-            keep.append(params)
-            d.n_beamlets = int(params.shape[0])
-            d.beamlet_params = nat.ptr(params, nat.c_double)
-        elif name == "DevicePhaseSpace":
+        if name == "DevicePhaseSpace":
             self._upload_population(source)
             d.kind = nat.SRC_RESIDENT
             d.particle = 0
-        elif name == "BeamletGridSource":
-            d.kind = nat.SRC_LATTICE
-            d.particle = 0
-            d.energy = float(source.energy)
-            d.z0 = float(source.z)
-            sel = list(beamlets) if beamlets is not None else list(range(source.n_beamlets))
-            rows = []
-            for j in sel:
-                x_lo, x_hi, y_lo, y_hi = source.beamlet_bounds(j)
-                rows.append((x_lo, x_hi - x_lo, y_lo, y_hi - y_lo))
-            params = np.ascontiguousarray(np.array(rows, dtype=np.float64))
-            keep.append(params)
-            d.n_beamlets = int(params.shape[0])
-            d.beamlet_params = nat.ptr(params, nat.c_double)
-            analytic, energy = True, float(source.energy)
-        else:
-            raise ValueError(
-                f"source type {name} has no in-kernel generator in this engine; supported: "
-                + ", ".join(sorted(_BUILTIN_SOURCES))
-            )
+            self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
+            return False, 0.0
+        d, keep, analytic, energy = generator_desc(source, primary_kind, beamlets)
         self._ctx.call("hmc_set_source", nat.ctypes.byref(d))
+        del keep  # the arrays only had to outlive the call
         return analytic, energy
 
     def _set_ray_weights(self, model) -> None:

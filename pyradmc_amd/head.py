@@ -24,9 +24,11 @@ play fair roulette. The population is normalised per primary (every weight times
 
 The device kernel gives each primary four fixed record slots (stack photon, air photon,
 air electron, transmitted photon) and a scan compacts the kept ones in that order, so
-the population, and every run drawn from it, is reproducible. Records are drawn from
-the primary's own counter-based stream; the reference's host pre-solve draws from
-numpy's PCG64, so the two agree in distribution, not bit for bit.
+the population, and every run drawn from it, is reproducible. Built-in open-field
+sources are generated in the same kernel (the reference samples every head source on
+the host, pyradmc/backends/warp/presolve.py:615-620), and the head's draws follow the
+source's on each primary's own counter-based stream; the reference's host pre-solve
+draws from numpy's PCG64, so the two agree in distribution, not bit for bit.
 """
 
 from __future__ import annotations
@@ -43,6 +45,9 @@ from pyradmc_amd.phasespace import DevicePhaseSpace, InMemoryPhaseSpaceSource
 __all__ = ["AirColumn", "presolve_head", "resolve_presolve"]
 
 MODES = ("auto", "first_compton", "attenuation")
+# Open-field photon sources whose primaries the pre-solve kernel draws itself.
+_GENERATED = frozenset({"PencilBeamSource", "ParallelBeamSource", "SpectralParallelBeamSource",
+                        "SpectralBeamSource", "GaussianSpotBeamSource"})
 _KIND_TO_IAEA = np.array([1, 2, 3], dtype=np.int32)
 _FETCH_BLOCK = 1 << 22
 
@@ -136,26 +141,37 @@ def presolve_head(source, *, cross_sections, n_histories: int, seed: int, exit_z
     """Pre-solve the head once on the GPU and return the exit-plane phase space
     (pyradmc/geometry/head.py:377-560, pyradmc/backends/warp/presolve.py:572-687).
 
-    ``source`` must have ``sample_batch`` and emit photons only (a
-    ``GaussianSpotBeamSource`` is the intended input); ``exit_z`` and the air column are
+    ``source`` emits photons only (a ``GaussianSpotBeamSource`` is the intended input).
+    The built-in open-field sources (pencil and parallel photon beams, spectral parallel
+    field, spectral and spot fans) are drawn in-kernel on history ``h``'s stream; any
+    other source is sampled on the host through ``sample_batch``. ``exit_z`` and the air column are
     local ``w`` in the beam frame (the stack's; pass ``frame`` without a stack). By
     default the population is copied back as an ``InMemoryPhaseSpaceSource``. With
     ``resident=True`` and an ``engine``, it stays in that engine's device memory and a
     ``DevicePhaseSpace`` is returned, which ``engine.run`` transports with no copy
     (another engine copies it over on first use)."""
-    from pyradmc_amd.engine import _device_index
+    from pyradmc_amd.engine import _device_index, _in_kernel, generator_desc
 
     resolved, frame = resolve_presolve(stack, frame, air, exit_z, mode)
     if resident and engine is None:
         raise ValueError("a resident pre-solve needs the engine that will transport it")
     if n_histories < 1:
         raise ValueError(f"need at least one history, got {n_histories}")
-    cols = source.sample_batch(seed, 0, int(n_histories))
-    if not np.all(np.asarray(cols["particle_type"]) == 1):
-        raise ValueError("the pre-solve transports photon sources only")
     keep = []
+    generator = None
+    if _in_kernel(source) and type(source).__name__ in _GENERATED:
+        # drawn in-kernel: no host sampling, no upload
+        generator, gen_keep, _, _ = generator_desc(source, "photon")
+        keep.extend(gen_keep)
+        cols = None
+    else:
+        cols = source.sample_batch(seed, 0, int(n_histories))
+        if not np.all(np.asarray(cols["particle_type"]) == 1):
+            raise ValueError("the pre-solve transports photon sources only")
 
     def f32(name):
+        if cols is None:
+            return None
         a = np.ascontiguousarray(np.asarray(cols[name], dtype=np.float32).reshape(-1))
         keep.append(a)
         return nat.ptr(a, nat.ctypes.c_float)
@@ -209,6 +225,7 @@ def presolve_head(source, *, cross_sections, n_histories: int, seed: int, exit_z
     try:
         n_out = nat.c_int64(0)
         ctx.call("hmc_presolve_head", nat.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), int(n_histories),
+                 None if generator is None else nat.ctypes.byref(generator),
                  *[f32(c) for c in ("energy", "x", "y", "z", "ux", "uy", "uz", "weight")],
                  None if stack_desc is None else nat.ctypes.byref(stack_desc), nat.ctypes.byref(q),
                  nat.ctypes.byref(n_out))

@@ -1,7 +1,8 @@
 """Treatment-head pre-solve on the GPU.
 
-* Attenuation mode is deterministic given the primaries: the device population equals
-  the oracle's (oracle/head_ref.py) record for record, to float32.
+* Attenuation mode is deterministic given the primaries: on host-sampled primaries the
+  device population equals the oracle's (oracle/head_ref.py) record for record, to
+  float32; primaries drawn in the kernel give the same population in distribution.
 * First-Compton mode draws random numbers: the device and oracle populations agree in
   their weight totals per particle type and in the photons' weight-energy histogram.
 * The same seed gives the same population, copied back or left resident.
@@ -27,9 +28,14 @@ FOCAL = (4.0, 4.0, -100.0)
 EXIT_W = 97.0  # exit plane 3 cm above the phantom face
 
 
-def _head(xs):
+def _head(xs, host_sampled: bool = False):
+    """The test head: jaws and an MLC, a spot fan above them (host-sampled through its
+    sample_batch when asked, else generated in the pre-solve kernel)."""
     from pyradmc_amd import GaussianSpotBeamSource, Spectrum
     from pyradmc_amd.collimation import MLC, BeamFrame, BeamLimitingStack, JawPair
+
+    class HostSampledSpot(GaussianSpotBeamSource):
+        """A subclass: the pre-solve samples it on the host."""
 
     frame = BeamFrame(FOCAL)
     stack = BeamLimitingStack(frame, (
@@ -38,7 +44,8 @@ def _head(xs):
         MLC(38.0, 45.0, tuple(np.linspace(-1.2, 1.2, 7)), (-0.5, -0.7, -0.9, -0.6, -0.4, -0.5),
             (0.3, 0.5, 0.6, 0.4, 0.2, 0.5), 8.0, 0, 18.0),
     ))
-    src = GaussianSpotBeamSource(Spectrum([1.0, 2.0, 3.0], [2.0, 1.0]), FOCAL, (4.0, 4.0, -85.0), 1.5, 1.5, 0.2, 0.2)
+    cls = HostSampledSpot if host_sampled else GaussianSpotBeamSource
+    src = cls(Spectrum([1.0, 2.0, 3.0], [2.0, 1.0]), FOCAL, (4.0, 4.0, -85.0), 1.5, 1.5, 0.2, 0.2)
     return stack, src
 
 
@@ -64,7 +71,7 @@ def test_attenuation_mode_matches_the_oracle_record_for_record(grid_xs):
     from pyradmc_amd.head import presolve_head
 
     xs = grid_xs[1]
-    stack, src = _head(xs)
+    stack, src = _head(xs, host_sampled=True)  # the oracle's primaries
     n = 200_000
     dev = presolve_head(src, cross_sections=xs, n_histories=n, seed=3, exit_z=EXIT_W, stack=stack,
                         mode="attenuation").columns()
@@ -115,6 +122,40 @@ def test_first_compton_population_matches_the_oracle_statistically(grid_xs):
         chi2 = float(np.sum((ha - hb)[live] ** 2 / (va + vb)[live]))
         assert chi2 < 12 + 6 * np.sqrt(24), (code, chi2)
     assert a[2][0].size > 1000  # contaminant electrons are there
+
+
+def test_generated_primaries_match_host_sampled_ones(grid_xs):
+    # The pre-solve kernel draws a built-in source's primaries itself; the population
+    # agrees in distribution with the one from the source's host sample_batch.
+    from pyradmc_amd.head import presolve_head
+
+    xs = grid_xs[1]
+    n = 400_000
+    pops = []
+    for host in (False, True):
+        stack, src = _head(xs, host_sampled=host)
+        pops.append(presolve_head(src, cross_sections=xs, n_histories=n, seed=7, exit_z=EXIT_W, stack=stack,
+                                  weight_floor=0.0).columns())
+    a, b = _by_type(pops[0], n), _by_type(pops[1], n)
+    (wa, ea), (wb, eb) = a[1], b[1]
+    assert abs(wa.sum() - wb.sum()) < 5.0 * np.sqrt((wa ** 2).sum() + (wb ** 2).sum())
+    ha = np.histogram(ea, bins=12, range=(0.0, 3.0), weights=wa)[0]
+    hb = np.histogram(eb, bins=12, range=(0.0, 3.0), weights=wb)[0]
+    va = np.histogram(ea, bins=12, range=(0.0, 3.0), weights=wa ** 2)[0]
+    vb = np.histogram(eb, bins=12, range=(0.0, 3.0), weights=wb ** 2)[0]
+    live = (va + vb) > 0
+    assert float(np.sum((ha - hb)[live] ** 2 / (va + vb)[live])) < 12 + 6 * np.sqrt(24)
+    # exit-plane positions over the field (photons scattered near 90 degrees reach the
+    # plane metres away, so a mean would be all outliers: compare histograms)
+    def hist(pop, weights):
+        return np.histogram(pop["x"], bins=20, range=(-1.0, 9.0), weights=weights)[0]
+
+    wa_all, wb_all = pops[0]["weight"] * n / pops[0]["weight"].size, pops[1]["weight"] * n / pops[1]["weight"].size
+    hx = hist(pops[0], wa_all) - hist(pops[1], wb_all)
+    vx = hist(pops[0], wa_all ** 2) + hist(pops[1], wb_all ** 2)
+    live = vx > 0
+    assert live.sum() >= 15
+    assert float(np.sum(hx[live] ** 2 / vx[live])) < 20 + 6 * np.sqrt(40)
 
 
 def test_same_seed_same_population_copied_back_or_resident(engine, grid_xs):
