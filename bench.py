@@ -547,6 +547,8 @@ def main() -> int:
     ap.add_argument("--no-pencil", action="store_true", help="skip the quick-start pencil leg")
     ap.add_argument("--no-dij", action="store_true", help="skip the batched Dij leg")
     ap.add_argument("--no-head", action="store_true", help="skip the treatment-head leg")
+    ap.add_argument("--head-only", action="store_true",
+                    help="run only the treatment-head leg and print its JSON (for profiling it)")
     ap.add_argument("--dij-median", type=float, default=2e5,
                     help="median per-beamlet history count of the Dij leg")
     ap.add_argument("--denoise-model", default=str(ROOT / "pyradmc_amd" / "models" / "denoiser_ct.npz"),
@@ -576,6 +578,12 @@ def main() -> int:
               "(histories shard across GPUs with no exchange for Dij columns; see the design notes)",
               file=sys.stderr)
         return 2
+    if args.head_only:
+        print(json.dumps({"head": _head_leg(repeats=2)}), flush=True)
+        from pyradmc_amd import _native
+
+        _native.shutdown()
+        return 0
 
     if args.copy_probe_mib > 0:
         import torch
