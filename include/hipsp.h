@@ -65,6 +65,13 @@ int hsp_csr(hsp_matrix *m, int64_t *row_ptr, int32_t *col, void *val);
  * of 8 rows. */
 long long hsp_block_entries(hsp_matrix *m);
 
+/* out[j] = sum_i D[i][j] v[i] for every column j: the transpose product the planning
+ * objective's gradient needs (pyradmc/study.py:93-114, D_roi.T @ residual). v: host
+ * float64 [n_rows]; out: host float64 [n_cols]. Each column is summed by one
+ * workgroup in a fixed order (strided partial sums, then a tree), so the result is
+ * reproducible but not the sequential sum bit for bit. kernel_ms optional. Blocking. */
+int hsp_apply_transpose(hsp_matrix *m, const double *v, double *out, double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,12 @@ struct hsp_matrix {
   long long out_r0 = 0, out_r1 = 0;
   int out_nw = 0, out_precision = HSP_F64;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // CSC copy for the transpose product, and its vector / output buffers
+  long long* csc_ptr = nullptr;  // [n_cols + 1]
+  int* csc_row = nullptr;        // [nnz]
+  void* csc_val = nullptr;       // [nnz] of value_type
+  double* tv = nullptr;          // [n_rows]
+  double* tout = nullptr;        // [n_cols]
   std::string error;
 };
 
@@ -536,6 +542,30 @@ int apply_on_device(hsp_matrix* m, const void* w, int n_w, long long r0, long lo
   return HSP_OK;
 }
 
+// out[j] = sum over column j's entries of val * v[row]: one 256-lane workgroup per
+// column (grid-stride over columns); lane t adds entries t, t + 256, ... in order, then
+// a fixed tree over the lanes. Products are rounded before the add (no contraction in
+// this file), so the result depends only on the matrix and v.
+template <typename V>
+__global__ void __launch_bounds__(256) k_transpose(const long long* __restrict__ ptr, const int* __restrict__ row,
+                                                   const V* __restrict__ val, long long n_cols,
+                                                   const double* __restrict__ v, double* __restrict__ out) {
+  __shared__ double red[256];
+  for (long long c = blockIdx.x; c < n_cols; c += gridDim.x) {
+    double acc = 0.0;
+    for (long long e = ptr[c] + threadIdx.x; e < ptr[c + 1]; e += blockDim.x)
+      acc += static_cast<double>(val[e]) * v[row[e]];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (static_cast<int>(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+    __syncthreads();  // red is reused by the next column
+  }
+}
+
 int check_apply(hsp_matrix* m, int n_w, long long r0, long long r1, int precision) {
   if (n_w < 1) return fail(m, HSP_EINVAL, "n_w must be at least 1");
   if (precision != HSP_F64 && precision != HSP_F32) return fail(m, HSP_EINVAL, "unknown precision");
@@ -558,6 +588,7 @@ int hsp_create(int device, long long n_rows, long long n_cols, const int64_t* in
   if (value_type != HSP_F64 && value_type != HSP_F32) return HSP_EINVAL;
   *out = nullptr;
   if (n_cols >= (1ll << 31)) return HSP_EINVAL;  // column indices are int32 on the device
+  if (n_rows >= (1ll << 31)) return HSP_EINVAL;  // and so are the CSC copy's row indices
   if (indptr[0] != 0) return HSP_EINVAL;
   for (long long c = 0; c < n_cols; ++c)
     if (indptr[c + 1] < indptr[c]) return HSP_EINVAL;
@@ -587,6 +618,17 @@ int hsp_create(int device, long long n_rows, long long n_cols, const int64_t* in
   step(hipMalloc(&m->col, sizeof(int) * nz));
   step(hipMalloc(&m->val, m->value_bytes * nz));
   step(hipMalloc(&m->blk_ptr, sizeof(long long) * (m->n_blocks + 1)));
+  // the CSC copy of the transpose product (row indices narrowed to int32)
+  step(hipMalloc(&m->csc_ptr, sizeof(long long) * (n_cols + 1)));
+  step(hipMalloc(&m->csc_row, sizeof(int) * nz));
+  step(hipMalloc(&m->csc_val, m->value_bytes * nz));
+  if (rc == HSP_OK) {
+    std::vector<int> rows32(nz, 0);
+    for (long long e = 0; e < nnz; ++e) rows32[e] = static_cast<int>(indices[e]);
+    step(hipMemcpy(m->csc_ptr, indptr, sizeof(long long) * (n_cols + 1), hipMemcpyHostToDevice));
+    step(hipMemcpy(m->csc_row, rows32.data(), sizeof(int) * nz, hipMemcpyHostToDevice));
+    if (nnz > 0) step(hipMemcpy(m->csc_val, values, m->value_bytes * nnz, hipMemcpyHostToDevice));
+  }
   if (rc == HSP_OK)
     rc = value_type == HSP_F64 ? build_forms(m, indptr, indices, static_cast<const double*>(values))
                                : build_forms(m, indptr, indices, static_cast<const float*>(values));
@@ -602,12 +644,45 @@ void hsp_destroy(hsp_matrix* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   for (void* p : {static_cast<void*>(m->row_ptr), static_cast<void*>(m->col), static_cast<void*>(m->val),
-                  static_cast<void*>(m->blk_ptr), static_cast<void*>(m->blk_col), m->blk_val, m->w, m->out})
+                  static_cast<void*>(m->blk_ptr), static_cast<void*>(m->blk_col), m->blk_val, m->w, m->out,
+                  static_cast<void*>(m->csc_ptr), static_cast<void*>(m->csc_row), m->csc_val,
+                  static_cast<void*>(m->tv), static_cast<void*>(m->tout)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev)
     if (e) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
+}
+
+int hsp_apply_transpose(hsp_matrix* m, const double* v, double* out, double* kernel_ms) {
+  if (!m) return HSP_EINVAL;
+  if (!v || !out) return fail(m, HSP_EINVAL, "null vector");
+  HSP_HIP(m, hipSetDevice(m->device));
+  if (!m->tv) HSP_HIP(m, hipMalloc(&m->tv, sizeof(double) * m->n_rows));
+  if (!m->tout) HSP_HIP(m, hipMalloc(&m->tout, sizeof(double) * m->n_cols));
+  HSP_HIP(m, hipMemcpyAsync(m->tv, v, sizeof(double) * m->n_rows, hipMemcpyHostToDevice, m->stream));
+  if (kernel_ms && !m->ev[0]) {
+    HSP_HIP(m, hipEventCreate(&m->ev[0]));
+    HSP_HIP(m, hipEventCreate(&m->ev[1]));
+  }
+  if (kernel_ms) HSP_HIP(m, hipEventRecord(m->ev[0], m->stream));
+  const unsigned int grid = static_cast<unsigned int>(std::min<long long>(m->n_cols, 65536));
+  if (m->value_type == HSP_F64)
+    hipLaunchKernelGGL(k_transpose<double>, dim3(grid), dim3(256), 0, m->stream, m->csc_ptr, m->csc_row,
+                       static_cast<const double*>(m->csc_val), m->n_cols, m->tv, m->tout);
+  else
+    hipLaunchKernelGGL(k_transpose<float>, dim3(grid), dim3(256), 0, m->stream, m->csc_ptr, m->csc_row,
+                       static_cast<const float*>(m->csc_val), m->n_cols, m->tv, m->tout);
+  HSP_HIP(m, hipGetLastError());
+  if (kernel_ms) HSP_HIP(m, hipEventRecord(m->ev[1], m->stream));
+  HSP_HIP(m, hipMemcpyAsync(out, m->tout, sizeof(double) * m->n_cols, hipMemcpyDeviceToHost, m->stream));
+  HSP_HIP(m, hipStreamSynchronize(m->stream));
+  if (kernel_ms) {
+    float ms = 0.0f;
+    HSP_HIP(m, hipEventElapsedTime(&ms, m->ev[0], m->ev[1]));
+    *kernel_ms = ms;
+  }
+  return HSP_OK;
 }
 
 int hsp_apply(hsp_matrix* m, const double* weights, int n_w, long long row_begin, long long row_end,

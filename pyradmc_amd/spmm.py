@@ -38,6 +38,7 @@ def _lib():
         "hsp_fetch": (ctypes.c_int, [_c_void, ll, ll, _c_void]),
         "hsp_csr": (ctypes.c_int, [_c_void, i64p, ctypes.POINTER(ctypes.c_int32), _c_void]),
         "hsp_block_entries": (ll, [_c_void]),
+        "hsp_apply_transpose": (ctypes.c_int, [_c_void, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(h, name)
@@ -160,6 +161,22 @@ class DeviceDij:
                                             ctypes.byref(ms)), "hsp_apply_device")
         self.last_kernel_ms = float(ms.value)
         return self.last_kernel_ms
+
+    def apply_transpose(self, vector) -> np.ndarray:
+        """``Dij.T @ v`` for one float64 vector over the rows: one value per beamlet
+        (the gradient's product in the planning objective). Each column is summed in a
+        fixed order on the device: reproducible, and equal to the sequential sum to
+        rounding, not bit for bit. ``last_kernel_ms`` holds the kernel's time."""
+        v = np.ascontiguousarray(np.asarray(vector, dtype=np.float64).reshape(-1))
+        if v.shape != (self.n_rows,):
+            raise ValueError(f"vector shape {v.shape} != ({self.n_rows},)")
+        out = np.empty(self.n_cols, dtype=np.float64)
+        ms = ctypes.c_double()
+        f64p = ctypes.POINTER(ctypes.c_double)
+        self._check(_lib().hsp_apply_transpose(self._h, v.ctypes.data_as(f64p), out.ctypes.data_as(f64p),
+                                               ctypes.byref(ms)), "hsp_apply_transpose")
+        self.last_kernel_ms = float(ms.value)
+        return out
 
     def dose_for_weights(self, weights) -> np.ndarray:
         """The reference's ``DijResult.dose_for_weights``: a float64 dose grid."""
