@@ -439,7 +439,57 @@ def _plan_leg(target: float, probe_n: int, n_vectors: int, cap: int,
     out["product"] = {"weight_vectors": n_vectors, "block_entries": dev.block_entries,
                       "output": "[voxels][vectors] on the device", **products}
     dev.close()
+    out["optimise"] = _guarded("plan optimisation", _optimise_leg, d, grid)
     return out, d
+
+
+def _optimise_leg(d, grid) -> dict:
+    """The study's fluence optimisation on the full plan's Dij: a 2.5 cm-radius
+    spherical PTV at the isocentre and a 2x6x6 cm OAR box beside it, prescription the
+    median PTV dose of uniform weights, OAR limit 0.3, penalty 10. L-BFGS-B on the host
+    with the region's rows on the device (wall time, iterations, per-evaluation time),
+    beside the same two products per evaluation in scipy on the host CPU."""
+    from scipy.sparse import csc_array
+
+    from pyradmc_amd.study import ToyPlanProblem, objective_and_gradient, optimize_weights
+
+    shape = tuple(int(n) for n in grid.shape)
+    iso = np.array([shape[i] * float(grid.spacing[i]) / 2.0 for i in range(3)])
+    centres = [(np.arange(shape[i]) + 0.5) * float(grid.spacing[i]) - iso[i] for i in range(3)]
+    x, y, z = np.meshgrid(*centres, indexing="ij")
+    ptv = (x ** 2 + y ** 2 + z ** 2 < 2.5 ** 2).ravel()
+    oar = ((x > 3.0) & (x < 5.0) & (np.abs(y) < 3.0) & (np.abs(z) < 3.0)).ravel()
+    dij = d.dose_csc()
+    uniform_ptv = csc_array(dij)[ptv, :] @ np.ones(d.n_beamlets)
+    rx = float(np.median(uniform_ptv))
+    started = time.perf_counter()
+    problem = ToyPlanProblem(d, ptv, oar, prescription=rx, oar_max=0.3, oar_penalty=10.0)
+    problem._roi  # region rows to the device
+    setup_s = time.perf_counter() - started
+    info: dict = {}
+    started = time.perf_counter()
+    w = optimize_weights(problem, info=info)
+    solve_s = time.perf_counter() - started
+    evals = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        objective_and_gradient(problem, w)
+        evals.append(time.perf_counter() - t0)
+    roi = csc_array(dij)[ptv | oar, :].tocsc()
+    a = np.ones(roi.shape[0])
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        roi @ w
+        roi.T @ a
+        cpu.append(time.perf_counter() - t0)
+    dose = csc_array(dij)[ptv, :] @ w
+    return {"ptv_voxels": int(ptv.sum()), "oar_voxels": int(oar.sum()), "region_nnz": int(roi.nnz),
+            "setup_s": round(setup_s, 3), "solve_s": round(solve_s, 3), **info,
+            "evaluation_ms": round(1e3 * float(np.median(evals)), 3),
+            "cpu_scipy_products_per_evaluation_ms": round(1e3 * float(np.median(cpu)), 2),
+            "ptv_d98_d2_over_rx": [round(float(np.percentile(dose, 2.0)) / rx, 4),
+                                   round(float(np.percentile(dose, 98.0)) / rx, 4)]}
 
 
 def _cpu_dose_for_weights(d, vectors: int = 2) -> dict:

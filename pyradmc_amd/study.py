@@ -119,10 +119,11 @@ def objective_and_gradient(problem: ToyPlanProblem, weights) -> tuple[float, np.
     return f, dev.apply_transpose(a)
 
 
-def optimize_weights(problem: ToyPlanProblem, w0=None) -> np.ndarray:
+def optimize_weights(problem: ToyPlanProblem, w0=None, info: dict | None = None) -> np.ndarray:
     """L-BFGS-B under ``w >= 0`` (pyradmc/study.py:117-138): from uniform weights scaled
     to the prescription's mean PTV dose unless ``w0`` is given, with the reference's
-    iteration limit and tolerances. Repeat calls return identical weights."""
+    iteration limit and tolerances. Repeat calls return identical weights. ``info``, if
+    given, receives the solver's iterations, evaluations and final objective."""
     from scipy.optimize import minimize
 
     if w0 is None:
@@ -133,6 +134,8 @@ def optimize_weights(problem: ToyPlanProblem, w0=None) -> np.ndarray:
     result = minimize(lambda w: objective_and_gradient(problem, w), np.asarray(w0, dtype=np.float64), jac=True,
                       method="L-BFGS-B", bounds=[(0.0, None)] * problem.n_beamlets,
                       options={"maxiter": 500, "ftol": 1.0e-12, "gtol": 1.0e-10})
+    if info is not None:
+        info.update(iterations=int(result.nit), evaluations=int(result.nfev), objective=float(result.fun))
     return np.asarray(result.x, dtype=np.float64)
 
 
