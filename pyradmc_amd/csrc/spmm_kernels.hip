@@ -17,8 +17,8 @@
 // product rounded before the add, this is the reference's sum bit for bit.
 //
 // Roofline (HBM): per product, 4 + value bytes per CSR nonzero (as row blocks: 4 + 8
-// values per block entry, read once per pass over 256 vectors), the output (8 or 4
-// bytes per (row, vector)) and the weights (n_cols * n_w elements, cache-resident).
+// values per block entry, read once per vector pass), the output (8 or 4 bytes per
+// (row, vector)) and the weights (n_cols * n_w elements, cache-resident).
 
 #include <hip/hip_runtime.h>
 
@@ -30,7 +30,8 @@
 #include "../../include/hipsp.h"
 
 namespace {
-constexpr int kBlockRows = 8;  // rows per row block
+constexpr int kBlockRows = 8;     // rows per row block
+constexpr int kSuperBlocks = 16;  // row blocks per super-block (one workgroup, a wave per block)
 }
 
 struct hsp_matrix {
@@ -60,6 +61,12 @@ struct hsp_matrix {
   void* csc_val = nullptr;       // [nnz] of value_type
   double* tv = nullptr;          // [n_rows]
   double* tout = nullptr;        // [n_cols]
+  // Super-blocks (kSuperBlocks consecutive row blocks) for the tiled product; built
+  // only when n_cols <= 2^24.
+  long long n_super = 0, n_super_entries = 0;
+  long long* sb_ptr = nullptr;   // [n_super + 1]
+  int* sb_col = nullptr;         // [n_super_entries]: distinct columns of the super-block, ascending
+  int* blk_word = nullptr;       // [n_block_entries]: (index in the super-block's columns << 8) | row mask
   std::string error;
 };
 
@@ -252,6 +259,72 @@ __global__ void k_block_fill(const long long* __restrict__ row_ptr, const int* _
   }
 }
 
+// Merge of the union columns of super-block sb's row blocks: calls emit(column, cur,
+// end) once per distinct column, in ascending order; emit advances every block's
+// cursor past its entries of that column (a repeated (row, column) pair gives a block
+// two entries of one column).
+template <typename Emit>
+__device__ void merge_super(const long long* __restrict__ blk_ptr, const int* __restrict__ blk_col,
+                            long long n_blocks, long long sb, Emit emit) {
+  long long cur[kSuperBlocks], end[kSuperBlocks];
+#pragma unroll
+  for (int g = 0; g < kSuperBlocks; ++g) {
+    const long long b = sb * kSuperBlocks + g;
+    cur[g] = b < n_blocks ? blk_ptr[b] : 0;
+    end[g] = b < n_blocks ? blk_ptr[b + 1] : 0;
+  }
+  for (;;) {
+    int c_min = 0x7fffffff;
+#pragma unroll
+    for (int g = 0; g < kSuperBlocks; ++g)
+      if (cur[g] < end[g]) c_min = min(c_min, blk_col[cur[g]]);
+    if (c_min == 0x7fffffff) return;
+    emit(c_min, cur, end);
+  }
+}
+
+__global__ void k_super_count(const long long* __restrict__ blk_ptr, const int* __restrict__ blk_col,
+                              long long n_blocks, long long n_super, unsigned int* __restrict__ count) {
+  for (long long sb = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; sb < n_super;
+       sb += static_cast<long long>(gridDim.x) * blockDim.x) {
+    unsigned int n = 0;
+    merge_super(blk_ptr, blk_col, n_blocks, sb, [&](int c, long long* cur, const long long* end) {
+#pragma unroll
+      for (int g = 0; g < kSuperBlocks; ++g)
+        while (cur[g] < end[g] && blk_col[cur[g]] == c) ++cur[g];
+      ++n;
+    });
+    count[sb] = n;
+  }
+}
+
+// The super-block's columns, and for each of its blocks' entries the word
+// (position of the entry's column among them << 8) | (bit i set: row i's value is not 0).
+template <typename V>
+__global__ void k_super_fill(const long long* __restrict__ blk_ptr, const int* __restrict__ blk_col,
+                             const V* __restrict__ blk_val, long long n_blocks, long long n_super,
+                             const long long* __restrict__ sb_ptr, int* __restrict__ sb_col,
+                             int* __restrict__ blk_word) {
+  for (long long sb = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; sb < n_super;
+       sb += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long o = sb_ptr[sb];
+    int k = 0;
+    merge_super(blk_ptr, blk_col, n_blocks, sb, [&](int c, long long* cur, const long long* end) {
+      sb_col[o + k] = c;
+#pragma unroll
+      for (int g = 0; g < kSuperBlocks; ++g)
+        while (cur[g] < end[g] && blk_col[cur[g]] == c) {
+          int mask = 0;
+          for (int i = 0; i < kBlockRows; ++i)
+            if (blk_val[cur[g] * kBlockRows + i] != V(0)) mask |= 1 << i;
+          blk_word[cur[g]] = static_cast<int>((static_cast<unsigned int>(k) << 8) | static_cast<unsigned int>(mask));
+          ++cur[g];
+        }
+      ++k;
+    });
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Product kernels.
 
@@ -337,6 +410,403 @@ __global__ void __launch_bounds__(256) k_spmm_wide(const long long* __restrict__
   }
 }
 
+// Word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family GPUs.
+constexpr unsigned int kRawBufferWord3 = 0x00020000u;
+
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, unsigned int voff, unsigned int soff,
+                                             float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, unsigned int voff, unsigned int soff,
+                                              double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// A value's bits as loaded (wave-uniform, so scalar loads), the test for a value other
+// than +0 / -0 made on those bits on the scalar unit (gfx950 has no scalar float
+// compare), and the value itself.
+template <typename V> struct ValueBits;
+template <> struct ValueBits<float> {
+  using U = uint32_t;
+  __device__ static bool nonzero(U b) { return (b << 1) != 0u; }
+  __device__ static float value(U b) { return __builtin_bit_cast(float, b); }
+};
+template <> struct ValueBits<double> {
+  using U = uint64_t;
+  __device__ static bool nonzero(U b) { return (b << 1) != 0ull; }
+  __device__ static double value(U b) { return __builtin_bit_cast(double, b); }
+};
+
+// k_spmm_wide for at most 2^24 columns and weights of at most 2 GiB, built so that a
+// wave never waits on memory latency in the entry loop:
+//  * the row blocks are read with vector loads, 32 entries at a time and one chunk
+//    ahead: lane 2c + h holds entry c's column and its rows 4h..4h+3. Once a chunk has
+//    arrived, each lane pair forms the word (column << 8) | mask, where bit i of the
+//    mask says row i's value is not zero. Per entry, one readlane gives that word to
+//    the scalar unit, and one readlane per non-zero row gives the value. (Scalar loads
+//    of the row blocks, which stream from HBM, stalled every wave for a full memory
+//    round trip per entry: all scalar loads retire under one counter, so they cannot
+//    be issued ahead.)
+//  * the weight rows are read through a buffer resource: the row's byte offset is one
+//    scalar multiply per entry and the lane's offsets within the row are fixed for the
+//    pass, so an entry's loads need no vector address arithmetic and no bounds
+//    branches. Lanes past n_w read the next row's first elements, or 0 past the
+//    buffer's end (the resource's range check), into accumulators that are never
+//    stored. They run one entry ahead of the multiply-adds, in two alternating register
+//    sets (no copies between them, so the wait before an entry's multiply-adds covers
+//    only that entry's loads); the prefetch past the block's last entry re-reads it.
+// Each row's additions keep ascending column order, as in k_spmm_wide.
+constexpr int kChunk = 32;  // row-block entries per vector-loaded chunk
+
+template <typename T, typename V, int KL>
+__global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restrict__ blk_ptr,
+                                                        const int* __restrict__ blk_col,
+                                                        const V* __restrict__ blk_val,
+                                                        const T* __restrict__ w, unsigned int w_bytes, int n_w,
+                                                        long long r0, long long r1, unsigned int groups_per_pass,
+                                                        T* __restrict__ out) {
+  using B = ValueBits<V>;
+  using U = typename B::U;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const unsigned int pass = blockIdx.x / groups_per_pass, local = blockIdx.x % groups_per_pass;
+  const long long group = static_cast<long long>(local & 7u) * (groups_per_pass >> 3) + (local >> 3);
+  const long long b = r0 / kBlockRows + group * 4 + wave;
+  if (b * kBlockRows >= r1) return;
+  const long long u0 = blk_ptr[b];
+  const int n_e = static_cast<int>(blk_ptr[b + 1] - u0);  // a block's union holds at most n_cols < 2^24
+  const int* __restrict__ cols = blk_col + u0;
+  const U* __restrict__ bits = reinterpret_cast<const U*>(blk_val) + u0 * kBlockRows;
+  const int k0 = static_cast<int>(pass) * 64 * KL;
+  const __amdgpu_buffer_rsrc_t rw =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(w), 0, static_cast<int>(w_bytes), kRawBufferWord3);
+  const unsigned int row_bytes = static_cast<unsigned int>(n_w) * static_cast<unsigned int>(sizeof(T));
+  const unsigned int voff = static_cast<unsigned int>(k0 + lane) * static_cast<unsigned int>(sizeof(T));
+  const int half = lane & 1;
+  T acc[kBlockRows][KL];
+#pragma unroll
+  for (int i = 0; i < kBlockRows; ++i)
+#pragma unroll
+    for (int q = 0; q < KL; ++q) acc[i][q] = T(0);
+
+  // This lane's part of the chunk starting at entry c0 (past the last entry: the last).
+  auto load_chunk = [&](int c0, int& col, U (&v4)[4]) {
+    const int e = min(c0 + (lane >> 1), n_e - 1);
+    col = cols[e];
+    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v4[i] = p[i];
+  };
+  // (column << 8) | mask of the lane pair's entry, on both lanes of the pair.
+  auto chunk_words = [&](int col, const U (&v4)[4]) {
+    int nib = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) nib |= B::nonzero(v4[i]) ? 1 << i : 0;
+    const int other = __shfl_xor(nib, 1);
+    return (col << 8) | (half ? other | nib << 4 : nib | other << 4);
+  };
+  auto fetch = [&](int word, T (&wts)[KL]) {
+    const unsigned int soff = (static_cast<unsigned int>(word) >> 8) * row_bytes;
+    // An empty asm makes the offset opaque here, so that vo + constant is not hoisted
+    // into one register per load but folded into each load's immediate offset.
+    unsigned int vo = voff;
+    __asm__ volatile("" : "+v"(vo));
+#pragma unroll
+    for (int q = 0; q < KL; ++q) wts[q] = buffer_load(rw, vo + static_cast<unsigned int>(q * 64 * sizeof(T)), soff, T(0));
+  };
+  auto value = [&](const U (&v4)[4], int i, int t) -> T {
+    const int src = 2 * t + (i >> 2);
+    if constexpr (sizeof(U) == 4) {
+      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src))));
+    } else {
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3] >> 32), src));
+      return static_cast<T>(B::value((static_cast<U>(hi) << 32) | lo));
+    }
+  };
+  auto accumulate = [&](int word, const U (&v4)[4], int t, const T (&wts)[KL]) {
+#pragma unroll
+    for (int i = 0; i < kBlockRows; ++i)
+      if (word >> i & 1) {
+        const T v = value(v4, i, t);
+#pragma unroll
+        for (int q = 0; q < KL; ++q) acc[i][q] = madd(acc[i][q], v, wts[q]);
+      }
+  };
+
+  if (n_e > 0) {
+    int col_n;
+    U v4[4], v4_n[4];
+    load_chunk(0, col_n, v4_n);
+    int words_n = chunk_words(col_n, v4_n);
+    T wa[KL], wb[KL];
+    fetch(__builtin_amdgcn_readlane(words_n, 0), wa);
+    for (int c0 = 0; c0 < n_e; c0 += kChunk) {
+      const int words = words_n;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v4[i] = v4_n[i];
+      load_chunk(c0 + kChunk, col_n, v4_n);
+      const int ce = min(kChunk, n_e - c0);
+      const bool more = c0 + kChunk < n_e;
+      // Entry t: the next entry's weights into `nxt` (past the chunk: the next chunk's
+      // first entry, whose words are formed here; past the block: entry t again), then
+      // entry t's multiply-adds from `cur`. Every path issues the same loads, so the
+      // wait before the multiply-adds is exact.
+      auto step = [&](int t, const T (&cur)[KL], T (&nxt)[KL]) {
+        int next_word;
+        if (t + 1 < ce) {
+          next_word = __builtin_amdgcn_readlane(words, 2 * (t + 1));
+        } else if (more) {
+          words_n = chunk_words(col_n, v4_n);
+          next_word = __builtin_amdgcn_readlane(words_n, 0);
+        } else {
+          next_word = __builtin_amdgcn_readlane(words, 2 * t);
+        }
+        fetch(next_word, nxt);
+        accumulate(__builtin_amdgcn_readlane(words, 2 * t), v4, t, cur);
+      };
+#pragma unroll
+      for (int t = 0; t < kChunk; t += 2) {
+        step(t, wa, wb);
+        if (t + 1 >= ce) break;
+        step(t + 1, wb, wa);
+        if (t + 2 >= ce) break;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kBlockRows; ++i) {
+    const long long r = b * kBlockRows + i;
+    if (r < r0 || r >= r1) continue;
+    T* dst = out + (r - r0) * n_w;
+#pragma unroll
+    for (int q = 0; q < KL; ++q) {
+      const int k = k0 + q * 64 + lane;
+      if (k < n_w) dst[k] = acc[i][q];
+    }
+  }
+}
+
+// The product for many vectors with the weights staged in LDS: a workgroup takes one
+// super-block (16 consecutive row blocks, a wave per block; 128 voxels) and one vector
+// pass. It walks the distinct columns of its blocks in ascending order, kTileSlots at
+// a time: the weight rows W[j][k0 : k0 + 64 KL] of a tile come from L2 once per
+// workgroup, straight into LDS (buffer loads with the LDS destination, 1 KiB per wave
+// load), and every wave then reads from LDS the rows its own entries in the tile need.
+// Tiles are double-buffered: the next tile's loads are issued right after the barrier
+// that publishes the current one. The columns being staged come from LDS too (up to
+// 1024 at a time), so that the loop makes no scalar loads: those retire under the same
+// counter as LDS reads and would serialize them.
+// Each wave's entries arrive in chunks of 32 with vector loads (lane 2c + h: entry c's
+// word and rows 4h..4h+3); readlane hands the word and each non-zero row's value to
+// the scalar unit. Three consecutive chunks sit in registers and a tile reads only
+// chunks requested before its barrier (which waits for them), so no chunk load is
+// waited for inside a tile. A tile's entries are a run of the chunk (the words
+// ascend), counted with a ballot. LDS reads of the weights run one entry ahead of the
+// multiply-adds in two alternating register sets, as in k_spmm_wide_rsrc.
+// Each row's additions keep ascending column order: tiles ascend, and so do the
+// entries within a tile.
+constexpr int kTileSlots = 32;  // weight rows per LDS tile
+
+template <typename T, typename V, int KL>
+__global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
+    const long long* __restrict__ blk_ptr, const int* __restrict__ blk_word, const V* __restrict__ blk_val,
+    const long long* __restrict__ sb_ptr, const int* __restrict__ sb_col, long long n_blocks,
+    const T* __restrict__ w, unsigned int w_bytes, int n_w, long long r0, long long r1,
+    unsigned int groups_per_pass, T* __restrict__ out) {
+  using B = ValueBits<V>;
+  using U = typename B::U;
+  constexpr int kSlotBytes = 64 * KL * static_cast<int>(sizeof(T));
+  static_assert(kSlotBytes == 1024 || kSlotBytes == 2048, "a slot is one or two 1 KiB wave loads");
+  constexpr int kSlotLoads = kSlotBytes / 1024;
+  constexpr int kThreads = 64 * kSuperBlocks;
+  constexpr int kChunkEntries = 32;
+  __shared__ __align__(16) unsigned char tiles[2][kTileSlots * kSlotBytes];
+  __shared__ int stage[kThreads];
+
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const unsigned int pass = blockIdx.x / groups_per_pass, local = blockIdx.x % groups_per_pass;
+  const long long group = static_cast<long long>(local & 7u) * (groups_per_pass >> 3) + (local >> 3);
+  const long long sb = r0 / (kBlockRows * kSuperBlocks) + group;
+  const long long sb_end = ((r1 + kBlockRows - 1) / kBlockRows + kSuperBlocks - 1) / kSuperBlocks;
+  if (sb >= sb_end) return;  // the whole workgroup
+  const long long s_lo = sb_ptr[sb];
+  const int n_s = static_cast<int>(sb_ptr[sb + 1] - s_lo);
+  const long long b = sb * kSuperBlocks + wave;
+  const bool active = b < n_blocks && b * kBlockRows < r1 && (b + 1) * kBlockRows > r0;
+  const long long u0 = active ? blk_ptr[b] : 0;
+  const int n_e = active ? static_cast<int>(blk_ptr[b + 1] - u0) : 0;
+  const int* __restrict__ words_g = blk_word + u0;
+  const U* __restrict__ bits = reinterpret_cast<const U*>(blk_val) + u0 * kBlockRows;
+  const int k0 = static_cast<int>(pass) * 64 * KL;
+  const __amdgpu_buffer_rsrc_t rw =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(w), 0, static_cast<int>(w_bytes), kRawBufferWord3);
+  const unsigned int row_bytes = static_cast<unsigned int>(n_w) * static_cast<unsigned int>(sizeof(T));
+  const int half = lane & 1, pair = lane >> 1;
+  T acc[kBlockRows][KL];
+#pragma unroll
+  for (int i = 0; i < kBlockRows; ++i)
+#pragma unroll
+    for (int q = 0; q < KL; ++q) acc[i][q] = T(0);
+
+  auto fill_stage = [&](int c_base) {
+    const int c = c_base + static_cast<int>(threadIdx.x);
+    stage[threadIdx.x] = c < n_s ? sb_col[s_lo + c] : 0;
+  };
+  // Tile t's weight rows into buffer buf: wave loads j = wave, wave + 16, ... of the
+  // tile's kTileSlots * kSlotLoads.
+  auto issue_tile = [&](int t, int buf) {
+#pragma unroll
+    for (int r = 0; r < (kTileSlots * kSlotLoads + kSuperBlocks - 1) / kSuperBlocks; ++r) {
+      const int j = wave + r * kSuperBlocks;
+      const int s = j / kSlotLoads, h = j % kSlotLoads;
+      if (j >= kTileSlots * kSlotLoads || t + s >= n_s) break;
+      const int col = __builtin_amdgcn_readfirstlane(stage[(t + s) % kThreads]);
+      auto* dst = reinterpret_cast<__attribute__((address_space(3))) void*>(
+          reinterpret_cast<uintptr_t>(&tiles[buf][s * kSlotBytes + h * 1024]));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rw, dst, 16, static_cast<int>(k0 * sizeof(T)) + h * 1024 + lane * 16,
+          static_cast<int>(static_cast<unsigned int>(col) * row_bytes), 0, 0);
+    }
+  };
+  // Entries c0 .. c0 + 31 of this wave (past the last: the last). Only when n_e > 0.
+  auto load_chunk = [&](int c0, int& word, U (&v4)[4]) {
+    const int e = min(c0 + pair, n_e - 1);
+    word = words_g[e];
+    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v4[i] = p[i];
+  };
+  // Entries of tile t in the chunk that starts at entry c0, from its entry o on (they
+  // form a run: the words ascend).
+  auto count_in_tile = [&](int word, int c0, int o, int t) {
+    const bool in = half == 0 && pair >= o && c0 + pair < n_e &&
+                    static_cast<int>(static_cast<unsigned int>(word) >> 8) < t + kTileSlots;
+    return __popcll(__ballot(in));
+  };
+  auto value = [&](const U (&v4)[4], int i, int c) -> T {
+    const int src = 2 * c + (i >> 2);
+    if constexpr (sizeof(U) == 4) {
+      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src))));
+    } else {
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3] >> 32), src));
+      return static_cast<T>(B::value((static_cast<U>(hi) << 32) | lo));
+    }
+  };
+  // Entries o .. o + cnt - 1 of a chunk against tile t in buffer buf.
+  auto process = [&](int word, const U (&v4)[4], int o, int cnt, int t, int buf) {
+    const T* tile = reinterpret_cast<const T*>(tiles[buf]);
+    auto read_w = [&](int c, T (&wts)[KL]) {
+      const int slot = static_cast<int>(static_cast<unsigned int>(__builtin_amdgcn_readlane(word, 2 * c)) >> 8) - t;
+      const T* src = tile + slot * (64 * KL) + lane;
+#pragma unroll
+      for (int q = 0; q < KL; ++q) wts[q] = src[q * 64];
+    };
+    auto accumulate = [&](int c, const T (&wts)[KL]) {
+      const int wd = __builtin_amdgcn_readlane(word, 2 * c);
+#pragma unroll
+      for (int i = 0; i < kBlockRows; ++i)
+        if (wd >> i & 1) {
+          const T v = value(v4, i, c);
+#pragma unroll
+          for (int q = 0; q < KL; ++q) acc[i][q] = madd(acc[i][q], v, wts[q]);
+        }
+    };
+    T wa[KL], wb[KL];
+    const int last = o + cnt - 1;
+    read_w(o, wa);
+    for (int c = o;; c += 2) {
+      read_w(min(c + 1, last), wb);
+      accumulate(c, wa);
+      if (c + 1 > last) break;
+      read_w(min(c + 2, last), wa);
+      accumulate(c + 1, wb);
+      if (c + 2 > last) break;
+    }
+  };
+
+  // Chunks k, k + 1 and k + 2 of this wave's entries (32 each) sit in registers. A tile
+  // reads only chunks whose loads were issued before its barrier, which waits for
+  // them: chunk k + 2 is requested when chunk k becomes current, and a tile may run on
+  // from chunk k into chunk k + 1. Only a tile with more entries than that (repeated
+  // entries) reloads the window with waits.
+  const int n_tiles = (n_s + kTileSlots - 1) / kTileSlots;
+  int w0 = 0, w1 = 0, w2 = 0;
+  U v0[4] = {}, v1[4] = {}, v2[4] = {};
+  int k = 0, e = 0;  // current chunk; this wave's next entry
+  auto load_window = [&]() {
+    if (kChunkEntries * k < n_e) load_chunk(kChunkEntries * k, w0, v0);
+    if (kChunkEntries * (k + 1) < n_e) load_chunk(kChunkEntries * (k + 1), w1, v1);
+    if (kChunkEntries * (k + 2) < n_e) load_chunk(kChunkEntries * (k + 2), w2, v2);
+  };
+  fill_stage(0);
+  load_window();
+  __syncthreads();
+  issue_tile(0, 0);
+  for (int ti = 0; ti < n_tiles; ++ti) {
+    const int t = ti * kTileSlots, buf = ti & 1;
+    // This wave's tile loads (and chunk loads) have landed; so have every wave's, and
+    // every wave is done with the other buffer.
+    __syncthreads();
+    if (e >= kChunkEntries * (k + 1)) {  // chunk k is used up: move the window on
+      w0 = w1;
+      w1 = w2;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v0[i] = v1[i];
+        v1[i] = v2[i];
+      }
+      ++k;
+      if (kChunkEntries * (k + 2) < n_e) load_chunk(kChunkEntries * (k + 2), w2, v2);
+    }
+    if (ti + 1 < n_tiles) {
+      if ((t + kTileSlots) % kThreads == 0) {
+        fill_stage(t + kTileSlots);
+        __syncthreads();
+      }
+      issue_tile(t + kTileSlots, buf ^ 1);
+    }
+    if (e < n_e) {
+      const int o = e - kChunkEntries * k;
+      const int cnt = count_in_tile(w0, kChunkEntries * k, o, t);
+      if (cnt > 0) process(w0, v0, o, cnt, t, buf);
+      e += cnt;
+      if (o + cnt == kChunkEntries && e < n_e) {  // the tile may run on into chunk k + 1
+        const int cnt1 = count_in_tile(w1, kChunkEntries * (k + 1), 0, t);
+        if (cnt1 > 0) process(w1, v1, 0, cnt1, t, buf);
+        e += cnt1;
+        // Chunk k + 1 used up as well: the tile's remaining entries one reloaded chunk
+        // at a time, then the window reloaded at the new position.
+        if (cnt1 == kChunkEntries && e < n_e) {
+          for (;;) {
+            k = e / kChunkEntries;
+            load_chunk(kChunkEntries * k, w0, v0);
+            const int c = count_in_tile(w0, kChunkEntries * k, 0, t);
+            if (c > 0) process(w0, v0, 0, c, t, buf);
+            e += c;
+            if (c < kChunkEntries || e >= n_e) break;
+          }
+          k = e / kChunkEntries;
+          load_window();
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kBlockRows; ++i) {
+    const long long r = b * kBlockRows + i;
+    if (r < r0 || r >= r1) continue;
+    T* dst = out + (r - r0) * n_w;
+#pragma unroll
+    for (int q = 0; q < KL; ++q) {
+      const int k = k0 + q * 64 + lane;
+      if (k < n_w) dst[k] = acc[i][q];
+    }
+  }
+}
+
 // Few vectors (n_w <= 8): 8 lanes per row block, a lane per row.
 template <typename T, typename V>
 __global__ void __launch_bounds__(256) k_spmm_narrow(const long long* __restrict__ blk_ptr,
@@ -375,10 +845,10 @@ struct ProductShape {
 };
 
 // Vectors per lane: in float32 beyond 256 vectors, 8 per lane (fewer passes, so the
-// row blocks are read and their per-entry scalar work done half as often; measured on
-// the full plan with 1000 vectors: 117 ms against 168 ms at 4 per lane, 134 ms at 16).
-// In float64 the accumulators of 8 per lane cost more occupancy than they save (271
-// against 241 ms), so 4 is the most.
+// row blocks are read and their per-entry work done half as often; measured with the
+// per-wave kernel on the full plan with 1000 vectors: 117 ms against 168 ms at 4 per
+// lane, 134 ms at 16). In float64 the accumulators of 8 per lane cost more occupancy
+// than they save (271 against 241 ms), so 4 is the most.
 ProductShape product_shape(int n_w, long long r0, long long r1, bool f32) {
   const long long blocks = (r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows;
   if (n_w <= 8) return {(blocks + 256 / kBlockRows - 1) / (256 / kBlockRows), 1, 0};
@@ -398,6 +868,42 @@ void launch_product(hsp_matrix* m, const T* w, int n_w, long long r0, long long 
   const unsigned int gpp = static_cast<unsigned int>(shape.groups_per_pass);
   const int kl = shape.kl;
   const dim3 grid(static_cast<unsigned int>(shape.groups_per_pass * shape.passes));
+  const unsigned long long w_bytes = static_cast<unsigned long long>(m->n_cols) * n_w * sizeof(T);
+  const int slot_bytes = 64 * kl * static_cast<int>(sizeof(T));
+  if (w_bytes <= 0x7fffffffull && m->sb_ptr && (slot_bytes == 1024 || slot_bytes == 2048)) {
+    const long long sb_lo = r0 / (kBlockRows * kSuperBlocks);
+    const long long sb_hi = ((r1 + kBlockRows - 1) / kBlockRows + kSuperBlocks - 1) / kSuperBlocks;
+    const long long groups = (sb_hi - sb_lo + 7) / 8 * 8;
+    const unsigned int wb = static_cast<unsigned int>(w_bytes);
+    const dim3 tgrid(static_cast<unsigned int>(groups * shape.passes));
+    const dim3 block(64 * kSuperBlocks);
+    const unsigned int g = static_cast<unsigned int>(groups);
+    // slot_bytes of 1 or 2 KiB: KL 2 or 4 in float64, 4 or 8 in float32
+    constexpr int kl_small = sizeof(T) == 8 ? 2 : 4, kl_large = sizeof(T) == 8 ? 4 : 8;
+    if (kl == kl_small)
+      hipLaunchKernelGGL((k_spmm_tiled<T, V, kl_small>), tgrid, block, 0, m->stream, m->blk_ptr, m->blk_word, bv,
+                         m->sb_ptr, m->sb_col, m->n_blocks, w, wb, n_w, r0, r1, g, out);
+    else
+      hipLaunchKernelGGL((k_spmm_tiled<T, V, kl_large>), tgrid, block, 0, m->stream, m->blk_ptr, m->blk_word, bv,
+                         m->sb_ptr, m->sb_col, m->n_blocks, w, wb, n_w, r0, r1, g, out);
+    return;
+  }
+  if (w_bytes <= 0x7fffffffull && m->n_cols <= (1ll << 24)) {
+    const unsigned int wb = static_cast<unsigned int>(w_bytes);
+    if (kl == 1)
+      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
+                         wb, n_w, r0, r1, gpp, out);
+    else if (kl == 2)
+      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 2>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
+                         wb, n_w, r0, r1, gpp, out);
+    else if (kl == 4 || sizeof(T) == 8)
+      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 4>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
+                         wb, n_w, r0, r1, gpp, out);
+    else
+      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 8>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
+                         wb, n_w, r0, r1, gpp, out);
+    return;
+  }
   if (kl == 1)
     hipLaunchKernelGGL((k_spmm_wide<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
                        r0, r1, gpp, out);
@@ -484,6 +990,20 @@ int build_forms(hsp_matrix* m, const int64_t* indptr, const int64_t* indices, co
     hipLaunchKernelGGL(k_block_fill<V>, dim3(grid_for(m->n_blocks)), dim3(256), 0, m->stream, m->row_ptr,
                        m->col, val, n_rows, m->n_blocks, m->blk_ptr, m->blk_col, static_cast<V*>(m->blk_val));
     HSP_HIP(m, hipGetLastError());
+    if (n_cols <= (1ll << 24)) {  // super-blocks: a column's position among them fits the word
+      m->n_super = (m->n_blocks + kSuperBlocks - 1) / kSuperBlocks;
+      HSP_HIP(m, hipMalloc(&m->sb_ptr, sizeof(long long) * (m->n_super + 1)));
+      hipLaunchKernelGGL(k_super_count, dim3(grid_for(m->n_super)), dim3(256), 0, m->stream, m->blk_ptr,
+                         m->blk_col, m->n_blocks, m->n_super, d_count);
+      HSP_HIP(m, hipGetLastError());
+      if (int rc = scan_counts(m, d_count, m->n_super, m->sb_ptr, &m->n_super_entries)) return rc;
+      HSP_HIP(m, hipMalloc(&m->sb_col, sizeof(int) * static_cast<size_t>(std::max(m->n_super_entries, 1ll))));
+      HSP_HIP(m, hipMalloc(&m->blk_word, sizeof(int) * ne));
+      hipLaunchKernelGGL(k_super_fill<V>, dim3(grid_for(m->n_super)), dim3(256), 0, m->stream, m->blk_ptr,
+                         m->blk_col, static_cast<const V*>(m->blk_val), m->n_blocks, m->n_super, m->sb_ptr,
+                         m->sb_col, m->blk_word);
+      HSP_HIP(m, hipGetLastError());
+    }
     HSP_HIP(m, hipStreamSynchronize(m->stream));
     return HSP_OK;
   };
@@ -646,7 +1166,8 @@ void hsp_destroy(hsp_matrix* m) {
   for (void* p : {static_cast<void*>(m->row_ptr), static_cast<void*>(m->col), static_cast<void*>(m->val),
                   static_cast<void*>(m->blk_ptr), static_cast<void*>(m->blk_col), m->blk_val, m->w, m->out,
                   static_cast<void*>(m->csc_ptr), static_cast<void*>(m->csc_row), m->csc_val,
-                  static_cast<void*>(m->tv), static_cast<void*>(m->tout)})
+                  static_cast<void*>(m->tv), static_cast<void*>(m->tout), static_cast<void*>(m->sb_ptr),
+                  static_cast<void*>(m->sb_col), static_cast<void*>(m->blk_word)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev)
     if (e) (void)hipEventDestroy(e);

@@ -106,7 +106,7 @@ def test_csr_form_is_sorted_with_repeats_in_csc_order(spmm, golden_dir):
     assert dev.block_entries == blocks
 
 
-@pytest.mark.parametrize("n_w", [1, 3, 8, 9, 64, 65, 128, 129, 300])
+@pytest.mark.parametrize("n_w", [1, 3, 8, 9, 64, 65, 128, 129, 300, 600])
 def test_random_products_bitwise(spmm, n_w):
     rng = np.random.default_rng(1000 + n_w)
     m = _beamlet_like(rng, value_dtype=np.float64 if n_w % 2 else np.float32)
@@ -119,7 +119,7 @@ def test_random_products_bitwise(spmm, n_w):
         assert np.array_equal(dev.apply(w, rows=(r0, r1)), ref[r0:r1]), (r0, r1)
 
 
-@pytest.mark.parametrize("n_w", [1, 5, 40, 200])
+@pytest.mark.parametrize("n_w", [1, 5, 40, 200, 300, 1000])
 def test_float32_products_within_rounding(spmm, n_w):
     rng = np.random.default_rng(2000 + n_w)
     m = _beamlet_like(rng, value_dtype=np.float64 if n_w % 2 else np.float32)
@@ -189,3 +189,59 @@ def test_empty_matrix_and_argument_errors(spmm):
     with pytest.raises(ValueError):
         dev.apply(np.ones(1), rows=(0, 6))
     assert np.array_equal(dev.apply(np.array([0.5])), [0.5, 0, 0, 0, 1.0])
+
+
+@pytest.mark.parametrize("precision,n_cols", [("f64", 280_000), ("f32", 540_000)])
+def test_weights_beyond_two_gib(spmm, precision, n_cols):
+    """Weights of more than 2 GiB take the kernel with 64-bit weight addressing; its
+    products equal those of the same matrix restricted to its non-empty columns,
+    whose weights take the buffer-resource kernel."""
+    rng = np.random.default_rng(6)
+    small = _beamlet_like(rng, shape=(8, 10, 50), n_cols=30)
+    n_w = 1000
+    cols = np.concatenate([np.arange(20), n_cols - 10 + np.arange(10)])  # where the 30 columns go
+    counts = np.zeros(n_cols, dtype=np.int64)
+    counts[cols] = np.diff(small.indptr)
+    big = _matrix(np.concatenate([[0], np.cumsum(counts)]), small.indices, small.dose, small.n_voxels)
+    w_small = rng.random((30, n_w))
+    w_big = np.zeros((n_cols, n_w))
+    w_big[cols] = w_small
+    assert n_cols * n_w * (8 if precision == "f64" else 4) > 2 ** 31
+    got = spmm.DeviceDij(big).apply(w_big, precision=precision)
+    del w_big
+    want = spmm.DeviceDij(small).apply(w_small, precision=precision)
+    assert np.array_equal(got, want)
+    if precision == "f64":
+        assert np.array_equal(want, _expected(small, w_small))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_many_repeated_entries_and_wide_column_sets(spmm, precision):
+    """Row blocks with more entries in one group of 16 distinct columns than a chunk
+    of 32 holds (each (row, column) pair repeated three times), and a 128-row group
+    that touches more than 1024 distinct columns. Float64 equals the oracle bit for
+    bit; float32 stays within its rounding bound."""
+    rng = np.random.default_rng(7)
+    n_rows, n_cols = 700, 1500
+    indptr, indices, values = [0], [], []
+    for c in range(n_cols):
+        rows = [c % 128]  # rows 0..127 see every column: 1500 distinct columns in that group
+        if c < 40:
+            rows += [1, 1, 1, 2, 2, 2]  # rows 1 and 2: three entries per column
+        rows += list(rng.choice(np.arange(128, n_rows), size=3, replace=False))
+        indices += rows
+        values += list(rng.random(len(rows)) * 10.0 ** rng.uniform(-3, 1, len(rows)))
+        indptr.append(len(indices))
+    m = _matrix(indptr, indices, np.asarray(values), n_rows)
+    dev = spmm.DeviceDij(m)
+    for n_w in (100, 300, 1000):
+        w = rng.random((n_cols, n_w))
+        ref = _expected(m, w)
+        got = dev.apply(w, precision=precision)
+        if precision == "f64":
+            assert np.array_equal(got, ref), n_w
+            assert np.array_equal(dev.apply(w, rows=(3, 650)), ref[3:650]), n_w
+        else:
+            absum = mc.dose_for_weights(m.indptr, m.indices, np.abs(m.dose), np.abs(w), m.n_voxels)
+            terms = np.bincount(m.indices, minlength=m.n_voxels)[:, None]
+            assert np.all(np.abs(got.astype(np.float64) - ref) <= (terms + 3) * 2.0 ** -24 * absum * 1.0001), n_w
