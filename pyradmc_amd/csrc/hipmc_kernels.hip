@@ -455,6 +455,11 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 #define HMC_ROUND_WAVES 5
 #endif
 
+// Idle lanes of a wave in the electron round that make a refill worth its pass (of 64;
+// measured on the bench's slab step: 8 and 32 were slower than 16, and the photon
+// round gained nothing from the same rule).
+constexpr int kRefillIdleLanes = 16;
+
 template <bool kFast>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
 k_photon_round(TransportParams P, Queue qin,
@@ -651,8 +656,15 @@ k_electron_round(TransportParams P, Queue qin,
   qe.lds = 1;
   RefillPool pool(P.refill_block);
   while (true) {
-    const unsigned int i = refill_index(pool, head, !active);
+    // Idle lanes take new particles together, once kRefillIdleLanes of the wave's
+    // looping lanes are idle (or all of them): the refill block then runs for more
+    // lanes at a time and in fewer passes. Each particle's history is its own, so
+    // the results do not depend on when a lane takes it.
+    const int n_idle = __popcll(__ballot(!active));
+    const bool refill_now = !active && (n_idle >= kRefillIdleLanes || n_idle == __popcll(__ballot(1)));
+    const unsigned int i = refill_index(pool, head, refill_now);
     if (!active) {
+      if (!refill_now) continue;
       if (i >= n_in) break;
       uint32_t meta;
       load_particle(qin.data, i, x, y, z, e, ux, uy, uz, w, rng, meta);
@@ -820,6 +832,27 @@ k_electron_round(TransportParams P, Queue qin,
       rotate(ux, uy, uz, cp, mphi + kPi);
       e -= delta;
     }
+
+    // Ended at or below the cutoff (a range-limited step ends exactly there): the
+    // terminal deposit the next pass would make, made now with the same position,
+    // energy, random state and books, so that the lane is idle from the next pass on
+    // instead of spending it on the deposit alone. A particle the next pass would
+    // count as trapped is left to it.
+    if (e <= P.ecut && steps < P.max_steps) {
+      tal.steps++;
+      tal.vox++;
+      int mat_end = 0;
+      if constexpr (!kFast) {
+        const int jx = axis_index(x, P.g.x_lo, P.g.inv_sx, P.g.nx);
+        const int jy = axis_index(y, P.g.y_lo, P.g.inv_sy, P.g.ny);
+        const int jz = axis_index(z, P.g.z_lo, P.g.inv_sz, P.g.nz);
+        mat_end = P.g.material[(static_cast<long long>(jx) * P.g.ny + jy) * P.g.nz + jz];
+      }
+      deposit<kFast>(P, tal, base, x, y, z, w * e, spr<kFast>(P, e, mat_end));
+      if (positron)
+        annihilate<kFast>(P, tal, qp, base, column, x, y, z, w, spr<kFast>(P, kElectronMass, mat_end), rng);
+      active = false;
+    }
   }
   seal_chunk(qp);
   seal_chunk(qe);
@@ -978,7 +1011,10 @@ __device__ inline Primary make_primary(const DevSource& S, int beamlet, long lon
 // (primary t -> slot t; the count is set once); host-sampled mixed-kind primaries
 // reserve slots in the queue of their kind with one atomic per queue per block and
 // pass. The emitted energy is reduced over the block and added with one atomic per
-// block (pyradmc/backends/warp/kernels.py:1888-2401 for the generators).
+// block (pyradmc/backends/warp/kernels.py:1888-2401 for the generators). kRays: the
+// source carries a collimator or mask factor (a separate build, so that sources
+// without one keep the smaller register budget).
+template <bool kRays>
 __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW, const DevItem* items,
                                                   int n_items, long long p0, long long n_gen,
                                                   uint64_t seed, Queue qp, Queue qe,
@@ -1014,10 +1050,12 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
       float e = q.e, w = q.w, x = q.x, y = q.y, z = q.z, ux = q.ux, uy = q.uy, uz = q.uz;
       bool book = q.book;
       kind = q.kind;
-      if (RW.kind != 0 && S.kind != HMC_SRC_UPLOADED) {
-        // collimator or mask factor of the primary's ray (no random number drawn)
-        w *= static_cast<float>(ray_weight(RW, e, x, y, z, ux, uy, uz));
-        book = true;
+      if constexpr (kRays) {
+        if (S.kind != HMC_SRC_UPLOADED) {
+          // collimator or mask factor of the primary's ray (no random number drawn)
+          w *= static_cast<float>(ray_weight(RW, e, x, y, z, ux, uy, uz));
+          book = true;
+        }
       }
       if (book) {
         const float latent = kind == POSITRON ? 2.0f * kElectronMass : 0.0f;
@@ -2730,9 +2768,14 @@ int hmc_transport(hmc_ctx* ctx, uint64_t seed, const hmc_work_item* items, int n
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, 8, 0u);
     const int pa = 0, ea = 1;  // queue roles at the start of every group of rounds
     Queue qpa{ctx->d_q[pa], counts + pa, cap}, qea{ctx->d_q[ea], counts + ea, cap};
-    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
-                       ctx->stream, ctx->src, ctx->rw, ctx->d_items, n_items, p0, n_gen,
-                       static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
+    if (ctx->rw.kind != 0)
+      hipLaunchKernelGGL(k_generate<true>, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
+                         ctx->stream, ctx->src, ctx->rw, ctx->d_items, n_items, p0, n_gen,
+                         static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
+    else
+      hipLaunchKernelGGL(k_generate<false>, dim3(blocks_for(n_gen, 256, 2048)), dim3(256), 0,
+                         ctx->stream, ctx->src, ctx->rw, ctx->d_items, n_items, p0, n_gen,
+                         static_cast<uint64_t>(seed), qpa, qea, ctx->d_ledger);
     HMC_HIP(ctx, hipGetLastError());
     while (true) {
       bool replayed = false;

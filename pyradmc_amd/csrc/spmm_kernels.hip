@@ -25,12 +25,14 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hipsp.h"
 
 namespace {
 constexpr int kBlockRows = 8;     // rows per row block
+constexpr int kHalfRows = kBlockRows / 2;  // rows per lane in the chunked kernels (a lane pair per entry)
 constexpr int kSuperBlocks = 16;  // row blocks per super-block (one workgroup, a wave per block)
 }
 
@@ -440,7 +442,7 @@ template <> struct ValueBits<double> {
 // k_spmm_wide for at most 2^24 columns and weights of at most 2 GiB, built so that a
 // wave never waits on memory latency in the entry loop:
 //  * the row blocks are read with vector loads, 32 entries at a time and one chunk
-//    ahead: lane 2c + h holds entry c's column and its rows 4h..4h+3. Once a chunk has
+//    ahead: lane 2c + h holds entry c's column and its rows of half h. Once a chunk has
 //    arrived, each lane pair forms the word (column << 8) | mask, where bit i of the
 //    mask says row i's value is not zero. Per entry, one readlane gives that word to
 //    the scalar unit, and one readlane per non-zero row gives the value. (Scalar loads
@@ -490,20 +492,20 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
     for (int q = 0; q < KL; ++q) acc[i][q] = T(0);
 
   // This lane's part of the chunk starting at entry c0 (past the last entry: the last).
-  auto load_chunk = [&](int c0, int& col, U (&v4)[4]) {
+  auto load_chunk = [&](int c0, int& col, U (&v4)[kHalfRows]) {
     const int e = min(c0 + (lane >> 1), n_e - 1);
     col = cols[e];
-    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * 4;
+    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * kHalfRows;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v4[i] = p[i];
+    for (int i = 0; i < kHalfRows; ++i) v4[i] = p[i];
   };
   // (column << 8) | mask of the lane pair's entry, on both lanes of the pair.
-  auto chunk_words = [&](int col, const U (&v4)[4]) {
+  auto chunk_words = [&](int col, const U (&v4)[kHalfRows]) {
     int nib = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) nib |= B::nonzero(v4[i]) ? 1 << i : 0;
+    for (int i = 0; i < kHalfRows; ++i) nib |= B::nonzero(v4[i]) ? 1 << i : 0;
     const int other = __shfl_xor(nib, 1);
-    return (col << 8) | (half ? other | nib << 4 : nib | other << 4);
+    return (col << 8) | (half ? other | nib << kHalfRows : nib | other << kHalfRows);
   };
   auto fetch = [&](int word, T (&wts)[KL]) {
     const unsigned int soff = (static_cast<unsigned int>(word) >> 8) * row_bytes;
@@ -514,17 +516,17 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
 #pragma unroll
     for (int q = 0; q < KL; ++q) wts[q] = buffer_load(rw, vo + static_cast<unsigned int>(q * 64 * sizeof(T)), soff, T(0));
   };
-  auto value = [&](const U (&v4)[4], int i, int t) -> T {
-    const int src = 2 * t + (i >> 2);
+  auto value = [&](const U (&v4)[kHalfRows], int i, int t) -> T {
+    const int src = 2 * t + i / kHalfRows;
     if constexpr (sizeof(U) == 4) {
-      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src))));
+      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows]), src))));
     } else {
-      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src));
-      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3] >> 32), src));
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows]), src));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows] >> 32), src));
       return static_cast<T>(B::value((static_cast<U>(hi) << 32) | lo));
     }
   };
-  auto accumulate = [&](int word, const U (&v4)[4], int t, const T (&wts)[KL]) {
+  auto accumulate = [&](int word, const U (&v4)[kHalfRows], int t, const T (&wts)[KL]) {
 #pragma unroll
     for (int i = 0; i < kBlockRows; ++i)
       if (word >> i & 1) {
@@ -536,7 +538,7 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
 
   if (n_e > 0) {
     int col_n;
-    U v4[4], v4_n[4];
+    U v4[kHalfRows], v4_n[kHalfRows];
     load_chunk(0, col_n, v4_n);
     int words_n = chunk_words(col_n, v4_n);
     T wa[KL], wb[KL];
@@ -544,7 +546,7 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
     for (int c0 = 0; c0 < n_e; c0 += kChunk) {
       const int words = words_n;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v4[i] = v4_n[i];
+      for (int i = 0; i < kHalfRows; ++i) v4[i] = v4_n[i];
       load_chunk(c0 + kChunk, col_n, v4_n);
       const int ce = min(kChunk, n_e - c0);
       const bool more = c0 + kChunk < n_e;
@@ -589,16 +591,16 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
 
 // The product for many vectors with the weights staged in LDS: a workgroup takes one
 // super-block (16 consecutive row blocks, a wave per block; 128 voxels) and one vector
-// pass. It walks the distinct columns of its blocks in ascending order, kTileSlots at
-// a time: the weight rows W[j][k0 : k0 + 64 KL] of a tile come from L2 once per
-// workgroup, straight into LDS (buffer loads with the LDS destination, 1 KiB per wave
-// load), and every wave then reads from LDS the rows its own entries in the tile need.
+// pass. It walks the distinct columns of its blocks in ascending order, a 64 KiB tile
+// of weight rows at a time: the rows W[j][k0 : k0 + 64 KL] of a tile come from L2 once
+// per workgroup, straight into LDS (buffer loads with the LDS destination, 1 KiB per
+// wave load), and every wave then reads from LDS the rows its own entries need.
 // Tiles are double-buffered: the next tile's loads are issued right after the barrier
 // that publishes the current one. The columns being staged come from LDS too (up to
 // 1024 at a time), so that the loop makes no scalar loads: those retire under the same
 // counter as LDS reads and would serialize them.
 // Each wave's entries arrive in chunks of 32 with vector loads (lane 2c + h: entry c's
-// word and rows 4h..4h+3); readlane hands the word and each non-zero row's value to
+// word and the rows of half h); readlane hands the word and each non-zero row's value to
 // the scalar unit. Three consecutive chunks sit in registers and a tile reads only
 // chunks requested before its barrier (which waits for them), so no chunk load is
 // waited for inside a tile. A tile's entries are a run of the chunk (the words
@@ -606,7 +608,7 @@ __global__ void __launch_bounds__(256) k_spmm_wide_rsrc(const long long* __restr
 // multiply-adds in two alternating register sets, as in k_spmm_wide_rsrc.
 // Each row's additions keep ascending column order: tiles ascend, and so do the
 // entries within a tile.
-constexpr int kTileSlots = 32;  // weight rows per LDS tile
+constexpr int kTileBytes = 65536;  // one LDS tile buffer: 32 weight rows of 2 KiB, 16 of 4 KiB
 
 template <typename T, typename V, int KL>
 __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
@@ -617,8 +619,9 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
   using B = ValueBits<V>;
   using U = typename B::U;
   constexpr int kSlotBytes = 64 * KL * static_cast<int>(sizeof(T));
-  static_assert(kSlotBytes == 1024 || kSlotBytes == 2048, "a slot is one or two 1 KiB wave loads");
+  static_assert(kSlotBytes % 1024 == 0, "a slot is a whole number of 1 KiB wave loads");
   constexpr int kSlotLoads = kSlotBytes / 1024;
+  constexpr int kTileSlots = kTileBytes / kSlotBytes;  // weight rows per tile
   constexpr int kThreads = 64 * kSuperBlocks;
   constexpr int kChunkEntries = 32;
   __shared__ __align__(16) unsigned char tiles[2][kTileSlots * kSlotBytes];
@@ -671,12 +674,12 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
     }
   };
   // Entries c0 .. c0 + 31 of this wave (past the last: the last). Only when n_e > 0.
-  auto load_chunk = [&](int c0, int& word, U (&v4)[4]) {
+  auto load_chunk = [&](int c0, int& word, U (&v4)[kHalfRows]) {
     const int e = min(c0 + pair, n_e - 1);
     word = words_g[e];
-    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * 4;
+    const U* p = bits + static_cast<long long>(e) * kBlockRows + half * kHalfRows;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v4[i] = p[i];
+    for (int i = 0; i < kHalfRows; ++i) v4[i] = p[i];
   };
   // Entries of tile t in the chunk that starts at entry c0, from its entry o on (they
   // form a run: the words ascend).
@@ -685,18 +688,18 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
                     static_cast<int>(static_cast<unsigned int>(word) >> 8) < t + kTileSlots;
     return __popcll(__ballot(in));
   };
-  auto value = [&](const U (&v4)[4], int i, int c) -> T {
-    const int src = 2 * c + (i >> 2);
+  auto value = [&](const U (&v4)[kHalfRows], int i, int c) -> T {
+    const int src = 2 * c + i / kHalfRows;
     if constexpr (sizeof(U) == 4) {
-      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src))));
+      return static_cast<T>(B::value(static_cast<U>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows]), src))));
     } else {
-      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3]), src));
-      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i & 3] >> 32), src));
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows]), src));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v4[i % kHalfRows] >> 32), src));
       return static_cast<T>(B::value((static_cast<U>(hi) << 32) | lo));
     }
   };
   // Entries o .. o + cnt - 1 of a chunk against tile t in buffer buf.
-  auto process = [&](int word, const U (&v4)[4], int o, int cnt, int t, int buf) {
+  auto process = [&](int word, const U (&v4)[kHalfRows], int o, int cnt, int t, int buf) {
     const T* tile = reinterpret_cast<const T*>(tiles[buf]);
     auto read_w = [&](int c, T (&wts)[KL]) {
       const int slot = static_cast<int>(static_cast<unsigned int>(__builtin_amdgcn_readlane(word, 2 * c)) >> 8) - t;
@@ -734,7 +737,7 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
   // entries) reloads the window with waits.
   const int n_tiles = (n_s + kTileSlots - 1) / kTileSlots;
   int w0 = 0, w1 = 0, w2 = 0;
-  U v0[4] = {}, v1[4] = {}, v2[4] = {};
+  U v0[kHalfRows] = {}, v1[kHalfRows] = {}, v2[kHalfRows] = {};
   int k = 0, e = 0;  // current chunk; this wave's next entry
   auto load_window = [&]() {
     if (kChunkEntries * k < n_e) load_chunk(kChunkEntries * k, w0, v0);
@@ -754,7 +757,7 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
       w0 = w1;
       w1 = w2;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < kHalfRows; ++i) {
         v0[i] = v1[i];
         v1[i] = v2[i];
       }
@@ -844,16 +847,33 @@ struct ProductShape {
   int kl;
 };
 
-// Vectors per lane: in float32 beyond 256 vectors, 8 per lane (fewer passes, so the
-// row blocks are read and their per-entry work done half as often; measured with the
-// per-wave kernel on the full plan with 1000 vectors: 117 ms against 168 ms at 4 per
-// lane, 134 ms at 16). In float64 the accumulators of 8 per lane cost more occupancy
-// than they save (271 against 241 ms), so 4 is the most.
+// Vectors per lane: the fewest passes whose accumulators (rows x vectors per lane)
+// stay within 64 registers, so that 4 waves fit per SIMD: with 8-row blocks, 8 per
+// lane in float32 and 4 in float64 (measured with the per-wave kernel on the full plan
+// with 1000 vectors: float32 117 ms against 168 ms at 4 per lane and 134 ms at 16;
+// float64 at 8 per lane 271 against 241 ms).
+template <typename T>
+constexpr int kMaxVectorsPerLane = 64 / (kBlockRows * static_cast<int>(sizeof(T) / 4));
+
 ProductShape product_shape(int n_w, long long r0, long long r1, bool f32) {
   const long long blocks = (r1 + kBlockRows - 1) / kBlockRows - r0 / kBlockRows;
   if (n_w <= 8) return {(blocks + 256 / kBlockRows - 1) / (256 / kBlockRows), 1, 0};
-  const int kl = n_w <= 64 ? 1 : n_w <= 128 ? 2 : (n_w <= 256 || !f32) ? 4 : 8;
+  const int kl_max = f32 ? kMaxVectorsPerLane<float> : kMaxVectorsPerLane<double>;
+  int kl = 1;
+  while (kl < kl_max && 64 * kl < n_w) kl *= 2;
   return {((blocks + 3) / 4 + 7) / 8 * 8, (n_w + 64 * kl - 1) / (64 * kl), kl};
+}
+
+// Launches Kernel<KL> for the run-time kl (a power of two up to the type's maximum).
+template <typename T, typename F>
+void with_kl(int kl, F launch) {
+  if (kl == 1) launch(std::integral_constant<int, 1>());
+  if (kl == 2) launch(std::integral_constant<int, 2>());
+  if (kl == 4) launch(std::integral_constant<int, 4>());
+  if constexpr (kMaxVectorsPerLane<T> >= 8)
+    if (kl == 8) launch(std::integral_constant<int, 8>());
+  if constexpr (kMaxVectorsPerLane<T> >= 16)
+    if (kl == 16) launch(std::integral_constant<int, 16>());
 }
 
 template <typename T, typename V>
@@ -870,52 +890,33 @@ void launch_product(hsp_matrix* m, const T* w, int n_w, long long r0, long long 
   const dim3 grid(static_cast<unsigned int>(shape.groups_per_pass * shape.passes));
   const unsigned long long w_bytes = static_cast<unsigned long long>(m->n_cols) * n_w * sizeof(T);
   const int slot_bytes = 64 * kl * static_cast<int>(sizeof(T));
-  if (w_bytes <= 0x7fffffffull && m->sb_ptr && (slot_bytes == 1024 || slot_bytes == 2048)) {
+  if (w_bytes <= 0x7fffffffull && m->sb_ptr && slot_bytes >= 1024) {
     const long long sb_lo = r0 / (kBlockRows * kSuperBlocks);
     const long long sb_hi = ((r1 + kBlockRows - 1) / kBlockRows + kSuperBlocks - 1) / kSuperBlocks;
     const long long groups = (sb_hi - sb_lo + 7) / 8 * 8;
     const unsigned int wb = static_cast<unsigned int>(w_bytes);
     const dim3 tgrid(static_cast<unsigned int>(groups * shape.passes));
-    const dim3 block(64 * kSuperBlocks);
     const unsigned int g = static_cast<unsigned int>(groups);
-    // slot_bytes of 1 or 2 KiB: KL 2 or 4 in float64, 4 or 8 in float32
-    constexpr int kl_small = sizeof(T) == 8 ? 2 : 4, kl_large = sizeof(T) == 8 ? 4 : 8;
-    if (kl == kl_small)
-      hipLaunchKernelGGL((k_spmm_tiled<T, V, kl_small>), tgrid, block, 0, m->stream, m->blk_ptr, m->blk_word, bv,
-                         m->sb_ptr, m->sb_col, m->n_blocks, w, wb, n_w, r0, r1, g, out);
-    else
-      hipLaunchKernelGGL((k_spmm_tiled<T, V, kl_large>), tgrid, block, 0, m->stream, m->blk_ptr, m->blk_word, bv,
-                         m->sb_ptr, m->sb_col, m->n_blocks, w, wb, n_w, r0, r1, g, out);
+    with_kl<T>(kl, [&](auto c) {
+      constexpr int K = decltype(c)::value;
+      if constexpr (64 * K * sizeof(T) >= 1024)
+        hipLaunchKernelGGL((k_spmm_tiled<T, V, K>), tgrid, dim3(64 * kSuperBlocks), 0, m->stream, m->blk_ptr,
+                           m->blk_word, bv, m->sb_ptr, m->sb_col, m->n_blocks, w, wb, n_w, r0, r1, g, out);
+    });
     return;
   }
   if (w_bytes <= 0x7fffffffull && m->n_cols <= (1ll << 24)) {
     const unsigned int wb = static_cast<unsigned int>(w_bytes);
-    if (kl == 1)
-      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
-                         wb, n_w, r0, r1, gpp, out);
-    else if (kl == 2)
-      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 2>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
-                         wb, n_w, r0, r1, gpp, out);
-    else if (kl == 4 || sizeof(T) == 8)
-      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 4>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
-                         wb, n_w, r0, r1, gpp, out);
-    else
-      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, 8>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w,
-                         wb, n_w, r0, r1, gpp, out);
+    with_kl<T>(kl, [&](auto c) {
+      hipLaunchKernelGGL((k_spmm_wide_rsrc<T, V, decltype(c)::value>), grid, dim3(256), 0, m->stream, m->blk_ptr,
+                         m->blk_col, bv, w, wb, n_w, r0, r1, gpp, out);
+    });
     return;
   }
-  if (kl == 1)
-    hipLaunchKernelGGL((k_spmm_wide<T, V, 1>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, gpp, out);
-  else if (kl == 2)
-    hipLaunchKernelGGL((k_spmm_wide<T, V, 2>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, gpp, out);
-  else if (kl == 4 || sizeof(T) == 8)
-    hipLaunchKernelGGL((k_spmm_wide<T, V, 4>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, gpp, out);
-  else
-    hipLaunchKernelGGL((k_spmm_wide<T, V, 8>), grid, dim3(256), 0, m->stream, m->blk_ptr, m->blk_col, bv, w, n_w,
-                       r0, r1, gpp, out);
+  with_kl<T>(kl, [&](auto c) {
+    hipLaunchKernelGGL((k_spmm_wide<T, V, decltype(c)::value>), grid, dim3(256), 0, m->stream, m->blk_ptr,
+                       m->blk_col, bv, w, n_w, r0, r1, gpp, out);
+  });
 }
 
 // Exclusive scan of n device counts into out[0..n]; returns the total.
