@@ -701,14 +701,17 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
   // Entries o .. o + cnt - 1 of a chunk against tile t in buffer buf.
   auto process = [&](int word, const U (&v4)[kHalfRows], int o, int cnt, int t, int buf) {
     const T* tile = reinterpret_cast<const T*>(tiles[buf]);
+    // The weights of entry c into wts; returns the entry's word (one readlane serves
+    // both the LDS address and, an entry later, the row tests).
     auto read_w = [&](int c, T (&wts)[KL]) {
-      const int slot = static_cast<int>(static_cast<unsigned int>(__builtin_amdgcn_readlane(word, 2 * c)) >> 8) - t;
+      const int wd = __builtin_amdgcn_readlane(word, 2 * c);
+      const int slot = static_cast<int>(static_cast<unsigned int>(wd) >> 8) - t;
       const T* src = tile + slot * (64 * KL) + lane;
 #pragma unroll
       for (int q = 0; q < KL; ++q) wts[q] = src[q * 64];
+      return wd;
     };
-    auto accumulate = [&](int c, const T (&wts)[KL]) {
-      const int wd = __builtin_amdgcn_readlane(word, 2 * c);
+    auto accumulate = [&](int c, int wd, const T (&wts)[KL]) {
 #pragma unroll
       for (int i = 0; i < kBlockRows; ++i)
         if (wd >> i & 1) {
@@ -719,13 +722,13 @@ __global__ void __launch_bounds__(64 * kSuperBlocks) k_spmm_tiled(
     };
     T wa[KL], wb[KL];
     const int last = o + cnt - 1;
-    read_w(o, wa);
+    int da = read_w(o, wa);
     for (int c = o;; c += 2) {
-      read_w(min(c + 1, last), wb);
-      accumulate(c, wa);
+      const int db = read_w(min(c + 1, last), wb);
+      accumulate(c, da, wa);
       if (c + 1 > last) break;
-      read_w(min(c + 2, last), wa);
-      accumulate(c + 1, wb);
+      da = read_w(min(c + 2, last), wa);
+      accumulate(c + 1, db, wb);
       if (c + 2 > last) break;
     }
   };
