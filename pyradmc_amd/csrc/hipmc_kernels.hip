@@ -460,6 +460,9 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 // round gained nothing from the same rule).
 constexpr int kRefillIdleLanes = 16;
 
+// Energy bins of the electron queue's sort (a quarter octave each above the cutoff).
+constexpr int kEnergyBins = 32;
+
 template <bool kFast>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
 k_photon_round(TransportParams P, Queue qin,
@@ -1453,7 +1456,7 @@ struct hmc_ctx {
   // queues: 4 queues + counts (4) + heads (4)
   Particle* d_q[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t q_capacity = 0;
-  unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads
+  unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads, then the energy sort's bins
   unsigned long long* d_ledger = nullptr;
   // Page-locked readback of the per-group counts (words 0-1: four u32), the ledger
   // (words 2..) and the fold's quanta (the last word). Copies into page-locked memory
@@ -1641,7 +1644,7 @@ int hmc_create(int device, hmc_ctx** out) {
     delete ctx;
     return HMC_EHIP;
   }
-  if (dev_alloc(ctx, &ctx->d_counts, 8) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1) ||
+  if (dev_alloc(ctx, &ctx->d_counts, 8 + 2 * kEnergyBins) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1) ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->h_stage), sizeof(unsigned long long) * kStageWords,
                     hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -2568,6 +2571,74 @@ int hmc_set_launch(hmc_ctx* ctx, int round_lanes_per_cu, int refill_block, int p
 // launches. With events (profiling), each round's two kernels are bracketed.
 constexpr int kRoundsPerSync = 4;
 
+// ------------------------------------------------------------- electron energy sort
+//
+// Before the first electron round of a group, its queue is reordered by energy into
+// kEnergyBins bins of a quarter octave above the cutoff, highest energies first, with
+// padding records dropped. The persistent lanes then take the longest histories first
+// and finish the round on short ones: the round's tail, where most lanes of the grid
+// are idle, gets shorter (on the bench's slab step the electron round fell from 1.22
+// to 1.17 ms; ascending order made it 1.32 ms). Each particle carries its own random
+// stream and the tallies are integer sums, so the results do not depend on the order.
+// bins[0..kEnergyBins) are the counts, bins[kEnergyBins..) the bins' next free slots.
+__device__ inline int energy_bin(float e, float inv_ecut) {
+  const int b = static_cast<int>(4.0f * __log2f(fmaxf(e * inv_ecut, 1.0f)));
+  return kEnergyBins - 1 - min(max(b, 0), kEnergyBins - 1);  // 0: the highest energies
+}
+
+__global__ void k_energy_count(const Particle* __restrict__ q, const unsigned int* count, unsigned int cap,
+                               float inv_ecut, unsigned int* bins) {
+  __shared__ unsigned int h[kEnergyBins];
+  if (threadIdx.x < kEnergyBins) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const unsigned int n = min(*count, cap);
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if ((q[i].meta & 3u) == kEmptyKind) continue;
+    atomicAdd(&h[energy_bin(q[i].e, inv_ecut)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kEnergyBins && h[threadIdx.x]) atomicAdd(bins + threadIdx.x, h[threadIdx.x]);
+}
+
+// Bin offsets, and the sorted queue's count and head.
+__global__ void k_energy_scan(unsigned int* bins, unsigned int* sorted_count, unsigned int* sorted_head) {
+  if (threadIdx.x == 0) {
+    unsigned int s = 0u;
+    for (int b = 0; b < kEnergyBins; ++b) {
+      bins[kEnergyBins + b] = s;
+      s += bins[b];
+    }
+    *sorted_count = s;
+    *sorted_head = 0u;
+  }
+}
+
+// Records to their bins' slots: per block and pass, one atomic per bin, the rank within
+// the bin from LDS.
+__global__ void k_energy_scatter(const Particle* __restrict__ q, const unsigned int* count, unsigned int cap,
+                                 float inv_ecut, unsigned int* bins, Particle* __restrict__ dst) {
+  __shared__ unsigned int h[kEnergyBins], base[kEnergyBins];
+  const unsigned int n = min(*count, cap);
+  for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    if (threadIdx.x < kEnergyBins) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned int i = b0 + threadIdx.x;
+    int bin = -1;
+    Particle p;
+    if (i < n) {
+      p = q[i];
+      if ((p.meta & 3u) != kEmptyKind) bin = energy_bin(p.e, inv_ecut);
+    }
+    const unsigned int rank = bin >= 0 ? atomicAdd(&h[bin], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < kEnergyBins && h[threadIdx.x])
+      base[threadIdx.x] = atomicAdd(bins + kEnergyBins + threadIdx.x, h[threadIdx.x]);
+    __syncthreads();
+    if (bin >= 0) dst[base[bin] + rank] = p;
+    __syncthreads();
+  }
+}
+
 // The rounds' fast build applies: dose to medium, no deposit spreading, and a scoring
 // grid with the transport grid's bounds, voxel sizes and counts.
 static bool fast_rounds(const TransportParams& P) {
@@ -2601,14 +2672,33 @@ static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned 
                          heads + pa, qa_e, qb_p);
     HMC_HIP(ctx, hipGetLastError());
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-    // Electron round: consume ea (including the photon round's pushes).
+    // Electron round: consume ea (including the photon round's pushes). In the first
+    // round of the group the queue is first sorted by energy into pa's records, which
+    // the photon round has consumed; pa's count and head then serve the sorted queue
+    // (both are reset with ea's after the round).
+    Queue qin_e = qa_e;
+    unsigned int* head_e = heads + ea;
+    if (r == 0) {
+      const float inv_ecut = 1.0f / P.ecut;
+      unsigned int* bins = counts + 8;
+      const unsigned int sort_blocks = 2048;
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(2 * kEnergyBins), 0, ctx->stream, bins, 2 * kEnergyBins, 0u);
+      hipLaunchKernelGGL(k_energy_count, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
+                         inv_ecut, bins);
+      hipLaunchKernelGGL(k_energy_scan, dim3(1), dim3(64), 0, ctx->stream, bins, counts + pa, heads + pa);
+      hipLaunchKernelGGL(k_energy_scatter, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
+                         inv_ecut, bins, qa_p.data);
+      HMC_HIP(ctx, hipGetLastError());
+      qin_e = Queue{qa_p.data, counts + pa, cap};
+      head_e = heads + pa;
+    }
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
     if (fast)
-      hipLaunchKernelGGL((k_electron_round<true>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
-                         heads + ea, qb_p, qb_e);
+      hipLaunchKernelGGL((k_electron_round<true>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qin_e,
+                         head_e, qb_p, qb_e);
     else
-      hipLaunchKernelGGL((k_electron_round<false>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qa_e,
-                         heads + ea, qb_p, qb_e);
+      hipLaunchKernelGGL((k_electron_round<false>), dim3(round_blocks), dim3(256), 0, ctx->stream, P, qin_e,
+                         head_e, qb_p, qb_e);
     HMC_HIP(ctx, hipGetLastError());
     if (ev) HMC_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     hipLaunchKernelGGL(k_reset_consumed, dim3(1), dim3(64), 0, ctx->stream, counts, heads, pa, ea, cap,
