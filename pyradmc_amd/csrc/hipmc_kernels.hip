@@ -2600,9 +2600,13 @@ __global__ void k_energy_count(const Particle* __restrict__ q, const unsigned in
   if (threadIdx.x < kEnergyBins && h[threadIdx.x]) atomicAdd(bins + threadIdx.x, h[threadIdx.x]);
 }
 
-// Bin offsets, and the sorted queue's count and head.
-__global__ void k_energy_scan(unsigned int* bins, unsigned int* sorted_count, unsigned int* sorted_head) {
+// Bin offsets, and the sorted queue's count and head. The count it replaces is a
+// consumed queue's, which k_reset_consumed would have checked against the capacity:
+// that check is made here first.
+__global__ void k_energy_scan(unsigned int* bins, unsigned int* sorted_count, unsigned int* sorted_head,
+                              unsigned int capacity, unsigned long long* ledger) {
   if (threadIdx.x == 0) {
+    if (*sorted_count > capacity) atomicMax(ledger + L_OVERFLOW, static_cast<unsigned long long>(*sorted_count));
     unsigned int s = 0u;
     for (int b = 0; b < kEnergyBins; ++b) {
       bins[kEnergyBins + b] = s;
@@ -2685,7 +2689,8 @@ static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned 
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(2 * kEnergyBins), 0, ctx->stream, bins, 2 * kEnergyBins, 0u);
       hipLaunchKernelGGL(k_energy_count, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
                          inv_ecut, bins);
-      hipLaunchKernelGGL(k_energy_scan, dim3(1), dim3(64), 0, ctx->stream, bins, counts + pa, heads + pa);
+      hipLaunchKernelGGL(k_energy_scan, dim3(1), dim3(64), 0, ctx->stream, bins, counts + pa, heads + pa, cap,
+                         ctx->d_ledger);
       hipLaunchKernelGGL(k_energy_scatter, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
                          inv_ecut, bins, qa_p.data);
       HMC_HIP(ctx, hipGetLastError());
