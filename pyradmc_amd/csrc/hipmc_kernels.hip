@@ -460,8 +460,10 @@ __device__ inline bool enter_grid(const DevGrid& g, float& x, float& y, float& z
 // round gained nothing from the same rule).
 constexpr int kRefillIdleLanes = 16;
 
-// Energy bins of the electron queue's sort (a quarter octave each above the cutoff).
+// Energy bins of the electron queue's sort (a quarter octave each above the cutoff),
+// and the blocks that sort it, each over one contiguous range of the queue.
 constexpr int kEnergyBins = 32;
+constexpr int kSortBlocks = 1024;
 
 template <bool kFast>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HMC_ROUND_WAVES, 8)))
@@ -1456,7 +1458,7 @@ struct hmc_ctx {
   // queues: 4 queues + counts (4) + heads (4)
   Particle* d_q[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t q_capacity = 0;
-  unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads, then the energy sort's bins
+  unsigned int* d_counts = nullptr;  // [0..3] counts, [4..7] heads, then the energy sort's histogram
   unsigned long long* d_ledger = nullptr;
   // Page-locked readback of the per-group counts (words 0-1: four u32), the ledger
   // (words 2..) and the fold's quanta (the last word). Copies into page-locked memory
@@ -1644,7 +1646,7 @@ int hmc_create(int device, hmc_ctx** out) {
     delete ctx;
     return HMC_EHIP;
   }
-  if (dev_alloc(ctx, &ctx->d_counts, 8 + 2 * kEnergyBins) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1) ||
+  if (dev_alloc(ctx, &ctx->d_counts, 8 + kEnergyBins * kSortBlocks) || dev_alloc(ctx, &ctx->d_ledger, L_COUNT + 1) ||
       hipHostMalloc(reinterpret_cast<void**>(&ctx->h_stage), sizeof(unsigned long long) * kStageWords,
                     hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -2577,69 +2579,89 @@ constexpr int kRoundsPerSync = 4;
 // kEnergyBins bins of a quarter octave above the cutoff, highest energies first, with
 // padding records dropped. The persistent lanes then take the longest histories first
 // and finish the round on short ones: the round's tail, where most lanes of the grid
-// are idle, gets shorter (on the bench's slab step the electron round fell from 1.22
-// to 1.17 ms; ascending order made it 1.32 ms). Each particle carries its own random
-// stream and the tallies are integer sums, so the results do not depend on the order.
-// bins[0..kEnergyBins) are the counts, bins[kEnergyBins..) the bins' next free slots.
+// are idle, gets shorter (ascending order instead made the round 8 % slower than no
+// sort). Each particle carries its own random stream and the tallies are integer sums,
+// so the results do not depend on the order.
+// A counting sort without global atomics: kSortBlocks blocks each take one contiguous
+// range of the queue, count it per bin (k_energy_count), one block scans the counts in
+// (bin, block) order into each block's first slot per bin (k_energy_scan), and each
+// block moves its records to the slots that LDS cursors hand out (k_energy_scatter).
 __device__ inline int energy_bin(float e, float inv_ecut) {
   const int b = static_cast<int>(4.0f * __log2f(fmaxf(e * inv_ecut, 1.0f)));
   return kEnergyBins - 1 - min(max(b, 0), kEnergyBins - 1);  // 0: the highest energies
 }
 
-__global__ void k_energy_count(const Particle* __restrict__ q, const unsigned int* count, unsigned int cap,
-                               float inv_ecut, unsigned int* bins) {
+// Block blockIdx.x's range [lo, hi) of the queue's n records.
+__device__ inline void sort_range(unsigned int n, unsigned int& lo, unsigned int& hi) {
+  const unsigned int per = (n + kSortBlocks - 1) / kSortBlocks;
+  lo = min(blockIdx.x * per, n);
+  hi = min(lo + per, n);
+}
+
+// hist[bin * kSortBlocks + block]: the block's records of that bin.
+__global__ void __launch_bounds__(256) k_energy_count(const Particle* __restrict__ q, const unsigned int* count,
+                                                      unsigned int cap, float inv_ecut, unsigned int* hist) {
   __shared__ unsigned int h[kEnergyBins];
   if (threadIdx.x < kEnergyBins) h[threadIdx.x] = 0u;
   __syncthreads();
-  const unsigned int n = min(*count, cap);
-  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if ((q[i].meta & 3u) == kEmptyKind) continue;
-    atomicAdd(&h[energy_bin(q[i].e, inv_ecut)], 1u);
-  }
+  unsigned int lo, hi;
+  sort_range(min(*count, cap), lo, hi);
+  for (unsigned int i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    if ((q[i].meta & 3u) != kEmptyKind) atomicAdd(&h[energy_bin(q[i].e, inv_ecut)], 1u);
   __syncthreads();
-  if (threadIdx.x < kEnergyBins && h[threadIdx.x]) atomicAdd(bins + threadIdx.x, h[threadIdx.x]);
+  if (threadIdx.x < kEnergyBins) hist[threadIdx.x * kSortBlocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Bin offsets, and the sorted queue's count and head. The count it replaces is a
-// consumed queue's, which k_reset_consumed would have checked against the capacity:
-// that check is made here first.
-__global__ void k_energy_scan(unsigned int* bins, unsigned int* sorted_count, unsigned int* sorted_head,
-                              unsigned int capacity, unsigned long long* ledger) {
-  if (threadIdx.x == 0) {
+// Exclusive scan of hist in place (one block of 1024 threads, 32 entries each), the
+// sorted queue's count and head. The count it replaces is a consumed queue's, which
+// k_reset_consumed would have checked against the capacity: that check is made here.
+__global__ void __launch_bounds__(1024) k_energy_scan(unsigned int* hist, unsigned int* sorted_count,
+                                                      unsigned int* sorted_head, unsigned int capacity,
+                                                      unsigned long long* ledger) {
+  constexpr int kPer = kEnergyBins * kSortBlocks / 1024;
+  __shared__ unsigned int part[1024];
+  const int t = static_cast<int>(threadIdx.x);
+  unsigned int v[kPer];
+  unsigned int sum = 0u;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    v[j] = hist[t * kPer + j];
+    sum += v[j];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int x = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  unsigned int run = part[t] - sum;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    hist[t * kPer + j] = run;
+    run += v[j];
+  }
+  if (t == 1023) {
     if (*sorted_count > capacity) atomicMax(ledger + L_OVERFLOW, static_cast<unsigned long long>(*sorted_count));
-    unsigned int s = 0u;
-    for (int b = 0; b < kEnergyBins; ++b) {
-      bins[kEnergyBins + b] = s;
-      s += bins[b];
-    }
-    *sorted_count = s;
+    *sorted_count = part[1023];
     *sorted_head = 0u;
   }
 }
 
-// Records to their bins' slots: per block and pass, one atomic per bin, the rank within
-// the bin from LDS.
-__global__ void k_energy_scatter(const Particle* __restrict__ q, const unsigned int* count, unsigned int cap,
-                                 float inv_ecut, unsigned int* bins, Particle* __restrict__ dst) {
-  __shared__ unsigned int h[kEnergyBins], base[kEnergyBins];
-  const unsigned int n = min(*count, cap);
-  for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-    if (threadIdx.x < kEnergyBins) h[threadIdx.x] = 0u;
-    __syncthreads();
-    const unsigned int i = b0 + threadIdx.x;
-    int bin = -1;
-    Particle p;
-    if (i < n) {
-      p = q[i];
-      if ((p.meta & 3u) != kEmptyKind) bin = energy_bin(p.e, inv_ecut);
-    }
-    const unsigned int rank = bin >= 0 ? atomicAdd(&h[bin], 1u) : 0u;
-    __syncthreads();
-    if (threadIdx.x < kEnergyBins && h[threadIdx.x])
-      base[threadIdx.x] = atomicAdd(bins + kEnergyBins + threadIdx.x, h[threadIdx.x]);
-    __syncthreads();
-    if (bin >= 0) dst[base[bin] + rank] = p;
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_energy_scatter(const Particle* __restrict__ q, const unsigned int* count,
+                                                        unsigned int cap, float inv_ecut,
+                                                        const unsigned int* __restrict__ first,
+                                                        Particle* __restrict__ dst) {
+  __shared__ unsigned int cursor[kEnergyBins];
+  if (threadIdx.x < kEnergyBins) cursor[threadIdx.x] = first[threadIdx.x * kSortBlocks + blockIdx.x];
+  __syncthreads();
+  unsigned int lo, hi;
+  sort_range(min(*count, cap), lo, hi);
+  for (unsigned int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const Particle p = q[i];
+    if ((p.meta & 3u) == kEmptyKind) continue;
+    dst[atomicAdd(&cursor[energy_bin(p.e, inv_ecut)], 1u)] = p;
   }
 }
 
@@ -2684,15 +2706,13 @@ static int enqueue_round_group(hmc_ctx* ctx, const TransportParams& P, unsigned 
     unsigned int* head_e = heads + ea;
     if (r == 0) {
       const float inv_ecut = 1.0f / P.ecut;
-      unsigned int* bins = counts + 8;
-      const unsigned int sort_blocks = 2048;
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(2 * kEnergyBins), 0, ctx->stream, bins, 2 * kEnergyBins, 0u);
-      hipLaunchKernelGGL(k_energy_count, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
-                         inv_ecut, bins);
-      hipLaunchKernelGGL(k_energy_scan, dim3(1), dim3(64), 0, ctx->stream, bins, counts + pa, heads + pa, cap,
+      unsigned int* hist = counts + 8;
+      hipLaunchKernelGGL(k_energy_count, dim3(kSortBlocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
+                         inv_ecut, hist);
+      hipLaunchKernelGGL(k_energy_scan, dim3(1), dim3(1024), 0, ctx->stream, hist, counts + pa, heads + pa, cap,
                          ctx->d_ledger);
-      hipLaunchKernelGGL(k_energy_scatter, dim3(sort_blocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count, cap,
-                         inv_ecut, bins, qa_p.data);
+      hipLaunchKernelGGL(k_energy_scatter, dim3(kSortBlocks), dim3(256), 0, ctx->stream, qa_e.data, qa_e.count,
+                         cap, inv_ecut, hist, qa_p.data);
       HMC_HIP(ctx, hipGetLastError());
       qin_e = Queue{qa_p.data, counts + pa, cap};
       head_e = heads + pa;
