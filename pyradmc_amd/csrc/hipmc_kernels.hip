@@ -2649,19 +2649,54 @@ __global__ void __launch_bounds__(1024) k_energy_scan(unsigned int* hist, unsign
   }
 }
 
+// Per 256 records: the block's bins for them in LDS (rank within the slice's bin by
+// LDS atomics, the slice's per-bin offsets by one wave's scan), the records regrouped
+// by bin in LDS, and then written out by consecutive threads, so that the lanes of a
+// wave write runs of consecutive slots.
 __global__ void __launch_bounds__(256) k_energy_scatter(const Particle* __restrict__ q, const unsigned int* count,
                                                         unsigned int cap, float inv_ecut,
                                                         const unsigned int* __restrict__ first,
                                                         Particle* __restrict__ dst) {
-  __shared__ unsigned int cursor[kEnergyBins];
+  __shared__ unsigned int cursor[kEnergyBins], slice_count[kEnergyBins], slice_off[kEnergyBins];
+  __shared__ Particle stage[256];
+  __shared__ int stage_bin[256];
   if (threadIdx.x < kEnergyBins) cursor[threadIdx.x] = first[threadIdx.x * kSortBlocks + blockIdx.x];
-  __syncthreads();
   unsigned int lo, hi;
   sort_range(min(*count, cap), lo, hi);
-  for (unsigned int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const Particle p = q[i];
-    if ((p.meta & 3u) == kEmptyKind) continue;
-    dst[atomicAdd(&cursor[energy_bin(p.e, inv_ecut)], 1u)] = p;
+  for (unsigned int s0 = lo; s0 < hi; s0 += blockDim.x) {
+    if (threadIdx.x < kEnergyBins) slice_count[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned int i = s0 + threadIdx.x;
+    int bin = -1;
+    Particle p;
+    if (i < hi) {
+      p = q[i];
+      if ((p.meta & 3u) != kEmptyKind) bin = energy_bin(p.e, inv_ecut);
+    }
+    const unsigned int rank = bin >= 0 ? atomicAdd(&slice_count[bin], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the slice's 32 bin counts by the first wave
+      const unsigned int c = threadIdx.x < kEnergyBins ? slice_count[threadIdx.x] : 0u;
+      unsigned int incl = c;
+      for (int off = 1; off < kEnergyBins; off <<= 1) {
+        const unsigned int x = __shfl_up(incl, off);
+        if (static_cast<int>(threadIdx.x) >= off) incl += x;
+      }
+      if (threadIdx.x < kEnergyBins) slice_off[threadIdx.x] = incl - c;
+    }
+    __syncthreads();
+    const unsigned int n_real = slice_off[kEnergyBins - 1] + slice_count[kEnergyBins - 1];
+    if (bin >= 0) {
+      stage[slice_off[bin] + rank] = p;
+      stage_bin[slice_off[bin] + rank] = bin;
+    }
+    __syncthreads();
+    if (threadIdx.x < n_real) {
+      const int b = stage_bin[threadIdx.x];
+      dst[cursor[b] + (threadIdx.x - slice_off[b])] = stage[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x < kEnergyBins) cursor[threadIdx.x] += slice_count[threadIdx.x];
   }
 }
 
