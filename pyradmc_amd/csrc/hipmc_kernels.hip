@@ -2658,7 +2658,7 @@ __global__ void __launch_bounds__(256) k_energy_scatter(const Particle* __restri
                                                         const unsigned int* __restrict__ first,
                                                         Particle* __restrict__ dst) {
   __shared__ unsigned int cursor[kEnergyBins], slice_count[kEnergyBins], slice_off[kEnergyBins];
-  __shared__ Particle stage[256];
+  __shared__ uint4 stage[256][3];  // records as three 16-byte words (x y z e | ux uy uz w | rng meta pad)
   __shared__ int stage_bin[256];
   if (threadIdx.x < kEnergyBins) cursor[threadIdx.x] = first[threadIdx.x * kSortBlocks + blockIdx.x];
   unsigned int lo, hi;
@@ -2668,10 +2668,13 @@ __global__ void __launch_bounds__(256) k_energy_scatter(const Particle* __restri
     __syncthreads();
     const unsigned int i = s0 + threadIdx.x;
     int bin = -1;
-    Particle p;
+    uint4 w0 = make_uint4(0u, 0u, 0u, 0u), w1 = w0, w2 = w0;
     if (i < hi) {
-      p = q[i];
-      if ((p.meta & 3u) != kEmptyKind) bin = energy_bin(p.e, inv_ecut);
+      const uint4* rec = reinterpret_cast<const uint4*>(q + i);
+      w0 = rec[0];
+      w1 = rec[1];
+      w2 = rec[2];
+      if ((w2.z & 3u) != kEmptyKind) bin = energy_bin(__uint_as_float(w0.w), inv_ecut);
     }
     const unsigned int rank = bin >= 0 ? atomicAdd(&slice_count[bin], 1u) : 0u;
     __syncthreads();
@@ -2687,13 +2690,19 @@ __global__ void __launch_bounds__(256) k_energy_scatter(const Particle* __restri
     __syncthreads();
     const unsigned int n_real = slice_off[kEnergyBins - 1] + slice_count[kEnergyBins - 1];
     if (bin >= 0) {
-      stage[slice_off[bin] + rank] = p;
-      stage_bin[slice_off[bin] + rank] = bin;
+      const unsigned int at = slice_off[bin] + rank;
+      stage[at][0] = w0;
+      stage[at][1] = w1;
+      stage[at][2] = w2;
+      stage_bin[at] = bin;
     }
     __syncthreads();
     if (threadIdx.x < n_real) {
       const int b = stage_bin[threadIdx.x];
-      dst[cursor[b] + (threadIdx.x - slice_off[b])] = stage[threadIdx.x];
+      uint4* out = reinterpret_cast<uint4*>(dst + cursor[b] + (threadIdx.x - slice_off[b]));
+      out[0] = stage[threadIdx.x][0];
+      out[1] = stage[threadIdx.x][1];
+      out[2] = stage[threadIdx.x][2];
     }
     __syncthreads();
     if (threadIdx.x < kEnergyBins) cursor[threadIdx.x] += slice_count[threadIdx.x];
