@@ -32,8 +32,9 @@ bounded sample of the same workload (and the quick-start pencil, ``pencil``).
 Further legs, reported beside the throughput line: ``pencil`` (the reference's
 quick-start pencil in both voxel readings), ``dij`` (the batched uneven 100-beamlet
 Dij), ``denoised_dij`` (denoised against plain Monte Carlo at the same error, with the
-denoiser's MFMA roofline) and ``full_plan`` (five gantry beams, 1125 columns, and the
-1000-vector planning product).
+denoiser's MFMA roofline), ``full_plan`` (five gantry beams, 1125 columns, and the
+1000-vector planning product) and ``full_plan_denoised`` (the same comparison as
+``denoised_dij`` on the full plan's gantry-0 beam).
 """
 
 from __future__ import annotations
@@ -254,11 +255,19 @@ def _window_errors(d, ref, source, grid) -> np.ndarray:
     from pyradmc_amd import denoise
 
     spacing = float(grid.spacing[0])
+    shape = tuple(int(n) for n in grid.shape)
     out = np.zeros(source.n_beamlets)
+
+    def window(dij, j, win):
+        x, y, z, dose, _ = denoise._window_entries(dij, j, win, shape)
+        a = np.zeros((denoise.WINDOW_XY, denoise.WINDOW_XY, shape[2]))
+        a[x, y, z] = dose
+        return a
+
     for j in range(source.n_beamlets):
         cx, cy, _ = source.centers[j]
-        wx, wy = denoise.beam_window((int(cx / spacing), int(cy / spacing)), tuple(grid.shape))
-        out[j] = denoise.dose_weighted_rms(d.column_dense(j)[wx, wy, :], ref.column_dense(j)[wx, wy, :])
+        win = denoise.beam_window((int(cx / spacing), int(cy / spacing)), shape)
+        out[j] = denoise.dose_weighted_rms(window(d, j, win), window(ref, j, win))
     return out
 
 
@@ -346,6 +355,67 @@ def _denoise_leg(model_path: Path, low: int, audit: float, target: float, launch
         "denoised_only_error_median": (round(float(np.median(den_err[~report.fallback])), 5)
                                        if (~report.fallback).any() else None),
         "inference_roofline": mfma,
+        "speedup": round(wall_plain / wall_den, 3) if wall_den > 0 else None,
+    }
+
+
+def _plan_denoise_leg(model_path: Path, low: int, audit: float, target: float,
+                      launch: dict | None = None) -> dict:
+    """The full plan's gantry-0 beam (225 beamlets of 5x5 mm along +z on the 256x256x128
+    phantom at 2 mm) at a dose-weighted RMS error target against a 100x-history
+    reference, plain Monte Carlo versus low-history Monte Carlo plus the trained
+    denoiser, as the 100-beamlet leg does it. It is the plan's one beam whose beamlets
+    run along a grid axis: the denoiser's windows are grid-aligned (64x64 voxels across,
+    the grid's whole depth along the beam), so the four rotated beams have none and
+    stay plain Monte Carlo. The network was trained on 2.5 mm phantoms 96 voxels deep;
+    here it sees 2 mm voxels 128 deep, and the plain-MC audit judges it on this plan."""
+    from pyradmc_amd import denoise, workloads
+
+    grid, xs, beams, kw = workloads.full_plan()
+    src = beams[0]
+    nb, trunc = kw["n_batches"], kw["truncation"]
+    engine = _shaped_engine(grid, xs, launch)
+    model = denoise.load(model_path)
+    engine.run_dij(src, 10 * nb, nb, seed=1, truncation=trunc)  # warm
+    ref = engine.run_dij(src, 100 * low, nb, seed=7, truncation=trunc)
+    probe_n = 100_000
+    probe = engine.run_dij(src, probe_n, nb, seed=8, truncation=trunc)
+    eps = float(np.percentile(_window_errors(probe, ref, src, grid), 90))
+    del probe
+    n_plain = int(math.ceil(probe_n * (eps / target) ** 2 / nb)) * nb
+    started = time.perf_counter()
+    plain = engine.run_dij(src, n_plain, nb, seed=9, truncation=trunc, value_dtype=np.float32)
+    wall_plain = time.perf_counter() - started
+    plain_err = _window_errors(plain, ref, src, grid)
+    del plain
+    builder = denoise.DenoisedDijBuilder(engine, model, low_histories=low, audit_fraction=audit,
+                                         tolerance=target, fallback_histories=n_plain, n_batches=nb)
+    builder.build(src, seed=11, value_dtype=np.float32)  # warm (kernels, allocator)
+    started = time.perf_counter()
+    den, report = builder.build(src, seed=12, truncation=trunc, value_dtype=np.float32)
+    wall_den = time.perf_counter() - started
+    den_err = _window_errors(den, ref, src, grid)
+    engine.close()
+    kept = ~report.fallback
+    return {
+        "workload": "full plan's gantry-0 beam: CT-like 256x256x128 @ 2 mm, 15x15 beamlets of 5x5 mm along +z "
+                    f"from 100 cm SAD (225 columns); reference {100 * low} histories per beamlet; the four "
+                    "rotated beams have no grid-aligned denoiser window",
+        "target_error": target,
+        "plain_histories_per_beamlet": n_plain,
+        "plain_wall_s": round(wall_plain, 4),
+        "plain_error_median": round(float(np.median(plain_err)), 5),
+        "plain_error_p90": round(float(np.percentile(plain_err, 90)), 5),
+        "denoised_low_histories_per_beamlet": low,
+        "denoised_audit_fraction": audit,
+        "denoised_wall_s": round(wall_den, 4),
+        "denoised_seconds": {k: round(v, 4) for k, v in report.seconds.items()},
+        "denoised_fallback_columns": int(report.fallback.sum()),
+        "denoised_error_median": round(float(np.median(den_err)), 5),
+        "denoised_error_p90": round(float(np.percentile(den_err, 90)), 5),
+        "audit_columns": int(report.audited.sum()),
+        "audit_error": round(float(report.audit_error), 5),
+        "denoised_only_error_median": round(float(np.median(den_err[kept])), 5) if kept.any() else None,
         "speedup": round(wall_plain / wall_den, 3) if wall_den > 0 else None,
     }
 
@@ -609,6 +679,10 @@ def main() -> int:
     ap.add_argument("--denoise-audit", type=float, default=0.1,
                     help="fraction of the denoised build's columns re-simulated with plain Monte Carlo as its judge")
     ap.add_argument("--no-plan", action="store_true", help="skip the full-plan Dij and product leg")
+    ap.add_argument("--no-plan-denoise", action="store_true",
+                    help="skip the denoised-versus-plain comparison on the full plan's gantry-0 beam")
+    ap.add_argument("--plan-denoise-only", action="store_true",
+                    help="run only that comparison and print its JSON")
     ap.add_argument("--plan-probe", type=int, default=20_000,
                     help="probe histories per beamlet of the full-plan leg")
     ap.add_argument("--plan-vectors", type=int, default=1000, help="weight vectors of the planning product")
@@ -647,6 +721,13 @@ def main() -> int:
         del a, b
 
     denoise_leg = not args.no_denoise and Path(args.denoise_model).is_file()
+    if args.plan_denoise_only:  # the engine's default launch shape
+        print(json.dumps({"full_plan_denoised": _plan_denoise_leg(Path(args.denoise_model), args.denoise_low,
+                                                                  args.denoise_audit, 0.02)}), flush=True)
+        from pyradmc_amd import _native
+
+        _native.shutdown()
+        return 0
     if denoise_leg:
         # The denoiser runs on torch's HIP runtime: load it before any engine exists.
         print("bench: loading the denoiser (PyTorch)", file=sys.stderr, flush=True)
@@ -725,6 +806,9 @@ def main() -> int:
     denoised = (_guarded("denoised_dij", _denoise_leg, Path(args.denoise_model), args.denoise_low,
                          args.denoise_audit, 0.02, dij_shape)
                 if denoise_leg else None)
+    plan_denoised = (_guarded("full_plan_denoised", _plan_denoise_leg, Path(args.denoise_model), args.denoise_low,
+                              args.denoise_audit, 0.02, dij_shape)
+                     if denoise_leg and not args.no_plan_denoise else None)
     plan, plan_dij = None, None
     if not args.no_plan:
         got = _guarded("full_plan", _plan_leg, 0.02, args.plan_probe, args.plan_vectors, args.plan_cap, shape)
@@ -772,6 +856,7 @@ def main() -> int:
         "dij": dij,
         "denoised_dij": denoised,
         "full_plan": plan,
+        "full_plan_denoised": plan_denoised,
         "head": head,
         "events_per_history": _events(counters, total),
     }
