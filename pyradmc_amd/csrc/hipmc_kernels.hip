@@ -1227,17 +1227,22 @@ __global__ void __launch_bounds__(256) k_fold(long long* edep, const double* mas
   long long local = 0;
   for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const long long col = i / n_vox;
-    const long long v = i - col * n_vox;
-    const double m = mass[v];
-    double a = s1[i];
-    double b = s2[i];
+    long long col = 0;
+    double m = 0.0, a = 0.0, b = 0.0;
     bool touched = false;
     for (int slot = 0; slot < n_slots; ++slot) {  // batch order
       const long long j = slot * slot_stride + i;
       const long long q = edep[j];
       if (q == 0) continue;
       edep[j] = 0;
+      if (!touched) {
+        // Most cells of a beamlet column see no deposit in a fold: their column, mass
+        // and sums are looked up only once a slot holds one.
+        col = i / n_vox;
+        m = mass[i - col * n_vox];
+        a = s1[i];
+        b = s2[i];
+      }
       double d = 0.0;
       if (m > 0.0) {
         const double energy = static_cast<double>(q) * 1.0e-9;
