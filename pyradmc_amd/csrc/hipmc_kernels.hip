@@ -1218,6 +1218,8 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 // pyradmc/scoring/dose.py:169-207).
 #pragma clang fp contract(off)
 
+constexpr int kFoldSlots = 4;
+
 __global__ void __launch_bounds__(256) k_fold(long long* edep, const double* mass,
                                               const double* nhist, long long n_vox,
                                               long long n_cols, int n_slots, double* s1,
@@ -1230,28 +1232,37 @@ __global__ void __launch_bounds__(256) k_fold(long long* edep, const double* mas
     long long col = 0;
     double m = 0.0, a = 0.0, b = 0.0;
     bool touched = false;
-    for (int slot = 0; slot < n_slots; ++slot) {  // batch order
-      const long long j = slot * slot_stride + i;
-      const long long q = edep[j];
-      if (q == 0) continue;
-      edep[j] = 0;
-      if (!touched) {
-        // Most cells of a beamlet column see no deposit in a fold: their column, mass
-        // and sums are looked up only once a slot holds one.
-        col = i / n_vox;
-        m = mass[i - col * n_vox];
-        a = s1[i];
-        b = s2[i];
+    for (int s0 = 0; s0 < n_slots; s0 += kFoldSlots) {
+      // The next kFoldSlots slots' tallies are loaded together (more bytes in flight
+      // per wave), then taken in batch order.
+      long long qs[kFoldSlots];
+#pragma unroll
+      for (int k = 0; k < kFoldSlots; ++k)
+        qs[k] = s0 + k < n_slots ? edep[(s0 + k) * slot_stride + i] : 0;
+#pragma unroll
+      for (int k = 0; k < kFoldSlots; ++k) {
+        const long long q = qs[k];
+        if (q == 0) continue;
+        const int slot = s0 + k;
+        edep[slot * slot_stride + i] = 0;
+        if (!touched) {
+          // Most cells of a beamlet column see no deposit in a fold: their column,
+          // mass and sums are looked up only once a slot holds one.
+          col = i / n_vox;
+          m = mass[i - col * n_vox];
+          a = s1[i];
+          b = s2[i];
+        }
+        double d = 0.0;
+        if (m > 0.0) {
+          const double energy = static_cast<double>(q) * 1.0e-9;
+          d = energy / (m * nhist[slot * n_cols + col]);
+        }
+        a = a + d;
+        b = b + d * d;
+        touched = true;
+        local += q;
       }
-      double d = 0.0;
-      if (m > 0.0) {
-        const double energy = static_cast<double>(q) * 1.0e-9;
-        d = energy / (m * nhist[slot * n_cols + col]);
-      }
-      a = a + d;
-      b = b + d * d;
-      touched = true;
-      local += q;
     }
     if (touched) {
       s1[i] = a;
