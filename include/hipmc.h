@@ -243,8 +243,15 @@ int hmc_transport(hmc_ctx *ctx, uint64_t seed, const hmc_work_item *items, int n
 /* Fold batch slots 0..n_slots_used-1, in slot order, into the running f64 sums and
  * consume the maps: accumulate_run_batch / accumulate_dij_batch (kernels.py:804-878).
  * nhist[slot * n_columns + column] = histories of that column in that batch.
- * total_quanta receives the folded quanta. */
+ * total_quanta receives the folded quanta after the fold has run on the device. With
+ * total_quanta NULL the call returns once the fold is queued, and the quanta of such
+ * folds add up until hmc_finalize reads them back (hmc_folded_quanta). */
 int hmc_fold(hmc_ctx *ctx, int n_slots_used, const double *nhist, int64_t *total_quanta);
+/* The quanta folded by hmc_fold calls without total_quanta since the previous call of
+ * this function: read back by the hmc_finalize after them (or here, waiting for the
+ * device, if none came). It counts the energy the reference's accumulate_*_batch add
+ * to the deposited total (kernels.py:804-878); resets the count to zero. */
+int hmc_folded_quanta(hmc_ctx *ctx, int64_t *total_quanta);
 /* Mean and sigma per (column, voxel): finalize_run (kernels.py:881-909). With
  * mean/sigma NULL the result stays on the device for hmc_truncate. */
 int hmc_finalize(hmc_ctx *ctx, int n_batches, double *mean, double *sigma);

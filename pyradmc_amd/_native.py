@@ -251,6 +251,7 @@ def lib():
         "hmc_set_launch": (c_int, [ctx_p, c_int, c_int, c_int, c_int]),
         "hmc_transport": (c_int, [ctx_p, c_uint64, POINTER(WorkItem), c_int, c_int64, c_int, POINTER(Ledger)]),
         "hmc_fold": (c_int, [ctx_p, c_int, _f64p, POINTER(c_int64)]),
+        "hmc_folded_quanta": (c_int, [ctx_p, POINTER(c_int64)]),
         "hmc_finalize": (c_int, [ctx_p, c_int, _f64p, _f64p]),
         "hmc_truncate": (c_int, [ctx_p, c_double, POINTER(c_int64), POINTER(c_int64)]),
         "hmc_fetch_csc": (c_int, [ctx_p, POINTER(c_int64), _f64p, _f64p]),

@@ -1440,6 +1440,11 @@ struct hmc_ctx {
   int* d_origins = nullptr;  // window origins for hmc_gather_windows / hmc_scatter_windows
   bool has_scoring = false;
   bool finalized = false;
+  // Folds made without a host wait (hmc_fold with total_quanta NULL) add their quanta to
+  // the device counter past the ledger; the next hmc_finalize reads it back with the
+  // means, and hmc_folded_quanta hands it out.
+  bool fold_pending = false;
+  long long folded_quanta = 0;
 
   // tables
   DevTables t{};
@@ -1816,6 +1821,8 @@ int hmc_reset_maps(hmc_ctx* ctx) {
   HMC_HIP(ctx, hipGetLastError());  // stream-ordered before the next transport: no host wait
   ctx->dirty_slots = 0;
   ctx->finalized = false;
+  ctx->fold_pending = false;
+  ctx->folded_quanta = 0;
   return HMC_OK;
 }
 
@@ -3036,18 +3043,39 @@ int hmc_fold(hmc_ctx* ctx, int n_slots_used, const double* nhist, int64_t* total
   HMC_HIP(ctx, hipMemcpyAsync(ctx->d_nhist, nhist, sizeof(double) * n_nh, hipMemcpyHostToDevice,
                               ctx->stream));
   unsigned long long* folded = ctx->d_ledger + L_COUNT;  // scratch slot past the ledger
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, folded, 1ll, 0ull);
+  // The counter starts from zero unless earlier folds' quanta are still to be read.
+  if (!ctx->fold_pending)
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, folded, 1ll, 0ull);
   const long long total = ctx->s.n_voxels * ctx->n_columns;
   hipLaunchKernelGGL(k_fold, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream,
                      ctx->d_edep, ctx->d_mass, ctx->d_nhist, ctx->s.n_voxels, ctx->n_columns,
                      n_slots_used, ctx->d_s1, ctx->d_s2, folded);
   HMC_HIP(ctx, hipGetLastError());
+  if (n_slots_used >= ctx->dirty_slots) ctx->dirty_slots = 0;  // the fold zeroes what it reads
+  if (!total_quanta) {  // no wait: the total comes back with the next hmc_finalize
+    ctx->fold_pending = true;
+    return HMC_OK;
+  }
   HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageFolded, folded, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const unsigned long long q = ctx->h_stage[kStageFolded];
-  if (n_slots_used >= ctx->dirty_slots) ctx->dirty_slots = 0;  // the fold zeroed what it read
-  if (total_quanta) *total_quanta = static_cast<int64_t>(q);
+  ctx->fold_pending = false;
+  *total_quanta = static_cast<int64_t>(ctx->h_stage[kStageFolded]);
+  return HMC_OK;
+}
+
+int hmc_folded_quanta(hmc_ctx* ctx, int64_t* total_quanta) {
+  if (!ctx || !total_quanta) return fail(ctx, HMC_EINVAL, "null");
+  if (ctx->fold_pending) {  // no finalize since those folds: read the counter now
+    HMC_HIP(ctx, hipSetDevice(ctx->device));
+    HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageFolded, ctx->d_ledger + L_COUNT, sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->folded_quanta = static_cast<long long>(ctx->h_stage[kStageFolded]);
+    ctx->fold_pending = false;
+  }
+  *total_quanta = static_cast<int64_t>(ctx->folded_quanta);
+  ctx->folded_quanta = 0;
   return HMC_OK;
 }
 
@@ -3068,7 +3096,15 @@ int hmc_finalize(hmc_ctx* ctx, int n_batches, double* mean, double* sigma) {
   if (sigma)
     HMC_HIP(ctx, hipMemcpyAsync(sigma, ctx->d_s2, sizeof(double) * total, hipMemcpyDeviceToHost,
                                 ctx->stream));
+  const bool take_folded = ctx->fold_pending;
+  if (take_folded)
+    HMC_HIP(ctx, hipMemcpyAsync(ctx->h_stage + kStageFolded, ctx->d_ledger + L_COUNT, sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, ctx->stream));
   HMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (take_folded) {
+    ctx->folded_quanta = static_cast<long long>(ctx->h_stage[kStageFolded]);
+    ctx->fold_pending = false;
+  }
   return HMC_OK;
 }
 

@@ -594,11 +594,17 @@ class HipEngine:
         )
         return ledger.as_dict()
 
-    def _fold(self, n_slots: int, nhist: np.ndarray) -> int:
-        """Fold slots 0..n_slots-1 in order; nhist is [slot][column]."""
-        total = nat.c_int64()
+    def _fold(self, n_slots: int, nhist: np.ndarray) -> None:
+        """Fold slots 0..n_slots-1 in order; nhist is [slot][column]. Returns once the
+        fold is queued: the folded quanta come back with the next finalize
+        (``_folded_quanta``)."""
         nh = np.ascontiguousarray(np.asarray(nhist, dtype=np.float64).reshape(-1))
-        self._ctx.call("hmc_fold", int(n_slots), nat.ptr(nh, nat.c_double), nat.ctypes.byref(total))
+        self._ctx.call("hmc_fold", int(n_slots), nat.ptr(nh, nat.c_double), None)
+
+    def _folded_quanta(self) -> int:
+        """Quanta folded since the last call (read back by the finalize after them)."""
+        total = nat.c_int64()
+        self._ctx.call("hmc_folded_quanta", nat.ctypes.byref(total))
         return int(total.value)
 
     @staticmethod
@@ -694,7 +700,7 @@ class HipEngine:
         chunk = self._chunk(per_batch * slots)
         emitter = ProgressEmitter(progress, n_histories)
         books: dict = {}
-        deposited_quanta = 0
+        self._folded_quanta()  # drop any count an interrupted earlier run left behind
         for b0 in range(0, n_batches, slots):
             block = range(b0, min(b0 + slots, n_batches))
             items = [(b * per_batch, per_batch, b - b0, 0) for b in block]
@@ -703,13 +709,14 @@ class HipEngine:
                                            books)
             else:
                 self._merge(books, self._transport(seed, items, chunk))
-            deposited_quanta += self._fold(len(block), np.full(len(block), float(per_batch)))
+            self._fold(len(block), np.full(len(block), float(per_batch)))
             for _ in block:
                 emitter.tick(per_batch)
         n_vox = scoring.n_voxels
         mean = host_array(n_vox, np.float64)
         sigma = host_array(n_vox, np.float64)
         self._ctx.call("hmc_finalize", int(n_batches), nat.ptr(mean, nat.c_double), nat.ptr(sigma, nat.c_double))
+        deposited_quanta = self._folded_quanta()
 
         if analytic:
             particle_rest = 0.0
@@ -760,7 +767,7 @@ class HipEngine:
         slots = self._batch_slots(n_batches, n_cols, scoring.n_voxels)
         self._set_scoring(scoring, n_cols, slots)
         chunk = self._chunk(int(per_batch[g0:group.stop].sum()) * slots)
-        deposited = 0
+        self._folded_quanta()  # drop any count an interrupted earlier call left behind
         for b0 in range(0, n_batches, slots):
             block = range(b0, min(b0 + slots, n_batches))
             items = []
@@ -776,9 +783,9 @@ class HipEngine:
             else:
                 self._merge(books, self._transport(seed, items, chunk))
             nhist = np.tile(per_batch[g0:group.stop].astype(np.float64), len(block))
-            deposited += self._fold(len(block), nhist)
+            self._fold(len(block), nhist)
         self._ctx.call("hmc_finalize", int(n_batches), None, None)
-        return analytic, mono_energy, deposited
+        return analytic, mono_energy, self._folded_quanta()
 
     def run_dij(
         self,

@@ -98,6 +98,60 @@ def test_fold_finalize_truncate_bitexact_vs_oracle(hip):
     ctx.close()
 
 
+def test_folds_without_wait_total_comes_back_with_finalize(hip):
+    # Folds queued without a host wait: the same bits as the waiting fold, and their
+    # quanta summed over the folds, read back by finalize (or by the query itself when
+    # no finalize came); a map reset drops a count nobody read.
+    from oracle import mc
+
+    rng = np.random.default_rng(13)
+    shape, cols, n_batches = (4, 6, 5), 2, 4
+    nv = int(np.prod(shape))
+    mass = rng.uniform(0.1, 2.0, size=nv)
+    mass[::7] = 0.0
+    ctx = hip.Context(0)
+    ctx.call("hmc_set_scoring", *shape, hip.ptr(np.zeros(3), ctypes.c_double),
+             hip.ptr(np.ones(3), ctypes.c_double), hip.ptr(mass, ctypes.c_double), cols, 1)
+    nhist = np.array([700.0, 90.0])
+    s1 = np.zeros((cols, nv))
+    s2 = np.zeros((cols, nv))
+    expected = 0
+    for _ in range(n_batches):
+        q = _synthetic(nv, cols, rng)
+        expected += mc.fold_quanta(q, mass, nhist, s1, s2)
+        ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+        ctx.call("hmc_fold", 1, hip.ptr(nhist, ctypes.c_double), None)
+    mean = np.empty(cols * nv)
+    sigma = np.empty(cols * nv)
+    ctx.call("hmc_finalize", n_batches, hip.ptr(mean, ctypes.c_double), hip.ptr(sigma, ctypes.c_double))
+    total = ctypes.c_int64()
+    ctx.call("hmc_folded_quanta", ctypes.byref(total))
+    assert total.value == expected
+    ctx.call("hmc_folded_quanta", ctypes.byref(total))
+    assert total.value == 0  # handed out once
+    m_ref, s_ref = mc.finalize(s1, s2, n_batches)
+    np.testing.assert_array_equal(mean.reshape(cols, nv), m_ref)
+    np.testing.assert_array_equal(sigma.reshape(cols, nv), s_ref)
+
+    # no finalize between: the query waits and reads the counter itself
+    ctx.call("hmc_reset_maps")
+    q = _synthetic(nv, cols, rng)
+    want = mc.fold_quanta(q, mass, nhist, np.zeros((cols, nv)), np.zeros((cols, nv)))
+    ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+    ctx.call("hmc_fold", 1, hip.ptr(nhist, ctypes.c_double), None)
+    ctx.call("hmc_folded_quanta", ctypes.byref(total))
+    assert total.value == want
+
+    # a count left unread is dropped by the next reset
+    ctx.call("hmc_reset_maps")
+    ctx.call("hmc_set_quanta", hip.ptr(q, ctypes.c_int64))
+    ctx.call("hmc_fold", 1, hip.ptr(nhist, ctypes.c_double), None)
+    ctx.call("hmc_reset_maps")
+    ctx.call("hmc_folded_quanta", ctypes.byref(total))
+    assert total.value == 0
+    ctx.close()
+
+
 def test_slot_fold_equals_sequential_batches(hip):
     # Several batches transported side by side (one map slot each) and folded in slot
     # order reproduce the one-batch-at-a-time fold bit for bit.
