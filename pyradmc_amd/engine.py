@@ -663,7 +663,12 @@ class HipEngine:
         progress=None,
         concurrent_batches: int = 1,
     ) -> TransportResult:
-        """Transport ``n_histories`` primaries in ``n_batches`` equal batches."""
+        """Transport ``n_histories`` primaries in ``n_batches`` equal batches.
+
+        ``concurrent_batches`` keeps the reference's signature: this engine already
+        transports as many batches side by side as the tally memory allows (one map slot
+        each, folded in batch order), so it is validated and otherwise has no effect on
+        the schedule or the results."""
         if n_histories < 1:
             raise ValueError(f"need at least one history, got {n_histories}")
         if concurrent_batches < 1:
