@@ -237,16 +237,28 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
 
   constexpr int kKC = SRC == kSrcLow8 ? 8 : CIN;  // channels read per tap
   constexpr int K = 27 * kKC;
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  // Weight fragments of k-step k0 (zero past K): global loads, issued one k-step ahead
+  // of their MFMAs so that their latency overlaps the previous step's work.
+  auto load_a = [&](int k0, bf16x8 (&a)[TM]) {
     const int k = k0 + 8 * kg;
-    bf16x8 a[TM], b[4];
+    if (k < K) {
+      const int tap = k / kKC;
+      const int c = k - tap * kKC;
+#pragma unroll
+      for (int m = 0; m < TM; ++m)
+        a[m] = load8(wt + (static_cast<long long>(tap) * cout + 16 * m + col) * CIN + c);
+    } else {
+#pragma unroll
+      for (int m = 0; m < TM; ++m) a[m] = zero8();
+    }
+  };
+  // Input fragments of k-step k0 from the LDS tile (zero past K).
+  auto load_b = [&](int k0, bf16x8 (&b)[4]) {
+    const int k = k0 + 8 * kg;
     if (k < K) {
       const int tap = k / kKC;
       const int c = k - tap * kKC;
       const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // halo-shifted: 0..2
-#pragma unroll
-      for (int m = 0; m < TM; ++m)
-        a[m] = load8(wt + (static_cast<long long>(tap) * cout + 16 * m + col) * CIN + c);
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
         const u32x4 v = tile[((dz * kTH + blk_row[nb] + dy) * kTW + dx + blk_x[nb] + col) * kC8 + c / 8];
@@ -254,14 +266,28 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
       }
     } else {
 #pragma unroll
-      for (int m = 0; m < TM; ++m) a[m] = zero8();
-#pragma unroll
       for (int nb = 0; nb < 4; ++nb) b[nb] = zero8();
     }
+  };
+  auto mfmas = [&](const bf16x8 (&a)[TM], const bf16x8 (&b)[4]) {
 #pragma unroll
     for (int m = 0; m < TM; ++m)
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) acc[m][nb] = mfma16(a[m], b[nb], acc[m][nb]);
+  };
+  // k-steps in order, two per trip: the next step's fragments load into the other
+  // register set while this step's MFMAs run (ping-pong, no copies).
+  bf16x8 a0[TM], a1[TM], b0[4], b1[4];
+  load_a(0, a0);
+  load_b(0, b0);
+  for (int k0 = 0; k0 < K; k0 += 64) {
+    load_a(k0 + 32, a1);
+    load_b(k0 + 32, b1);
+    mfmas(a0, b0);
+    if (k0 + 32 >= K) break;
+    load_a(k0 + 64, a0);
+    load_b(k0 + 64, b0);
+    mfmas(a1, b1);
   }
 
   const long long P = static_cast<long long>(n) * d * h * kW;
