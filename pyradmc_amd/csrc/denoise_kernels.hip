@@ -191,30 +191,45 @@ __global__ void __launch_bounds__(256) k_conv_fwd_tile(const __bf16* __restrict_
   const int z = static_cast<int>(t % d);
   const int nn = static_cast<int>(t / d);
   const int h0 = kRows * hb;
-  for (int i = threadIdx.x; i < kTZ * kTH * kTW * kC8; i += 256) {
-    const int c8 = i % kC8;
-    const int pos = i / kC8;
-    const int xx = pos % kTW, yy = (pos / kTW) % kTH, zz = pos / (kTW * kTH);
-    const int gz = z + zz - 1, gy = h0 + yy - 1, gx = xx - 1;
-    u32x4 v = u32x4{0u, 0u, 0u, 0u};
-    if (gz >= 0 && gz < d && gy >= 0 && gy < h && gx >= 0 && gx < kW) {
-      if constexpr (SRC == kSrcUp) {
-        const long long q =
-            ((static_cast<long long>(nn) * (d >> 1) + (gz >> 1)) * (h >> 1) + (gy >> 1)) * (kW >> 1) + (gx >> 1);
-        v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
-      } else {
-        const long long q = ((static_cast<long long>(nn) * d + gz) * h + gy) * kW + gx;
-        v = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
-        if constexpr (SRC == kSrcAdd) {
-          bf16x8 a8, b8;
-          std::memcpy(&a8, &v, 16);
-          b8 = load8(x2 + q * CIN + 8 * c8);
-          a8 = add8(a8, b8);
-          std::memcpy(&v, &a8, 16);
-        }
-      }
+  // The tile fill: each thread issues kFill loads before it stores any, so their
+  // latencies overlap. A load whose position lies outside the volume (or past the tile)
+  // reads the tensor's first element instead, always valid, and is replaced by zero.
+  constexpr int kUnits = kTZ * kTH * kTW * kC8;
+  constexpr int kFill = 8;
+  for (int i0 = threadIdx.x; i0 < kUnits; i0 += 256 * kFill) {
+    u32x4 v[kFill], v2[kFill];
+    bool ok[kFill];
+#pragma unroll
+    for (int j = 0; j < kFill; ++j) {
+      const int i = i0 + 256 * j;
+      const int c8 = i % kC8;
+      const int pos = i / kC8;
+      const int xx = pos % kTW, yy = (pos / kTW) % kTH, zz = pos / (kTW * kTH);
+      const int gz = z + zz - 1, gy = h0 + yy - 1, gx = xx - 1;
+      ok[j] = i < kUnits && gz >= 0 && gz < d && gy >= 0 && gy < h && gx >= 0 && gx < kW;
+      long long q;
+      if constexpr (SRC == kSrcUp)
+        q = ((static_cast<long long>(nn) * (d >> 1) + (gz >> 1)) * (h >> 1) + (gy >> 1)) * (kW >> 1) + (gx >> 1);
+      else
+        q = ((static_cast<long long>(nn) * d + gz) * h + gy) * kW + gx;
+      q = ok[j] ? q : 0;
+      v[j] = *reinterpret_cast<const u32x4*>(x + q * CIN + 8 * c8);
+      if constexpr (SRC == kSrcAdd) v2[j] = *reinterpret_cast<const u32x4*>(x2 + q * CIN + 8 * c8);
     }
-    tile[i] = v;
+#pragma unroll
+    for (int j = 0; j < kFill; ++j) {
+      const int i = i0 + 256 * j;
+      u32x4 val = v[j];
+      if constexpr (SRC == kSrcAdd) {
+        bf16x8 a8, b8;
+        std::memcpy(&a8, &v[j], 16);
+        std::memcpy(&b8, &v2[j], 16);
+        a8 = add8(a8, b8);
+        std::memcpy(&val, &a8, 16);
+      }
+      if (!ok[j]) val = u32x4{0u, 0u, 0u, 0u};
+      if (i < kUnits) tile[i] = val;
+    }
   }
   __syncthreads();
 
