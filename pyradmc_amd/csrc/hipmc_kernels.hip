@@ -1027,6 +1027,23 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
   __shared__ unsigned int wave_count[2][4];
   __shared__ unsigned int block_base[2];
   __shared__ long long red[4];
+  // The work items and the spectrum's cdf and bin edges, when small enough, are read
+  // from LDS: each primary's two binary searches are chains of dependent loads.
+  constexpr int kLdsItems = 512, kLdsBins = 256;
+  __shared__ DevItem s_items[kLdsItems];
+  __shared__ float s_cdf[kLdsBins], s_edges[kLdsBins + 1];
+  const bool items_in_lds = n_items <= kLdsItems;
+  if (items_in_lds)
+    for (int k = static_cast<int>(threadIdx.x); k < n_items; k += static_cast<int>(blockDim.x)) s_items[k] = items[k];
+  const DevItem* item_table = items_in_lds ? s_items : items;
+  DevSource Sl = S;
+  if (S.cdf && S.edges && S.n_bins > 0 && S.n_bins <= kLdsBins) {
+    for (int k = static_cast<int>(threadIdx.x); k < S.n_bins; k += static_cast<int>(blockDim.x)) s_cdf[k] = S.cdf[k];
+    for (int k = static_cast<int>(threadIdx.x); k <= S.n_bins; k += static_cast<int>(blockDim.x)) s_edges[k] = S.edges[k];
+    Sl.cdf = s_cdf;
+    Sl.edges = s_edges;
+  }
+  __syncthreads();
   const int wave = static_cast<int>(threadIdx.x >> 6);
   const int lane = static_cast<int>(__lane_id());
   const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
@@ -1044,14 +1061,14 @@ __global__ void __launch_bounds__(256) k_generate(DevSource S, DevRayWeights RW,
       int lo = 0, hi = n_items;  // last item with start <= g
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (items[mid].start <= g) lo = mid;
+        if (item_table[mid].start <= g) lo = mid;
         else hi = mid;
       }
-      const DevItem item = items[lo];
+      const DevItem item = item_table[lo];
       const long long i = g - item.start;
       const uint32_t column = static_cast<uint32_t>(item.column);
       uint64_t rng = rng_init(seed, static_cast<uint64_t>(item.key_base + i));
-      Primary q = make_primary(S, item.beamlet, g, rng);
+      Primary q = make_primary(Sl, item.beamlet, g, rng);
       float e = q.e, w = q.w, x = q.x, y = q.y, z = q.z, ux = q.ux, uy = q.uy, uz = q.uz;
       bool book = q.book;
       kind = q.kind;
